@@ -1,0 +1,228 @@
+"""Benchmark: K3 TFLOP/s at 2592 DoF (BF16x3 and FP32) and IFC3 assembly GB/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 6] [--no-cpu-baseline]
+
+Workload: 12-6 LJ argon (sigma 3.4 A, epsilon 0.24037 kcal/mol, r_cut 8.5 A, 39.95 amu)
+on FCC 6x6x6 (864 atoms, 2592 DoF) with seeded Gaussian displacements sigma_u = 0.05 A.
+One step of each phase is one pass of the hot path over the whole system:
+  assembly:  fused third_order (FP64 Psi, 139 GB) + mass-weighted FP32 Psi (70 GB)
+  mcc3:      K3 = Psi x1 phi x2 phi x3 phi (6 n^4 flop) from the FP32 Psi, phi from the
+             host FP64 eigendecomposition of the dynamical matrix
+Inputs are resident in HBM when the timed region starts.  The JSON line's `value`
+is the BF16x3 K3 throughput, 6 n^4 / (time per mcc3).  Errors are relative
+Frobenius errors of sampled K3 slabs against an FP64 host contraction of the same
+FP32 Psi and phi.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "forceconstants.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "K3 TFLOP/s at 2592 DoF, bf16x3 vs fp32 @≤1e-5 rel err; IFC3 assembly GB/s"
+PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "hbm": 8.0e12}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def profile(ctx, kind):
+    import ctypes
+    t = ctypes.c_double()
+    c = ctypes.c_int()
+    from fc3._lib import check
+    check(ctx.lib.fc3_profile_query(ctx.h, kind, ctypes.byref(t), ctypes.byref(c)))
+    return float(t.value), int(c.value)
+
+
+def profile_enable(ctx, on=True):
+    from fc3._lib import check
+    check(ctx.lib.fc3_profile_enable(ctx.h, 1 if on else 0))
+
+
+def cpu_baseline_mcc3(n=1024, seed=0):
+    """Host FP32 three-GEMM contraction (numpy/OpenBLAS on the host cores): the
+    reference has no CPU mcc3, so this restates its three mode products on the CPU."""
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except Exception:
+        threads = os.cpu_count() or 1
+    rng = np.random.default_rng(seed)
+    psi = rng.standard_normal((n, n, n), dtype=np.float32)
+    phi = rng.standard_normal((n, n), dtype=np.float32)
+    t0 = time.perf_counter()
+    k1 = (phi.T @ psi.reshape(n, n * n)).reshape(n, n, n)           # mode 1
+    k2 = np.matmul(k1.transpose(2, 0, 1), phi).transpose(1, 2, 0)  # mode 2 (per slab)
+    k3 = (np.ascontiguousarray(k2).reshape(n * n, n) @ phi)         # mode 3
+    dt = time.perf_counter() - t0
+    del k1, k2, k3, psi
+    return {"value": 6.0 * n ** 4 / dt / 1e12, "unit": "TFLOP/s", "cores": int(threads),
+            "kind": "port", "sample": f"FP32 mcc3 of a random {n}^3 tensor (numpy, 3 GEMMs), "
+            f"{dt:.1f} s"}
+
+
+def cpu_baseline_assembly():
+    """The oracle's restatement of the reference loop (dense FP64, ASR!, apply_tols!)
+    on FCC 4x4x4 (768 DoF), one host core."""
+    from fc3 import systems as S
+    from oracle import fc3_oracle as O
+    sysd = S.fcc(S.ARGON_A, (4, 4, 4), S.ARGON_MASS, 0.05, seed=768)
+    calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
+    t0 = time.perf_counter()
+    psi = O.lj_third_order(sysd, S.ARGON_LJ, calc)
+    dt = time.perf_counter() - t0
+    n = sysd.n_dof
+    del psi
+    return {"value": 8.0 * n ** 3 / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"LJ FCC 4x4x4 ({n} DoF) third_order, dense FP64, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp32-steps", type=int, default=1)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {a.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import fc3
+    from fc3 import systems as S
+    from oracle import fc3_oracle as O
+
+    ctx = fc3.Context(0)
+    cells = (a.cells,) * 3
+    sysd = S.fcc(S.ARGON_A, cells, S.ARGON_MASS, 0.05, seed=2592)
+    calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
+    n = sysd.n_dof
+    log(f"workload FCC {cells} n={n}")
+
+    # phi: IFC2 on the GPU, FP64 eigendecomposition of the dynamical matrix on the host
+    dyn = fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx)
+    _, phi = fc3.get_modes(dyn)
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    dphi = fc3.DeviceArray.from_host(phi32, ctx=ctx)
+    log("phi ready")
+
+    # ---------------- assembly ----------------
+    d64 = fc3.DeviceArray((n, n, n), np.float64, ctx)
+    d32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    for _ in range(a.warmup):
+        fc3.third_order_device(sysd, S.ARGON_LJ, calc, psi64=d64, psi32=d32,
+                               masses=sysd.masses, ctx=ctx)
+    ctx.synchronize()
+    profile_enable(ctx)
+    ctx.timer_start()
+    for _ in range(a.steps):
+        fc3.third_order_device(sysd, S.ARGON_LJ, calc, psi64=d64, psi32=d32,
+                               masses=sysd.masses, ctx=ctx)
+    t_asm = ctx.timer_stop() / a.steps
+    w_ms, w_cnt = profile(ctx, 0)
+    profile_enable(ctx, False)
+    asm_bytes = 12.0 * n ** 3
+    asm = {"ms_per_step": t_asm, "GB_per_s": asm_bytes / (t_asm * 1e-3) / 1e9,
+           "writer_kernel_ms": w_ms / max(w_cnt, 1),
+           "writer_GB_per_s": asm_bytes / (w_ms / max(w_cnt, 1) * 1e-3) / 1e9,
+           "bytes_per_step": asm_bytes}
+    log(f"assembly {asm}")
+    d64.free()
+
+    # ---------------- mcc3 ----------------
+    k3 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    coo = O.lj_third_order_coo(sysd, S.ARGON_LJ, calc)
+    v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
+    coo32 = (coo[0], coo[1], coo[2], v32)
+    phi64 = phi32.astype(np.float64)
+    slabs = [0, n // 2 + 7, n - 1]
+    ref = {l: O.mcc3_fp64_slab(coo32, phi64, l) for l in slabs}
+    log("oracle slabs ready")
+    flop = 6.0 * n ** 4
+    results = {}
+    for prec, steps in (("bf16x3", a.steps), ("fp32", a.fp32_steps)):
+        for _ in range(a.warmup):
+            fc3.mcc3(d32, dphi, precision=prec, out=k3)
+        ctx.synchronize()
+        profile_enable(ctx)
+        ctx.timer_start()
+        for _ in range(steps):
+            fc3.mcc3(d32, dphi, precision=prec, out=k3)
+        t = ctx.timer_stop() / steps
+        m3_ms, m3_c = profile(ctx, 1)
+        m12_ms, m12_c = profile(ctx, 2)
+        profile_enable(ctx, False)
+        errs = []
+        import ctypes
+        from fc3._lib import check
+        buf = np.empty((n, n), dtype=np.float32, order="F")
+        num = den = 0.0
+        for l in slabs:
+            check(ctx.lib.fc3_memcpy_d2h(ctx.h, buf.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_void_p(k3.ptr.value + 4 * n * n * l), 4 * n * n))
+            d = buf.astype(np.float64) - ref[l]
+            num += float((d * d).sum())
+            den += float((ref[l] * ref[l]).sum())
+        err = (num / den) ** 0.5
+        mul = 3.0 if prec == "bf16x3" else 1.0
+        results[prec] = {
+            "ms_per_step": t, "TFLOP_per_s": flop / (t * 1e-3) / 1e12, "rel_fro_err": err,
+            "mode3_ms_per_launch": m3_ms / max(m3_c, 1), "mode3_launches": m3_c // steps,
+            "mode12_ms_per_step": m12_ms / steps, "mode12_launches": m12_c // steps,
+            # MFMA flops the kernels issue (3 bf16 products per FP32-equivalent product)
+            "mfma_flop_per_step": mul * flop,
+        }
+        log(f"mcc3 {prec}: {results[prec]}")
+
+    bf = results["bf16x3"]
+    # dominant kernel family by device time: mode-1/2 GEMMs (2/3 of the flops)
+    m12_flop_launch = 3.0 * 4.0 * n ** 4 / max(bf["mode12_launches"], 1)
+    m12_ms_launch = bf["mode12_ms_per_step"] / max(bf["mode12_launches"], 1)
+    m3_flop = 3.0 * 2.0 * n ** 4
+    fam = {"mode12": (m12_flop_launch, m12_ms_launch), "mode3": (m3_flop, bf["mode3_ms_per_launch"])}
+    dom = max(fam, key=lambda k: fam[k][1] * (bf["mode12_launches"] if k == "mode12" else 1))
+    f_launch, ms_launch = fam[dom]
+    achieved = f_launch / (ms_launch * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": f"k_gemm<BF16X3> {dom}", "achieved": achieved,
+                "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK["bf16"] / 1e12),
+                "traffic": None}
+
+    out = {
+        "metric": METRIC, "value": bf["TFLOP_per_s"], "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": bf["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "data": "synthetic: seeded FCC LJ argon, sigma_u = 0.05 A (numpy PCG64 seed 2592)",
+        "config": {"workload": f"LJ argon FCC {a.cells}x{a.cells}x{a.cells} ({n} DoF): IFC3 "
+                               "assembly + mcc3 (BF16x3 headline, FP32 alongside)",
+                   "n_dof": n, "n_atoms": sysd.n_atoms},
+        "roofline": roofline,
+        "mcc3": results,
+        "fp32_vs_bf16x3_speedup": results["fp32"]["ms_per_step"] / bf["ms_per_step"],
+        "assembly": asm,
+    }
+    if not a.no_cpu_baseline:
+        log("cpu baselines")
+        out["cpu_baseline"] = cpu_baseline_mcc3()
+        out["cpu_baseline_assembly"] = cpu_baseline_assembly()
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,439 @@
+// C-ABI entry points (include/fc3.h): argument validation with the reference's
+// assertion messages, workspace management and dispatch to the HIP kernels.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fc3_internal.hpp"
+
+namespace fc3 {
+
+namespace {
+thread_local std::string g_err = "";
+}
+
+int set_error(int status, const std::string& msg)
+{
+    g_err = msg;
+    return status;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+int ensure_workspace(fc3_context* h, size_t bytes)
+{
+    if (bytes <= h->ws_bytes) return FC3_OK;
+    if (h->ws) {
+        FC3_HIP(hipStreamSynchronize(h->stream));
+        FC3_HIP(hipFree(h->ws));
+        h->ws = nullptr;
+        h->ws_bytes = 0;
+    }
+    const size_t want = (bytes + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+    hipError_t e = hipMalloc(&h->ws, want);
+    if (e != hipSuccess) {
+        h->ws = nullptr;
+        return set_error(FC3_ERR_OUT_OF_MEMORY,
+                         "workspace allocation of " + std::to_string(want) + " bytes failed: " +
+                             hipGetErrorString(e));
+    }
+    h->ws_bytes = want;
+    return FC3_OK;
+}
+
+}  // namespace fc3
+
+using namespace fc3;
+
+namespace {
+
+int check_handle(fc3_handle h)
+{
+    if (!h) return set_error(FC3_ERR_ARGUMENT, "null fc3 handle");
+    FC3_HIP(hipSetDevice(h->device));
+    set_current(h);
+    return FC3_OK;
+}
+
+int check_system(const double* pos, int n_atoms, const double* box)
+{
+    if (!pos || !box) return set_error(FC3_ERR_ARGUMENT, "positions and box must be non-null");
+    if (n_atoms < 1) return set_error(FC3_ERR_ARGUMENT, "n_atoms must be >= 1");
+    if ((long long)n_atoms * 3 > 65535)
+        return set_error(FC3_ERR_UNSUPPORTED, "3*n_atoms must be <= 65535");
+    for (int c = 0; c < 3; ++c)
+        if (!(box[c] > 0.0) || !std::isfinite(box[c]))
+            return set_error(FC3_ERR_ARGUMENT, "box sizes must be positive and finite");
+    return FC3_OK;
+}
+
+// Neighbour list + position upload into the handle workspace.  `extra` bytes of
+// workspace are reserved after the neighbour data for the caller.
+struct Prepared {
+    double* pos = nullptr;
+    double* masses = nullptr;
+    NeighborList nl;
+    char* extra = nullptr;
+};
+
+int prepare(fc3_handle h, const double* pos, int N, const double* box, double r_cut,
+            const double* masses, size_t extra_per_slot, size_t extra_fixed, Prepared& P)
+{
+    // phase 1: counts (small workspace)
+    const size_t head = Carve::need(sizeof(double) * 3 * N) + Carve::need(sizeof(double) * N) +
+                        Carve::need(sizeof(int) * N) + Carve::need(sizeof(int));
+    FC3_TRY(ensure_workspace(h, head));
+    {
+        Carve cv(h->ws, h->ws_bytes);
+        P.pos = cv.take<double>(3 * (size_t)N);
+        P.masses = cv.take<double>(N);
+        int* count = cv.take<int>(N);
+        int* maxc = cv.take<int>(1);
+        FC3_HIP(hipMemcpyAsync(P.pos, pos, sizeof(double) * 3 * N, hipMemcpyHostToDevice,
+                               h->stream));
+        int hmax = 0;
+        FC3_TRY(neighbor_count(h->stream, P.pos, N, box, r_cut, count, maxc, &hmax));
+        P.nl.cap = hmax > 0 ? hmax : 1;
+    }
+    // phase 2: full workspace
+    const int cap = P.nl.cap;
+    const size_t slots = (size_t)N * cap;
+    const size_t total = head + 2 * Carve::need(sizeof(int) * slots) +
+                         Carve::need(extra_per_slot * slots) + Carve::need(extra_fixed) + 1024;
+    FC3_TRY(ensure_workspace(h, total));  // may move the workspace: re-upload below
+    Carve cv(h->ws, h->ws_bytes);
+    P.pos = cv.take<double>(3 * (size_t)N);
+    P.masses = cv.take<double>(N);
+    P.nl.n_atoms = N;
+    P.nl.count = cv.take<int>(N);
+    cv.take<int>(1);
+    P.nl.nbr = cv.take<int>(slots);
+    P.nl.rev = cv.take<int>(slots);
+    P.extra = cv.take<char>(extra_per_slot * slots + extra_fixed);
+    FC3_HIP(hipMemcpyAsync(P.pos, pos, sizeof(double) * 3 * N, hipMemcpyHostToDevice, h->stream));
+    if (masses)
+        FC3_HIP(hipMemcpyAsync(P.masses, masses, sizeof(double) * N, hipMemcpyHostToDevice,
+                               h->stream));
+    // counts again (cheap) so that they live in the final workspace
+    int* maxc = reinterpret_cast<int*>(reinterpret_cast<char*>(P.nl.count) +
+                                       Carve::need(sizeof(int) * N));
+    int hmax = 0;
+    FC3_TRY(neighbor_count(h->stream, P.pos, N, box, r_cut, P.nl.count, maxc, &hmax));
+    if (hmax > cap) return set_error(FC3_ERR_CAPACITY, "neighbour count changed between passes");
+    FC3_TRY(neighbor_fill(h->stream, P.pos, N, box, r_cut, P.nl));
+    return FC3_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fc3_version(void) { return 100; }
+
+const char* fc3_last_error(void) { return fc3::last_error(); }
+
+const char* fc3_status_string(int status)
+{
+    switch (status) {
+        case FC3_OK: return "ok";
+        case FC3_ERR_ARGUMENT: return "invalid argument";
+        case FC3_ERR_CUTOFF: return "cutoff larger than box";
+        case FC3_ERR_BLOCK_SIZE: return "invalid block_size";
+        case FC3_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case FC3_ERR_HIP: return "HIP runtime error";
+        case FC3_ERR_UNSUPPORTED: return "unsupported";
+        case FC3_ERR_CAPACITY: return "capacity exceeded";
+        default: return "unknown status";
+    }
+}
+
+int fc3_create(fc3_handle* out, int device)
+{
+    if (!out) return set_error(FC3_ERR_ARGUMENT, "out must be non-null");
+    *out = nullptr;
+    int ndev = 0;
+    FC3_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return set_error(FC3_ERR_ARGUMENT, "device " + std::to_string(device) +
+                                               " out of range (" + std::to_string(ndev) +
+                                               " devices)");
+    FC3_HIP(hipSetDevice(device));
+    auto* h = new fc3_context();
+    h->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev_stop);
+    if (e != hipSuccess) {
+        delete h;
+        return set_error(FC3_ERR_HIP, std::string("fc3_create: ") + hipGetErrorString(e));
+    }
+    *out = h;
+    return FC3_OK;
+}
+
+int fc3_destroy(fc3_handle h)
+{
+    if (!h) return FC3_OK;
+    hipSetDevice(h->device);
+    hipStreamSynchronize(h->stream);
+    if (h->ws) hipFree(h->ws);
+    for (auto& r : h->prof.records) {
+        hipEventDestroy(r.start);
+        hipEventDestroy(r.stop);
+    }
+    for (auto e : h->prof.pool) hipEventDestroy(e);
+    if (h->ev_start) hipEventDestroy(h->ev_start);
+    if (h->ev_stop) hipEventDestroy(h->ev_stop);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return FC3_OK;
+}
+
+int fc3_synchronize(fc3_handle h)
+{
+    FC3_TRY(check_handle(h));
+    FC3_HIP(hipStreamSynchronize(h->stream));
+    return FC3_OK;
+}
+
+int fc3_get_device(fc3_handle h, int* device)
+{
+    FC3_TRY(check_handle(h));
+    if (!device) return set_error(FC3_ERR_ARGUMENT, "device must be non-null");
+    *device = h->device;
+    return FC3_OK;
+}
+
+int fc3_device_alloc(fc3_handle h, void** dptr, size_t bytes)
+{
+    FC3_TRY(check_handle(h));
+    if (!dptr) return set_error(FC3_ERR_ARGUMENT, "dptr must be non-null");
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e != hipSuccess) {
+        *dptr = nullptr;
+        return set_error(FC3_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(bytes) +
+                                                    " bytes failed: " + hipGetErrorString(e));
+    }
+    return FC3_OK;
+}
+
+int fc3_device_free(fc3_handle h, void* dptr)
+{
+    FC3_TRY(check_handle(h));
+    if (!dptr) return FC3_OK;
+    FC3_HIP(hipStreamSynchronize(h->stream));
+    FC3_HIP(hipFree(dptr));
+    return FC3_OK;
+}
+
+int fc3_memcpy_h2d(fc3_handle h, void* dst, const void* src, size_t bytes)
+{
+    FC3_TRY(check_handle(h));
+    if (bytes == 0) return FC3_OK;
+    if (!dst || !src) return set_error(FC3_ERR_ARGUMENT, "null pointer in fc3_memcpy_h2d");
+    FC3_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+    FC3_HIP(hipStreamSynchronize(h->stream));
+    return FC3_OK;
+}
+
+int fc3_memcpy_d2h(fc3_handle h, void* dst, const void* src, size_t bytes)
+{
+    FC3_TRY(check_handle(h));
+    if (bytes == 0) return FC3_OK;
+    if (!dst || !src) return set_error(FC3_ERR_ARGUMENT, "null pointer in fc3_memcpy_d2h");
+    FC3_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+    FC3_HIP(hipStreamSynchronize(h->stream));
+    return FC3_OK;
+}
+
+int fc3_memset_zero(fc3_handle h, void* dst, size_t bytes)
+{
+    FC3_TRY(check_handle(h));
+    if (bytes == 0) return FC3_OK;
+    if (!dst) return set_error(FC3_ERR_ARGUMENT, "null pointer in fc3_memset_zero");
+    FC3_HIP(hipMemsetAsync(dst, 0, bytes, h->stream));
+    return FC3_OK;
+}
+
+int fc3_device_memory(fc3_handle h, size_t* free_bytes, size_t* total_bytes)
+{
+    FC3_TRY(check_handle(h));
+    size_t f = 0, t = 0;
+    FC3_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return FC3_OK;
+}
+
+int fc3_timer_start(fc3_handle h)
+{
+    FC3_TRY(check_handle(h));
+    FC3_HIP(hipEventRecord(h->ev_start, h->stream));
+    return FC3_OK;
+}
+
+int fc3_timer_stop(fc3_handle h, float* ms)
+{
+    FC3_TRY(check_handle(h));
+    FC3_HIP(hipEventRecord(h->ev_stop, h->stream));
+    FC3_HIP(hipEventSynchronize(h->ev_stop));
+    float t = 0.0f;
+    FC3_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
+    if (ms) *ms = t;
+    return FC3_OK;
+}
+
+int fc3_second_order_lj(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                        double sigma, double epsilon, double pot_r_cut, double r_cut, double tol,
+                        double* ifc2)
+{
+    (void)pot_r_cut;
+    FC3_TRY(check_handle(h));
+    FC3_TRY(check_system(positions, n_atoms, box));
+    if (!ifc2) return set_error(FC3_ERR_ARGUMENT, "ifc2 must be non-null");
+    const int N = n_atoms;
+    Prepared P;
+    const size_t per_slot = Carve::need(sizeof(double) * 27) + Carve::need(sizeof(double) * 9);
+    FC3_TRY(prepare(h, positions, N, box, r_cut, nullptr, sizeof(double) * (27 + 9),
+                    sizeof(double) * N * (27 + 9) + 2048, P));
+    (void)per_slot;
+    const size_t slots = (size_t)N * P.nl.cap;
+    double* E3 = reinterpret_cast<double*>(P.extra);
+    double* E2 = E3 + slots * 27;
+    double* S2 = E2 + slots * 9;
+    FC3_TRY(lj_pair_blocks(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, nullptr, E2, S2));
+    FC3_TRY(lj_write_ifc2(h->stream, P.nl, E2, S2, tol, ifc2));
+    return FC3_OK;
+}
+
+int fc3_third_order_lj(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                       double sigma, double epsilon, double pot_r_cut, double r_cut, double tol,
+                       const double* masses, double* psi64, float* psi32)
+{
+    FC3_TRY(check_handle(h));
+    FC3_TRY(check_system(positions, n_atoms, box));
+    for (int c = 0; c < 3; ++c)
+        if (!(pot_r_cut < box[c])) return set_error(FC3_ERR_CUTOFF, "Cutoff larger than L/2");
+    if (!psi64 && !psi32) return set_error(FC3_ERR_ARGUMENT, "psi64 and psi32 are both null");
+    if (masses)
+        for (int i = 0; i < n_atoms; ++i)
+            if (!(masses[i] > 0.0))
+                return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    const int N = n_atoms;
+    Prepared P;
+    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, sizeof(double) * (27 + 9),
+                    sizeof(double) * N * (27 + 9) + 2048, P));
+    const size_t slots = (size_t)N * P.nl.cap;
+    double* E3 = reinterpret_cast<double*>(P.extra);
+    double* E2 = E3 + slots * 27;
+    double* S3 = E2 + slots * 9;
+    FC3_TRY(lj_pair_blocks(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, S3, E2, nullptr));
+    Ifc3Blocks b;
+    b.E = E3;
+    b.S = S3;
+    FC3_TRY(write_ifc3_pairwise(h->stream, P.nl, b, tol, masses ? P.masses : nullptr, psi64,
+                                psi32));
+    return FC3_OK;
+}
+
+int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int dim,
+                                const double* masses)
+{
+    FC3_TRY(check_handle(h));
+    if (dim < 1 || dim > 3) return set_error(FC3_ERR_ARGUMENT, "D must be in 1:3");
+    if (dim != 3) return set_error(FC3_ERR_UNSUPPORTED, "only D = 3 is supported");
+    if (!psi64 || !masses || n_atoms < 1)
+        return set_error(FC3_ERR_ARGUMENT, "psi64/masses must be non-null, n_atoms >= 1");
+    FC3_TRY(ensure_workspace(h, sizeof(double) * n_atoms + 256));
+    double* dm = static_cast<double*>(h->ws);
+    FC3_HIP(hipMemcpyAsync(dm, masses, sizeof(double) * n_atoms, hipMemcpyHostToDevice,
+                           h->stream));
+    FC3_TRY(mass_weight3(h->stream, psi64, n_atoms, dm));
+    return FC3_OK;
+}
+
+int fc3_second_order_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                        const double* sw_params, double r_cut, double tol, double* ifc2)
+{
+    (void)positions; (void)n_atoms; (void)box; (void)sw_params; (void)r_cut; (void)tol;
+    (void)ifc2;
+    FC3_TRY(check_handle(h));
+    return set_error(FC3_ERR_UNSUPPORTED, "fc3_second_order_sw: not built yet");
+}
+
+int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                       const double* sw_params, double r_cut, double tol, const double* masses,
+                       double* psi64, float* psi32)
+{
+    (void)positions; (void)n_atoms; (void)box; (void)sw_params; (void)r_cut; (void)tol;
+    (void)masses; (void)psi64; (void)psi32;
+    FC3_TRY(check_handle(h));
+    return set_error(FC3_ERR_UNSUPPORTED, "fc3_third_order_sw: not built yet");
+}
+
+int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precision, float* k3)
+{
+    FC3_TRY(check_handle(h));
+    if (!psi || !phi || !k3) return set_error(FC3_ERR_ARGUMENT, "psi, phi, k3 must be non-null");
+    if (n < 1) return set_error(FC3_ERR_ARGUMENT, "n must be >= 1");
+    if (precision != FC3_PREC_FP32 && precision != FC3_PREC_BF16X3)
+        return set_error(FC3_ERR_ARGUMENT, "unknown precision");
+    if ((long long)n * n >= (1LL << 31))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: n^2 must be < 2^31");
+    if ((const void*)psi == (const void*)k3)
+        return set_error(FC3_ERR_ARGUMENT, "k3 must not alias psi (use fc3_mcc3_inplace)");
+    const size_t ws = mcc3_workspace_bytes(n, precision, false);
+    FC3_TRY(ensure_workspace(h, ws));
+    return mcc3_run(h->stream, psi, phi, n, precision, k3, h->ws, h->ws_bytes);
+}
+
+int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int precision)
+{
+    FC3_TRY(check_handle(h));
+    if (!psi || !phi) return set_error(FC3_ERR_ARGUMENT, "psi, phi must be non-null");
+    if (n < 1) return set_error(FC3_ERR_ARGUMENT, "n must be >= 1");
+    if (precision != FC3_PREC_FP32 && precision != FC3_PREC_BF16X3)
+        return set_error(FC3_ERR_ARGUMENT, "unknown precision");
+    if ((long long)n * n >= (1LL << 31))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: n^2 must be < 2^31");
+    const size_t ws = mcc3_workspace_bytes(n, precision, true);
+    FC3_TRY(ensure_workspace(h, ws));
+    return mcc3_run_inplace(h->stream, psi, phi, n, precision, h->ws, h->ws_bytes);
+}
+
+int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n, int block_size,
+                     int precision, float* k3_host)
+{
+    FC3_TRY(check_handle(h));
+    if (!psi || !phi || !k3_host)
+        return set_error(FC3_ERR_ARGUMENT, "psi, phi, k3_host must be non-null");
+    if (block_size <= 0 || n % block_size != 0)
+        return set_error(FC3_ERR_BLOCK_SIZE, "size(phi)[1] % block_size == 0 && block_size > 0");
+    // Compute the full K3 on the device, then stream it to the host slab by slab
+    // (block_size slabs of n^2 floats per copy).
+    const size_t bytes = sizeof(float) * (size_t)n * n * n;
+    float* k3 = nullptr;
+    hipError_t e = hipMalloc(&k3, bytes);
+    if (e != hipSuccess)
+        return set_error(FC3_ERR_OUT_OF_MEMORY, std::string("mcc3 blocked: ") + hipGetErrorString(e));
+    int st = fc3_mcc3(h, psi, phi, n, precision, k3);
+    if (st == FC3_OK) {
+        const size_t slab = sizeof(float) * (size_t)n * n * block_size;
+        for (int l0 = 0; l0 < n && st == FC3_OK; l0 += block_size) {
+            e = hipMemcpyAsync(reinterpret_cast<char*>(k3_host) + slab / block_size * l0,
+                               reinterpret_cast<char*>(k3) + slab / block_size * l0, slab,
+                               hipMemcpyDeviceToHost, h->stream);
+            if (e != hipSuccess) st = set_error(FC3_ERR_HIP, hipGetErrorString(e));
+        }
+        e = hipStreamSynchronize(h->stream);
+        if (st == FC3_OK && e != hipSuccess) st = set_error(FC3_ERR_HIP, hipGetErrorString(e));
+    }
+    hipStreamSynchronize(h->stream);
+    hipFree(k3);
+    return st;
+}
+
+}  // extern "C"

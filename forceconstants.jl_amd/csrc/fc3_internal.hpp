@@ -1,0 +1,129 @@
+// Internal declarations shared by the C-ABI layer and the HIP kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fc3.h"
+
+namespace fc3 {
+
+// Thread-local error message; set_error returns the status for `return set_error(...)`.
+int set_error(int status, const std::string& msg);
+const char* last_error();
+
+#define FC3_HIP(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::fc3::set_error(e_ == hipErrorOutOfMemory ? FC3_ERR_OUT_OF_MEMORY  \
+                                                              : FC3_ERR_HIP,           \
+                                    std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define FC3_TRY(expr)                    \
+    do {                                 \
+        int s_ = (expr);                 \
+        if (s_ != FC3_OK) return s_;     \
+    } while (0)
+
+// Per-handle kernel timing: when enabled, launch sites bracket their kernels with
+// HIP events on the launch stream (ProfScope) and fc3_profile_query sums them.
+struct ProfRecord {
+    int kind;
+    hipEvent_t start, stop;
+};
+struct Profiler {
+    bool enabled = false;
+    std::vector<ProfRecord> records;
+    std::vector<hipEvent_t> pool;
+};
+}  // namespace fc3
+
+struct fc3_context {
+    fc3::Profiler prof;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // Workspace: reserved (size computed per call), grown with a stream sync.
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+namespace fc3 {
+
+// The handle whose API call is running on this thread (set by the C-ABI layer).
+void set_current(fc3_context* h);
+struct ProfScope {
+    fc3_context* h = nullptr;
+    hipStream_t s = nullptr;
+    int rec = -1;
+    ProfScope(int kind, hipStream_t stream);
+    ~ProfScope();
+};
+
+// Ensure the handle's workspace holds at least `bytes`.
+int ensure_workspace(fc3_context* h, size_t bytes);
+
+// Bump allocator over a workspace region.
+struct Carve {
+    char* p;
+    size_t off = 0, cap;
+    Carve(void* base, size_t c) : p(static_cast<char*>(base)), cap(c) {}
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* r = reinterpret_cast<T*>(p + off);
+        off += count * sizeof(T);
+        return r;
+    }
+    static size_t need(size_t bytes) { return ((bytes + 255) & ~size_t(255)); }
+};
+
+// ---- neighbour lists (neighbors.hip) ---------------------------------------------
+// Full neighbour lists of every atom, sorted by atom index, minimum-image rule of
+// nearest_mirror! with the reference's strict cutoff test dist^2 < r_cut^2.
+struct NeighborList {
+    int n_atoms = 0;
+    int cap = 0;          // slots per atom
+    int* count = nullptr; // [N]
+    int* nbr = nullptr;   // [N*cap]
+    int* rev = nullptr;   // [N*cap]  slot of atom a in the list of nbr[a][s]
+};
+
+// Counts neighbours and returns the maximum count (synchronises the stream).
+int neighbor_count(hipStream_t s, const double* d_pos, int N, const double box[3],
+                   double r_cut, int* d_count, int* d_maxcount, int* h_max);
+int neighbor_fill(hipStream_t s, const double* d_pos, int N, const double box[3],
+                  double r_cut, NeighborList& nl);
+
+// ---- LJ kernels (lj.hip) ---------------------------------------------------------
+int lj_pair_blocks(hipStream_t s, const double* d_pos, const double box[3], double sigma,
+                   double epsilon, const NeighborList& nl, double* E3, double* S3,
+                   double* E2, double* S2);
+int lj_write_ifc2(hipStream_t s, const NeighborList& nl, const double* E2, const double* S2,
+                  double tol, double* ifc2);
+
+// Generic dense third-order writer over per-pair blocks (shared by LJ and SW):
+// see the kernel comment in ifc3_writer.hip.
+struct Ifc3Blocks {
+    const double* E = nullptr;   // pair/direct blocks per (atom, slot), 27 doubles
+    const double* S = nullptr;   // self blocks per atom, 27 doubles
+};
+int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b,
+                        double tol, const double* d_masses, double* psi64, float* psi32);
+
+int mass_weight3(hipStream_t s, double* psi, int N, const double* d_masses);
+
+// ---- mcc3 (mcc3.hip) --------------------------------------------------------------
+size_t mcc3_workspace_bytes(int n, int precision, bool inplace);
+int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int precision,
+             float* k3, void* ws, size_t ws_bytes);
+int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int precision,
+                     void* ws, size_t ws_bytes);
+
+}  // namespace fc3

@@ -1,0 +1,20 @@
+"""fc3: MI355X-native third-order force constants and modal coupling constants.
+
+A drop-in for ForceConstants.jl's third-order pipeline (second_order, third_order,
+mass_weight_third_order!, mcc3) backed by hand-written HIP kernels for gfx950 in
+libfc3.so.  See api.py for the reference-named entry points.
+"""
+from . import systems
+from .api import (Context, DenseForceConstants, DeviceArray, dynamical_matrix, get_modes,
+                  mass_weight_third_order_, mcc3, mcc3_, second_order, second_order_device,
+                  third_order, third_order_device)
+from .systems import (AnalyticalCalculator, AutoDiffCalculator, LJ, StillingerWeberSilicon,
+                      SuperCellSystem, SWParams, diamond, fcc)
+
+__all__ = [
+    "Context", "DenseForceConstants", "DeviceArray", "dynamical_matrix", "get_modes",
+    "mass_weight_third_order_", "mcc3", "mcc3_", "second_order", "second_order_device",
+    "third_order", "third_order_device", "systems", "AnalyticalCalculator",
+    "AutoDiffCalculator", "LJ", "StillingerWeberSilicon", "SuperCellSystem", "SWParams",
+    "diamond", "fcc",
+]

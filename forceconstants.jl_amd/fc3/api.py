@@ -1,0 +1,281 @@
+"""Host-side mirror of ForceConstants.jl's third-order API over the HIP library.
+
+Same names, argument meaning and error behaviour as the reference (Julia `!` is
+spelled with a trailing underscore):
+
+  second_order(sys, pot, calc)              src/ifc.jl:3-18
+  third_order(sys, pot, calc)               src/ifc.jl:20-35
+  mass_weight_third_order_(ifc3, masses)    src/analytical/third_order_analytical.jl:185-214
+  dynamical_matrix(sys, pot, calc)          src/dynmat.jl:5-33
+  get_modes(dynmat, num_rigid_translation)  src/dynmat.jl:123-131
+  mcc3(psi, phi) / mcc3(psi, phi, block_size) / mcc3_(psi, phi)   src/mcc.jl:29,60,89
+
+GPU tensors are `DeviceArray`s (a device pointer with Fortran-order shape), the
+counterpart of CuArray in the reference's examples (README.md:72-76).  All compute
+goes through libfc3.so; nothing here computes force constants on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import FC3_PREC_BF16X3, FC3_PREC_FP32, check
+from .systems import (AnalyticalCalculator, AutoDiffCalculator, LJ, StillingerWeberSilicon,
+                      SuperCellSystem)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One fc3 handle: a device, a HIP stream and a workspace."""
+
+    _default: Optional["Context"] = None
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.lib.fc3_create(ctypes.byref(h), int(device)))
+        self.h = h
+        self.device = device
+
+    @classmethod
+    def default(cls) -> "Context":
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fc3_destroy(self.h)
+            self.h = None
+
+    def synchronize(self):
+        check(self.lib.fc3_synchronize(self.h))
+
+    def timer_start(self):
+        check(self.lib.fc3_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = ctypes.c_float()
+        check(self.lib.fc3_timer_stop(self.h, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def memory(self) -> Tuple[int, int]:
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        check(self.lib.fc3_device_memory(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return int(f.value), int(t.value)
+
+
+class DeviceArray:
+    """A device allocation with a Fortran-order shape and a dtype."""
+
+    def __init__(self, shape, dtype, ctx: Optional[Context] = None, zero: bool = False):
+        self.ctx = ctx or Context.default()
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(self.ctx.lib.fc3_device_alloc(self.ctx.h, ctypes.byref(p), self.nbytes))
+        self.ptr = p
+        if zero:
+            check(self.ctx.lib.fc3_memset_zero(self.ctx.h, self.ptr, self.nbytes))
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, dtype=None, ctx: Optional[Context] = None) -> "DeviceArray":
+        a = np.asfortranarray(a, dtype=dtype or a.dtype)
+        d = cls(a.shape, a.dtype, ctx)
+        check(d.ctx.lib.fc3_memcpy_h2d(d.ctx.h, d.ptr, a.ctypes.data_as(ctypes.c_void_p), d.nbytes))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        check(self.ctx.lib.fc3_memcpy_d2h(self.ctx.h, out.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                                          self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.fc3_device_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class DenseForceConstants:
+    """types.jl:76-90: values, units, tol.  `values` is a host array (Fortran order)."""
+
+    values: np.ndarray
+    units: str
+    tol: float
+
+    def __getitem__(self, idx):
+        return self.values[idx]
+
+    @property
+    def shape(self):
+        return self.values.shape
+
+
+def _units(pot, order: int) -> str:
+    e, l = ("kcal/mol", "Angstrom") if isinstance(pot, LJ) else ("eV", "Angstrom")
+    return f"{e} / {l}^{order}"
+
+
+def _sys_args(sys: SuperCellSystem):
+    pos = np.ascontiguousarray(sys.positions, dtype=np.float64)
+    box = np.ascontiguousarray(sys.box_sizes, dtype=np.float64)
+    return pos, box
+
+
+def second_order_device(sys: SuperCellSystem, pot, calc, out: Optional[DeviceArray] = None,
+                        ctx: Optional[Context] = None) -> DeviceArray:
+    ctx = ctx or Context.default()
+    n = sys.n_dof
+    out = out or DeviceArray((n, n), np.float64, ctx)
+    pos, box = _sys_args(sys)
+    if isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator):
+        st = ctx.lib.fc3_second_order_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
+                                         pot.epsilon, pot.r_cut, calc.r_cut, calc.tol, out.ptr)
+    elif isinstance(pot, StillingerWeberSilicon) and isinstance(calc, AutoDiffCalculator):
+        prm = pot.params.as_array()
+        st = ctx.lib.fc3_second_order_sw(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), _ptr(prm),
+                                         calc.r_cut, calc.tol, out.ptr)
+    else:
+        raise NotImplementedError(f"second_order for {type(pot).__name__} with {type(calc).__name__}")
+    check(st)
+    return out
+
+
+def second_order(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = None
+                 ) -> DenseForceConstants:
+    """second_order(sys, pot, calc) -> DenseForceConstants{2} (host values)."""
+    d = second_order_device(sys, pot, calc, ctx=ctx)
+    vals = d.to_host()
+    d.free()
+    return DenseForceConstants(vals, _units(pot, 2), calc.tol)
+
+
+def third_order_device(sys: SuperCellSystem, pot, calc, psi64: Optional[DeviceArray] = None,
+                       psi32: Optional[DeviceArray] = None, masses=None,
+                       ctx: Optional[Context] = None) -> None:
+    """Fill psi64 (FP64 third_order result) and/or psi32 (FP32, mass weighted when
+    `masses` is given) on the device."""
+    ctx = ctx or Context.default()
+    pos, box = _sys_args(sys)
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    mp = _ptr(m) if m is not None else None
+    p64 = psi64.ptr if psi64 is not None else None
+    p32 = psi32.ptr if psi32 is not None else None
+    if isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator):
+        st = ctx.lib.fc3_third_order_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
+                                        pot.epsilon, pot.r_cut, calc.r_cut, calc.tol, mp, p64, p32)
+    elif isinstance(pot, StillingerWeberSilicon) and isinstance(calc, AutoDiffCalculator):
+        prm = pot.params.as_array()
+        st = ctx.lib.fc3_third_order_sw(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), _ptr(prm),
+                                        calc.r_cut, calc.tol, mp, p64, p32)
+    else:
+        raise NotImplementedError(f"third_order for {type(pot).__name__} with {type(calc).__name__}")
+    check(st)
+
+
+def third_order(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = None
+                ) -> DenseForceConstants:
+    """third_order(sys, pot, calc) -> DenseForceConstants{3} (host values)."""
+    n = sys.n_dof
+    d = DeviceArray((n, n, n), np.float64, ctx)
+    third_order_device(sys, pot, calc, psi64=d, ctx=ctx)
+    vals = d.to_host()
+    d.free()
+    return DenseForceConstants(vals, _units(pot, 3), calc.tol)
+
+
+def mass_weight_third_order_(ifc3, masses, ctx: Optional[Context] = None):
+    """mass_weight_third_order!(Psi, masses): divides nonzero entries by
+    sqrt(m_i m_j m_k) in place.  Accepts a DeviceArray (FP64) or a
+    DenseForceConstants whose host values are weighted on the GPU."""
+    ctx = ctx or Context.default()
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    if isinstance(ifc3, DeviceArray):
+        n = ifc3.shape[0]
+        dim = n // len(m)
+        check(ctx.lib.fc3_mass_weight_third_order(ctx.h, ifc3.ptr, len(m), dim, _ptr(m)))
+        return ifc3
+    d = DeviceArray.from_host(np.asfortranarray(ifc3.values, dtype=np.float64), ctx=ctx)
+    n = d.shape[0]
+    check(ctx.lib.fc3_mass_weight_third_order(ctx.h, d.ptr, len(m), n // len(m), _ptr(m)))
+    ifc3.values = d.to_host()
+    d.free()
+    ifc3.units = ifc3.units + " / amu"
+    return ifc3
+
+
+def dynamical_matrix(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = None
+                     ) -> DenseForceConstants:
+    """dynmat.jl:5-33: IFC2 on the GPU, divided by sqrt(m_i m_j) on the host."""
+    ifc2 = second_order(sys, pot, calc, ctx=ctx)
+    m = np.repeat(sys.masses, 3)
+    vals = ifc2.values / np.sqrt(np.outer(m, m))
+    return DenseForceConstants(np.asfortranarray(vals), ifc2.units + " / amu", 0.0)
+
+
+def get_modes(dynmat, num_rigid_translation: int = 3):
+    """dynmat.jl:123-131: FP64 LAPACK eigendecomposition of the Hermitian matrix;
+    the num_rigid_translation eigenvalues of smallest |w^2| are set to zero."""
+    D = dynmat.values if isinstance(dynmat, DenseForceConstants) else np.asarray(dynmat)
+    w2, phi = np.linalg.eigh(D)
+    idx = np.argsort(np.abs(w2), kind="stable")
+    w2 = w2.copy()
+    w2[idx[:num_rigid_translation]] = 0.0
+    return w2, np.asfortranarray(phi)
+
+
+_PREC = {"fp32": FC3_PREC_FP32, "bf16x3": FC3_PREC_BF16X3}
+
+
+def _prec(precision) -> int:
+    if isinstance(precision, str):
+        return _PREC[precision.lower()]
+    return int(precision)
+
+
+def mcc3(psi: DeviceArray, phi: DeviceArray, block_size: Optional[int] = None,
+         precision="fp32", out: Optional[DeviceArray] = None):
+    """mcc3(Psi, phi) -> device K3;  mcc3(Psi, phi, block_size) -> host K3."""
+    ctx = psi.ctx
+    n = phi.shape[0]
+    if psi.dtype != np.float32 or phi.dtype != np.float32:
+        raise TypeError("mcc3 takes Float32 device arrays")
+    if psi.shape != (n, n, n) or phi.shape != (n, n):
+        raise AssertionError("mcc3: Psi must be n x n x n and phi n x n")
+    p = _prec(precision)
+    if block_size is None:
+        out = out or DeviceArray((n, n, n), np.float32, ctx)
+        check(ctx.lib.fc3_mcc3(ctx.h, psi.ptr, phi.ptr, n, p, out.ptr))
+        return out
+    host = np.empty((n, n, n), dtype=np.float32, order="F")
+    check(ctx.lib.fc3_mcc3_blocked(ctx.h, psi.ptr, phi.ptr, n, int(block_size), p,
+                                   host.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return host
+
+
+def mcc3_(psi: DeviceArray, phi: DeviceArray, precision="fp32") -> DeviceArray:
+    """mcc3!(Psi, phi): K3 overwrites Psi; returns the same DeviceArray."""
+    ctx = psi.ctx
+    n = phi.shape[0]
+    if psi.shape != (n, n, n) or phi.shape != (n, n):
+        raise AssertionError("mcc3!: Psi must be n x n x n and phi n x n")
+    check(ctx.lib.fc3_mcc3_inplace(ctx.h, psi.ptr, phi.ptr, n, _prec(precision)))
+    return psi

@@ -1,0 +1,158 @@
+/*
+ * fc3.h -- C ABI of the MI355X (gfx950) third-order force-constant library.
+ *
+ * The drop-in boundary for ForceConstants.jl's third-order pipeline.  Every entry
+ * point takes plain pointers and sizes; no HIP or torch types cross it.  A Julia
+ * `ccall` layer (or Python ctypes) binds these symbols and keeps the reference's
+ * signatures second_order / third_order / mass_weight_third_order! / mcc3.
+ *
+ * Conventions
+ *  - Tensors are column-major (Julia layout), 0-based: Psi[i + n*j + n*n*k], n = 3N.
+ *  - Positions are host arrays of 3*N doubles, atom-major (x0,y0,z0,x1,...), Angstrom.
+ *  - Output tensors are device pointers (from fc3_device_alloc or any HIP allocation
+ *    on the handle's device) unless the name says host.
+ *  - Every call returns an fc3_status; on failure fc3_last_error() holds a message
+ *    (thread-local).  Assertion failures of the reference return the reference's
+ *    message text so the binding can raise the same AssertionError.
+ *  - A handle owns one HIP stream and a workspace; calls on one handle are
+ *    serialised on its stream and a handle must not be used from two threads at
+ *    once.  Distinct handles are independent.
+ *  - Entry points that produce device results return once the work is enqueued;
+ *    fc3_synchronize(h) waits for it.  Entry points with host outputs return after
+ *    the data has arrived.
+ */
+#ifndef FC3_H
+#define FC3_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FC3_OK = 0,
+    FC3_ERR_ARGUMENT = 1,      /* shape / null pointer / size mismatch (AssertionError) */
+    FC3_ERR_CUTOFF = 2,        /* r_cut >= box (third_order_analytical.jl:7) */
+    FC3_ERR_BLOCK_SIZE = 3,    /* n % block_size != 0 or block_size <= 0 (mcc.jl:91-92) */
+    FC3_ERR_OUT_OF_MEMORY = 4, /* device allocation failed */
+    FC3_ERR_HIP = 5,           /* HIP runtime error */
+    FC3_ERR_UNSUPPORTED = 6,   /* option not supported (e.g. D != 3) */
+    FC3_ERR_CAPACITY = 7       /* neighbour list capacity exceeded */
+} fc3_status;
+
+typedef enum {
+    FC3_PREC_FP32 = 0,    /* plain FP32 on FP32 MFMA (the reference's Float32 path) */
+    FC3_PREC_BF16X3 = 1   /* BF16 MFMA, hi/lo split, 3 products, FP32 accumulate */
+} fc3_precision;
+
+typedef struct fc3_context* fc3_handle;
+
+/* ---- library / errors ------------------------------------------------------ */
+int fc3_version(void);                       /* major*10000 + minor*100 + patch */
+const char* fc3_last_error(void);            /* thread-local, never NULL */
+const char* fc3_status_string(int status);
+
+/* ---- handle ---------------------------------------------------------------- */
+int fc3_create(fc3_handle* out, int device);
+int fc3_destroy(fc3_handle h);
+int fc3_synchronize(fc3_handle h);
+int fc3_get_device(fc3_handle h, int* device);
+
+/* ---- device memory (owned by the caller) ----------------------------------- */
+int fc3_device_alloc(fc3_handle h, void** dptr, size_t bytes);
+int fc3_device_free(fc3_handle h, void* dptr);
+int fc3_memcpy_h2d(fc3_handle h, void* dst, const void* src, size_t bytes);  /* synchronous */
+int fc3_memcpy_d2h(fc3_handle h, void* dst, const void* src, size_t bytes);  /* synchronous */
+int fc3_memset_zero(fc3_handle h, void* dst, size_t bytes);                 /* async */
+int fc3_device_memory(fc3_handle h, size_t* free_bytes, size_t* total_bytes);
+
+/* ---- timing on the handle's stream (for measurements) ---------------------- */
+int fc3_timer_start(fc3_handle h);
+int fc3_timer_stop(fc3_handle h, float* ms);   /* waits for the stream */
+
+/* ---- per-kernel timing ------------------------------------------------------ */
+/* Kernel families timed when profiling is enabled on a handle. */
+typedef enum {
+    FC3_KERNEL_IFC3_WRITE = 0,   /* dense IFC3 writer (ASR + tol + mass weight fused) */
+    FC3_KERNEL_GEMM_MODE3 = 1,   /* mcc3 mode-3 GEMM (M = n^2) */
+    FC3_KERNEL_GEMM_MODE12 = 2,  /* mcc3 per-slab mode-1 and mode-2 GEMMs */
+    FC3_KERNEL_PAIR_BLOCKS = 3,  /* pair / triplet block evaluation */
+    FC3_KERNEL_NEIGHBORS = 4,    /* neighbour list construction */
+    FC3_KERNEL_COUNT = 5
+} fc3_kernel_kind;
+/* Enable (1) or disable (0) event timing of each kernel launch on this handle;
+ * enabling clears previous records.  Not a reference interface: measurement aid. */
+int fc3_profile_enable(fc3_handle h, int enable);
+/* Total device time (ms) and launch count of one kernel family since enabling;
+ * waits for the handle's stream. */
+int fc3_profile_query(fc3_handle h, int kind, double* total_ms, int* launches);
+
+/* ---- second order ----------------------------------------------------------- */
+/* second_order(sys, LJ(sigma, epsilon, pot_r_cut), AnalyticalCalculator(tol, r_cut))
+ * replaces src/ifc.jl:3-18 -> src/analytical/second_order_analytical.jl:3-42,
+ * src/asr.jl:3-17, src/helper_funcs.jl:20-28.  ifc2: device, n x n FP64.        */
+int fc3_second_order_lj(fc3_handle h, const double* positions, int n_atoms,
+                        const double box[3], double sigma, double epsilon,
+                        double pot_r_cut, double r_cut, double tol, double* ifc2);
+
+/* second_order(sys, StillingerWeberSilicon(), AutoDiffCalculator(tol, r_cut))
+ * replaces src/ifc.jl:3-18 -> src/autodiff/second_order_AD.jl:39-111.
+ * sw_params: 10 doubles {epsilon, sigma, a, lambda, gamma, cos_theta0, A, B, p, q}
+ * (SW.jl:3-14).                                                                 */
+int fc3_second_order_sw(fc3_handle h, const double* positions, int n_atoms,
+                        const double box[3], const double* sw_params, double r_cut,
+                        double tol, double* ifc2);
+
+/* ---- third order ------------------------------------------------------------ */
+/* third_order(sys, LJ, AnalyticalCalculator) replaces src/ifc.jl:20-35 ->
+ * src/analytical/third_order_analytical.jl:4-69, src/asr.jl:19-40,
+ * src/helper_funcs.jl:20-28, fused with mass_weight_third_order!
+ * (third_order_analytical.jl:185-214) and the Float32 conversion the reference's
+ * callers do before mcc3 (README.md:69-74).
+ *   psi64: device n^3 FP64, the third_order result (ASR + tolerance), or NULL.
+ *   psi32: device n^3 FP32, the same values divided by sqrt(m_i m_j m_k) when
+ *          masses != NULL (unweighted otherwise), or NULL.
+ *   masses: host, n_atoms doubles, or NULL.                                      */
+int fc3_third_order_lj(fc3_handle h, const double* positions, int n_atoms,
+                       const double box[3], double sigma, double epsilon,
+                       double pot_r_cut, double r_cut, double tol,
+                       const double* masses, double* psi64, float* psi32);
+
+/* third_order(sys, StillingerWeberSilicon(), AutoDiffCalculator) replaces
+ * src/ifc.jl:20-35 -> src/autodiff/third_order_AD.jl:71-224 (+ ASR!, apply_tols!),
+ * with the same fused outputs as fc3_third_order_lj.                            */
+int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms,
+                       const double box[3], const double* sw_params, double r_cut,
+                       double tol, const double* masses, double* psi64, float* psi32);
+
+/* mass_weight_third_order!(Psi, masses) replaces
+ * src/analytical/third_order_analytical.jl:185-214: psi64 (device, n^3, n = dim *
+ * n_atoms) is divided in place by sqrt(m_i m_j m_k) where nonzero.              */
+int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int dim,
+                                const double* masses);
+
+/* ---- modal coupling constants ----------------------------------------------- */
+/* mcc3(Psi::CuArray{Float32,3}, phi::CuArray{Float32,2}) replaces src/mcc.jl:29-51
+ * (and mcc3_tens_opt, src/mcc.jl:11-20):
+ *   k3[n,m,l] = sum_ijk psi[i,j,k] phi[i,n] phi[j,m] phi[k,l]
+ * psi, phi, k3: device FP32; k3 must not alias psi.                              */
+int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precision,
+             float* k3);
+
+/* mcc3!(Psi, phi) replaces src/mcc.jl:60-80: the result overwrites psi (device),
+ * using O(n^2) workspace instead of the reference's two extra n^3 tensors.       */
+int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int precision);
+
+/* mcc3(Psi, phi, block_size) replaces src/mcc.jl:89-120: same result as fc3_mcc3,
+ * delivered into a HOST array k3_host (n^3 floats) in slabs of block_size, with the
+ * reference's assertions (n % block_size == 0, block_size > 0).                 */
+int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n,
+                     int block_size, int precision, float* k3_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FC3_H */

@@ -1,0 +1,255 @@
+"""Python face of the CPU oracle.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
+module; the product never does.  Loops live in fc3_oracle.c (a restatement of the
+reference's CPU algorithm, built by oracle/Makefile into oracle/liboracle.so); this
+file adds the dense FP64 linear algebra the reference does on the host:
+
+  dynamical_matrix   src/dynmat.jl:5-33
+  get_modes          src/dynmat.jl:123-131 (LAPACK eigen of the Hermitian matrix,
+                     the num_rigid smallest |omega^2| set to zero)
+  mcc3_fp64          src/mcc.jl:15-17, K3[n,m,l] = sum Psi[i,j,k] phi[i,n] phi[j,m] phi[k,l]
+                     evaluated in FP64 as three mode products
+
+Parity pins (tests/test_oracle_*.py): the 12 known-answer values of runtests.jl,
+an independent closed-form LJ, sympy/mpmath SW derivatives, FP64 finite
+differences of the SW energy, and ASR / permutation-symmetry invariants.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+# cos(theta0) constants of the reference's SW derivative blocks: the
+# FastDifferentiation output bakes -0.333333333333 (SW/H3_exec_ijk.jl:39 and the
+# other generated blocks); the runtime-built iik block uses params.cos_theta0 = -1/3.
+SW_C0_FD = -0.333333333333
+SW_C0_IIK = -1.0 / 3.0
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.fco_lj_ifc2.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.fco_lj_ifc3.argtypes = L.fco_lj_ifc2.argtypes
+        L.fco_lj_ifc3_columns.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, _i64p, _i64p, _dp]
+        L.fco_mass_weight3.argtypes = [_dp, ctypes.c_int, _dp]
+        L.fco_mass_weight3.restype = None
+        L.fco_sw_blocks3.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
+        L.fco_sw_blocks3.restype = None
+        L.fco_sw_blocks2.argtypes = [_dp, _dp, ctypes.c_double, _dp]
+        L.fco_sw_blocks2.restype = None
+        L.fco_sw_pair_blocks.argtypes = [_dp, _dp, _dp, _dp]
+        L.fco_sw_pair_blocks.restype = None
+        L.fco_sw_energy.argtypes = [_dp, ctypes.c_int, _dp, _dp, ctypes.c_double]
+        L.fco_sw_energy.restype = ctypes.c_double
+        L.fco_sw_ifc3.argtypes = [_dp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, _dp]
+        L.fco_sw_ifc2.argtypes = [_dp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _dp]
+        _i32p = ctypes.POINTER(ctypes.c_int32)
+        L.fco_lj_ifc3_coo.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_long, _i32p, _i32p, _i32p, _dp]
+        L.fco_lj_ifc3_coo.restype = ctypes.c_long
+        L.fco_ipow.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.fco_ipow.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _sysargs(sys):
+    pos = np.ascontiguousarray(sys.positions, dtype=np.float64)
+    box = np.ascontiguousarray(sys.box_sizes, dtype=np.float64)
+    return pos, box
+
+
+def lj_second_order(sys, pot, calc) -> np.ndarray:
+    """second_order(sys, LJ, AnalyticalCalculator) -> n x n (column-major values)."""
+    pos, box = _sysargs(sys)
+    n = 3 * sys.n_atoms
+    out = np.zeros((n, n), dtype=np.float64, order="F")
+    rc = lib().fco_lj_ifc2(_p(pos), sys.n_atoms, _p(box), pot.sigma, pot.epsilon, pot.r_cut,
+                           calc.r_cut, calc.tol, _p(out))
+    if rc:
+        raise AssertionError("Cutoff larger than L/2")
+    return out
+
+
+def lj_third_order(sys, pot, calc) -> np.ndarray:
+    """third_order(sys, LJ, AnalyticalCalculator) -> n x n x n (Fortran order)."""
+    pos, box = _sysargs(sys)
+    n = 3 * sys.n_atoms
+    out = np.zeros((n, n, n), dtype=np.float64, order="F")
+    rc = lib().fco_lj_ifc3(_p(pos), sys.n_atoms, _p(box), pot.sigma, pot.epsilon, pot.r_cut,
+                           calc.r_cut, calc.tol, _p(out))
+    if rc:
+        raise AssertionError("Cutoff larger than L/2")
+    return out
+
+
+def lj_third_order_columns(sys, pot, calc, jj, kk) -> np.ndarray:
+    """Columns Psi[:, jj[c], kk[c]] of third_order(sys, LJ, ...) without the dense tensor."""
+    pos, box = _sysargs(sys)
+    jj = np.ascontiguousarray(jj, dtype=np.int64)
+    kk = np.ascontiguousarray(kk, dtype=np.int64)
+    n = 3 * sys.n_atoms
+    out = np.zeros((n, len(jj)), dtype=np.float64, order="F")
+    rc = lib().fco_lj_ifc3_columns(_p(pos), sys.n_atoms, _p(box), pot.sigma, pot.epsilon,
+                                   pot.r_cut, calc.r_cut, calc.tol, len(jj),
+                                   jj.ctypes.data_as(_i64p), kk.ctypes.data_as(_i64p), _p(out))
+    if rc:
+        raise AssertionError("Cutoff larger than L/2")
+    return out
+
+
+def lj_third_order_coo(sys, pot, calc):
+    """Nonzero entries (i, j, k, value) of third_order(sys, LJ, AnalyticalCalculator)
+    without forming the dense tensor (for sizes a dense host tensor cannot hold)."""
+    pos, box = _sysargs(sys)
+    N = sys.n_atoms
+    cap = N * 80 * 81 + N * 27
+    while True:
+        oi = np.empty(cap, np.int32)
+        oj = np.empty(cap, np.int32)
+        ok = np.empty(cap, np.int32)
+        ov = np.empty(cap, np.float64)
+        _i32p = ctypes.POINTER(ctypes.c_int32)
+        m = lib().fco_lj_ifc3_coo(_p(pos), N, _p(box), pot.sigma, pot.epsilon, pot.r_cut,
+                                  calc.r_cut, calc.tol, cap, oi.ctypes.data_as(_i32p),
+                                  oj.ctypes.data_as(_i32p), ok.ctypes.data_as(_i32p), _p(ov))
+        if m == -1:
+            raise AssertionError("Cutoff larger than L/2")
+        if m == -2:
+            cap *= 2
+            continue
+        return oi[:m].copy(), oj[:m].copy(), ok[:m].copy(), ov[:m].copy()
+
+
+def coo_mass_weight(coo, masses):
+    """mass_weight_third_order! applied to a COO tensor (returns new values)."""
+    i, j, k, v = coo
+    m = np.asarray(masses, dtype=np.float64)
+    return v / np.sqrt((m[i // 3] * m[j // 3]) * m[k // 3])
+
+
+def mass_weight_third_order(psi: np.ndarray, masses) -> np.ndarray:
+    """mass_weight_third_order! on a Fortran-ordered FP64 array (in place)."""
+    assert psi.flags.f_contiguous and psi.dtype == np.float64
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    lib().fco_mass_weight3(_p(psi), len(m), _p(m))
+    return psi
+
+
+def sw_blocks3(r9, params, c0_fd=SW_C0_FD, c0_iik=SW_C0_IIK) -> np.ndarray:
+    """7 x (3,3,3) blocks iij, ijj, iik, ikk, jjk, jkk, ijk at r_arr = [r_i; r_j; r_k]."""
+    r9 = np.ascontiguousarray(r9, dtype=np.float64)
+    out = np.zeros(7 * 27, dtype=np.float64)
+    lib().fco_sw_blocks3(_p(r9), _p(params.as_array()), c0_fd, c0_iik, _p(out))
+    return out.reshape(7, 3, 3, 3, order="C").transpose(0, 3, 2, 1).copy()  # [blk,a,b,c]
+
+
+def sw_blocks2(r9, params, c0_fd=SW_C0_FD) -> np.ndarray:
+    r9 = np.ascontiguousarray(r9, dtype=np.float64)
+    out = np.zeros(27, dtype=np.float64)
+    lib().fco_sw_blocks2(_p(r9), _p(params.as_array()), c0_fd, _p(out))
+    return out.reshape(3, 3, 3, order="C").transpose(0, 2, 1).copy()  # [blk,a,b]
+
+
+def sw_pair_blocks(r3, params):
+    r3 = np.ascontiguousarray(r3, dtype=np.float64)
+    h2 = np.zeros(9)
+    h3 = np.zeros(27)
+    lib().fco_sw_pair_blocks(_p(r3), _p(params.as_array()), _p(h2), _p(h3))
+    return h2.reshape(3, 3, order="F"), h3.reshape(3, 3, 3, order="F")
+
+
+def sw_energy(sys, params, r_cut) -> float:
+    pos, box = _sysargs(sys)
+    return lib().fco_sw_energy(_p(pos), sys.n_atoms, _p(box), _p(params.as_array()), r_cut)
+
+
+def sw_third_order(sys, pot, calc, c0_fd=SW_C0_FD, c0_iik=SW_C0_IIK) -> np.ndarray:
+    pos, box = _sysargs(sys)
+    n = 3 * sys.n_atoms
+    out = np.zeros((n, n, n), dtype=np.float64, order="F")
+    rc = lib().fco_sw_ifc3(_p(pos), sys.n_atoms, _p(box), _p(pot.params.as_array()),
+                           calc.r_cut, calc.tol, c0_fd, c0_iik, _p(out))
+    if rc:
+        raise AssertionError("For AutoDiff SW silicon force constant cutoff must be less than potential cutoff")
+    return out
+
+
+def sw_second_order(sys, pot, calc, c0_fd=SW_C0_FD) -> np.ndarray:
+    pos, box = _sysargs(sys)
+    n = 3 * sys.n_atoms
+    out = np.zeros((n, n), dtype=np.float64, order="F")
+    rc = lib().fco_sw_ifc2(_p(pos), sys.n_atoms, _p(box), _p(pot.params.as_array()),
+                           calc.r_cut, calc.tol, c0_fd, _p(out))
+    if rc:
+        raise AssertionError("For SW silicon force constant cutoff must be less than potential cutoff")
+    return out
+
+
+def dynamical_matrix(ifc2: np.ndarray, masses) -> np.ndarray:
+    m = np.repeat(np.asarray(masses, dtype=np.float64), 3)
+    return ifc2 / np.sqrt(np.outer(m, m))
+
+
+def get_modes(dynmat: np.ndarray, num_rigid_translation: int = 3):
+    w2, phi = np.linalg.eigh(dynmat)
+    idx = np.argsort(np.abs(w2), kind="stable")
+    w2 = w2.copy()
+    w2[idx[:num_rigid_translation]] = 0.0
+    return w2, np.asfortranarray(phi)
+
+
+def mcc3_fp64(psi: np.ndarray, phi: np.ndarray) -> np.ndarray:
+    """K3[n,m,l] = sum_ijk Psi[i,j,k] phi[i,n] phi[j,m] phi[k,l] in FP64 (Fortran order)."""
+    P = np.asarray(psi, dtype=np.float64)
+    F = np.asarray(phi, dtype=np.float64)
+    n = P.shape[0]
+    # mode 1: K1[n,j,k] = sum_i phi[i,n] Psi[i,j,k]
+    K1 = (F.T @ P.reshape(n, n * n, order="F")).reshape(n, n, n, order="F")
+    # mode 2: K2[n,m,k] = sum_j K1[n,j,k] phi[j,m]
+    K2 = np.einsum("njk,jm->nmk", K1, F, optimize=True)
+    # mode 3: K3[n,m,l] = sum_k K2[n,m,k] phi[k,l]
+    K3 = (K2.reshape(n * n, n, order="F") @ F).reshape(n, n, n, order="F")
+    return np.asfortranarray(K3)
+
+
+def mcc3_fp64_slab(psi_sparse, phi: np.ndarray, l: int) -> np.ndarray:
+    """K3[:,:,l] in FP64 from a sparse Psi given as (rows, cols, slabs, values) COO,
+    for tensors too large to hold densely on the host."""
+    i, j, k, v = psi_sparse
+    F = np.asarray(phi, dtype=np.float64)
+    n = F.shape[0]
+    w = v * F[k, l]
+    import scipy.sparse as sp
+    M = sp.coo_matrix((w, (i, j)), shape=(n, n)).tocsr()
+    return F.T @ (M @ F)

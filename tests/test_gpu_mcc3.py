@@ -1,0 +1,104 @@
+"""mcc3 on the GPU against an FP64 host contraction (src/mcc.jl:15-17 definition).
+
+Bars (relative Frobenius error vs FP64): FP32 <= 2e-6 (plain FP32 accumulation
+over n-term sums), BF16X3 <= 1e-5.  Exact invariants: phi = I gives K3 = Psi and a
+permutation phi permutes Psi (FP32).
+"""
+import numpy as np
+import pytest
+
+import fc3
+from fc3 import systems as S
+from oracle import fc3_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 2e-6, "bf16x3": 1e-5}
+
+
+def rel_fro(a, b):
+    return float(np.linalg.norm((a.astype(np.float64) - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def random_case(n, seed):
+    rng = np.random.default_rng(seed)
+    psi = rng.standard_normal((n, n, n))
+    # symmetrise like a force-constant tensor
+    psi = (psi + psi.transpose(1, 0, 2) + psi.transpose(0, 2, 1) + psi.transpose(2, 1, 0) +
+           psi.transpose(1, 2, 0) + psi.transpose(2, 0, 1)) / 6
+    phi, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return np.asfortranarray(psi.astype(np.float32)), np.asfortranarray(phi.astype(np.float32))
+
+
+@pytest.mark.parametrize("n", [96, 200, 99, 256])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mcc3_random(ctx, n, prec):
+    psi, phi = random_case(n, 100 + n)
+    ref = O.mcc3_fp64(psi, phi)
+    dpsi = fc3.DeviceArray.from_host(psi, ctx=ctx)
+    dphi = fc3.DeviceArray.from_host(phi, ctx=ctx)
+    k3 = fc3.mcc3(dpsi, dphi, precision=prec).to_host()
+    err = rel_fro(k3, ref)
+    print(f"mcc3 n={n} {prec}: rel fro err {err:.3e}", flush=True)
+    assert err <= TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mcc3_inplace_and_blocked(ctx, prec):
+    n = 96
+    psi, phi = random_case(n, 7)
+    ref = O.mcc3_fp64(psi, phi)
+    dphi = fc3.DeviceArray.from_host(phi, ctx=ctx)
+    dpsi = fc3.DeviceArray.from_host(psi, ctx=ctx)
+    blocked = fc3.mcc3(dpsi, dphi, 32, precision=prec)      # host result, block_size = 32
+    assert isinstance(blocked, np.ndarray)
+    assert rel_fro(blocked, ref) <= TOL[prec]
+    fc3.mcc3_(dpsi, dphi, precision=prec)                   # mcc3!: overwrites Psi
+    assert rel_fro(dpsi.to_host(), ref) <= TOL[prec]
+
+
+def test_mcc3_block_size_assertion(ctx):
+    n = 96
+    psi, phi = random_case(n, 8)
+    dpsi = fc3.DeviceArray.from_host(psi, ctx=ctx)
+    dphi = fc3.DeviceArray.from_host(phi, ctx=ctx)
+    with pytest.raises(AssertionError):
+        fc3.mcc3(dpsi, dphi, 50)
+    with pytest.raises(AssertionError):
+        fc3.mcc3(dpsi, dphi, 0)
+
+
+def test_mcc3_identity_and_permutation_exact(ctx):
+    n = 132
+    psi, _ = random_case(n, 9)
+    dpsi = fc3.DeviceArray.from_host(psi, ctx=ctx)
+    eye = fc3.DeviceArray.from_host(np.eye(n, dtype=np.float32), ctx=ctx)
+    assert np.array_equal(fc3.mcc3(dpsi, eye, precision="fp32").to_host(), psi)
+    perm = np.random.default_rng(3).permutation(n)
+    P = np.zeros((n, n), dtype=np.float32)
+    P[perm, np.arange(n)] = 1.0   # phi[:, c] = e_{perm[c]}
+    k3 = fc3.mcc3(dpsi, fc3.DeviceArray.from_host(P, ctx=ctx), precision="fp32").to_host()
+    assert np.array_equal(k3, psi[np.ix_(perm, perm, perm)])
+    # BF16X3 with phi = I reproduces psi to the split's representation error
+    k3b = fc3.mcc3(dpsi, eye, precision="bf16x3").to_host()
+    assert rel_fro(k3b, psi.astype(np.float64)) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mcc3_lj_fcc222_modes(ctx, prec):
+    """The reference test's setting (runtests.jl:62-107): 96 modes of FCC 2x2x2 argon,
+    F3 ./= sqrt(m^3), phi from the dynamical matrix; the reference's criterion is
+    norm(K3 - K3_ref) <= 1e-6 (its TEP.jld2 data is not available, so K3_ref is the
+    FP64 contraction of the oracle's tensors)."""
+    sysd = S.fcc(S.ARGON_A, (2, 2, 2), S.ARGON_MASS)
+    calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
+    f3 = O.lj_third_order(sysd, S.ARGON_LJ, calc) / np.sqrt(S.ARGON_MASS ** 3)
+    f2 = O.lj_second_order(sysd, S.ARGON_LJ, calc)
+    _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+    ref = O.mcc3_fp64(f3.astype(np.float32), phi.astype(np.float32))
+    dpsi = fc3.DeviceArray.from_host(f3, dtype=np.float32, ctx=ctx)
+    dphi = fc3.DeviceArray.from_host(phi, dtype=np.float32, ctx=ctx)
+    k3 = fc3.mcc3(dpsi, dphi, precision=prec).to_host()
+    err = np.linalg.norm((k3.astype(np.float64) - ref).ravel())
+    print(f"fcc222 {prec}: |K3 - K3_64| = {err:.3e}, rel {rel_fro(k3, ref):.3e}", flush=True)
+    assert err <= 1e-6 or rel_fro(k3, ref) <= TOL[prec]

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "fc3_internal.hpp"
@@ -75,10 +76,11 @@ struct Prepared {
     double* masses = nullptr;
     NeighborList nl;
     char* extra = nullptr;
+    size_t extra_bytes = 0;
 };
 
 int prepare(fc3_handle h, const double* pos, int N, const double* box, double r_cut,
-            const double* masses, size_t extra_per_slot, size_t extra_fixed, Prepared& P)
+            const double* masses, const std::function<size_t(int)>& extra_bytes, Prepared& P)
 {
     // phase 1: counts (small workspace)
     const size_t head = Carve::need(sizeof(double) * 3 * N) + Carve::need(sizeof(double) * N) +
@@ -99,8 +101,8 @@ int prepare(fc3_handle h, const double* pos, int N, const double* box, double r_
     // phase 2: full workspace
     const int cap = P.nl.cap;
     const size_t slots = (size_t)N * cap;
-    const size_t total = head + 2 * Carve::need(sizeof(int) * slots) +
-                         Carve::need(extra_per_slot * slots) + Carve::need(extra_fixed) + 1024;
+    const size_t extra = extra_bytes(cap);
+    const size_t total = head + 2 * Carve::need(sizeof(int) * slots) + Carve::need(extra) + 1024;
     FC3_TRY(ensure_workspace(h, total));  // may move the workspace: re-upload below
     Carve cv(h->ws, h->ws_bytes);
     P.pos = cv.take<double>(3 * (size_t)N);
@@ -110,7 +112,8 @@ int prepare(fc3_handle h, const double* pos, int N, const double* box, double r_
     cv.take<int>(1);
     P.nl.nbr = cv.take<int>(slots);
     P.nl.rev = cv.take<int>(slots);
-    P.extra = cv.take<char>(extra_per_slot * slots + extra_fixed);
+    P.extra = cv.take<char>(extra);
+    P.extra_bytes = extra;
     FC3_HIP(hipMemcpyAsync(P.pos, pos, sizeof(double) * 3 * N, hipMemcpyHostToDevice, h->stream));
     if (masses)
         FC3_HIP(hipMemcpyAsync(P.masses, masses, sizeof(double) * N, hipMemcpyHostToDevice,
@@ -297,8 +300,9 @@ int fc3_second_order_lj(fc3_handle h, const double* positions, int n_atoms, cons
     const int N = n_atoms;
     Prepared P;
     const size_t per_slot = Carve::need(sizeof(double) * 27) + Carve::need(sizeof(double) * 9);
-    FC3_TRY(prepare(h, positions, N, box, r_cut, nullptr, sizeof(double) * (27 + 9),
-                    sizeof(double) * N * (27 + 9) + 2048, P));
+    FC3_TRY(prepare(h, positions, N, box, r_cut, nullptr, [&](int cap) {
+        return sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * (27 + 9)) + 4096;
+    }, P));
     (void)per_slot;
     const size_t slots = (size_t)N * P.nl.cap;
     double* E3 = reinterpret_cast<double*>(P.extra);
@@ -324,8 +328,9 @@ int fc3_third_order_lj(fc3_handle h, const double* positions, int n_atoms, const
                 return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
     const int N = n_atoms;
     Prepared P;
-    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, sizeof(double) * (27 + 9),
-                    sizeof(double) * N * (27 + 9) + 2048, P));
+    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
+        return sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * (27 + 9)) + 4096;
+    }, P));
     const size_t slots = (size_t)N * P.nl.cap;
     double* E3 = reinterpret_cast<double*>(P.extra);
     double* E2 = E3 + slots * 27;
@@ -358,20 +363,41 @@ int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int di
 int fc3_second_order_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
                         const double* sw_params, double r_cut, double tol, double* ifc2)
 {
-    (void)positions; (void)n_atoms; (void)box; (void)sw_params; (void)r_cut; (void)tol;
-    (void)ifc2;
     FC3_TRY(check_handle(h));
-    return set_error(FC3_ERR_UNSUPPORTED, "fc3_second_order_sw: not built yet");
+    FC3_TRY(check_system(positions, n_atoms, box));
+    if (!sw_params || !ifc2) return set_error(FC3_ERR_ARGUMENT, "sw_params, ifc2 must be non-null");
+    if (!(r_cut <= sw_params[2] * sw_params[1]))
+        return set_error(FC3_ERR_ARGUMENT,
+                         "For SW silicon force constant cutoff must be less than potential cutoff");
+    const int N = n_atoms;
+    Prepared P;
+    FC3_TRY(prepare(h, positions, N, box, r_cut, nullptr,
+                    [&](int cap) { return sw_workspace_bytes(N, cap, false); }, P));
+    return sw_assemble(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl,
+                       P.extra, P.extra_bytes, tol, nullptr, nullptr, nullptr, ifc2);
 }
 
 int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
                        const double* sw_params, double r_cut, double tol, const double* masses,
                        double* psi64, float* psi32)
 {
-    (void)positions; (void)n_atoms; (void)box; (void)sw_params; (void)r_cut; (void)tol;
-    (void)masses; (void)psi64; (void)psi32;
     FC3_TRY(check_handle(h));
-    return set_error(FC3_ERR_UNSUPPORTED, "fc3_third_order_sw: not built yet");
+    FC3_TRY(check_system(positions, n_atoms, box));
+    if (!sw_params) return set_error(FC3_ERR_ARGUMENT, "sw_params must be non-null");
+    if (!psi64 && !psi32) return set_error(FC3_ERR_ARGUMENT, "psi64 and psi32 are both null");
+    if (!(r_cut <= sw_params[2] * sw_params[1]))
+        return set_error(FC3_ERR_ARGUMENT,
+                         "For AutoDiff SW silicon force constant cutoff must be less than potential cutoff");
+    if (masses)
+        for (int i = 0; i < n_atoms; ++i)
+            if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    const int N = n_atoms;
+    Prepared P;
+    FC3_TRY(prepare(h, positions, N, box, r_cut, masses,
+                    [&](int cap) { return sw_workspace_bytes(N, cap, true); }, P));
+    return sw_assemble(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl,
+                       P.extra, P.extra_bytes, tol, masses ? P.masses : nullptr, psi64, psi32,
+                       nullptr);
 }
 
 int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precision, float* k3)

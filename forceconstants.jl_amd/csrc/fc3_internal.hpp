@@ -81,6 +81,7 @@ struct Carve {
         off += count * sizeof(T);
         return r;
     }
+    bool ok() const { return off <= cap; }
     static size_t need(size_t bytes) { return ((bytes + 255) & ~size_t(255)); }
 };
 
@@ -118,6 +119,15 @@ int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks&
                         double tol, const double* d_masses, double* psi64, float* psi32);
 
 int mass_weight3(hipStream_t s, double* psi, int N, const double* d_masses);
+
+// ---- Stillinger-Weber (sw.hip) ------------------------------------------------------
+// Workspace for one SW assembly with neighbour capacity `cap` (upper bound).
+size_t sw_workspace_bytes(int N, int cap, bool third);
+// IFC3 (psi64/psi32, either may be null) when third, else IFC2 into ifc2.  params: the
+// 10 SW parameters; c0_fd / c0_iik: cos(theta0) of the generated / runtime-built blocks.
+int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const double* params,
+                double c0_fd, double c0_iik, const NeighborList& nl, void* ws, size_t ws_bytes,
+                double tol, const double* d_masses, double* psi64, float* psi32, double* ifc2);
 
 // ---- mcc3 (mcc3.hip) --------------------------------------------------------------
 size_t mcc3_workspace_bytes(int n, int precision, bool inplace);

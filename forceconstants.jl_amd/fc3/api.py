@@ -150,7 +150,7 @@ def second_order_device(sys: SuperCellSystem, pot, calc, out: Optional[DeviceArr
         st = ctx.lib.fc3_second_order_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
                                          pot.epsilon, pot.r_cut, calc.r_cut, calc.tol, out.ptr)
     elif isinstance(pot, StillingerWeberSilicon) and isinstance(calc, AutoDiffCalculator):
-        prm = pot.params.as_array()
+        prm = pot.params.as_abi_array()
         st = ctx.lib.fc3_second_order_sw(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), _ptr(prm),
                                          calc.r_cut, calc.tol, out.ptr)
     else:
@@ -183,7 +183,7 @@ def third_order_device(sys: SuperCellSystem, pot, calc, psi64: Optional[DeviceAr
         st = ctx.lib.fc3_third_order_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
                                         pot.epsilon, pot.r_cut, calc.r_cut, calc.tol, mp, p64, p32)
     elif isinstance(pot, StillingerWeberSilicon) and isinstance(calc, AutoDiffCalculator):
-        prm = pot.params.as_array()
+        prm = pot.params.as_abi_array()
         st = ctx.lib.fc3_third_order_sw(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), _ptr(prm),
                                         calc.r_cut, calc.tol, mp, p64, p32)
     else:

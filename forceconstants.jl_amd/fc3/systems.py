@@ -95,6 +95,10 @@ class LJ:
     r_cut: float
 
 
+# cos(theta0) as baked into the reference's generated SW derivative blocks
+GENERATED_COS_THETA0 = -0.333333333333
+
+
 @dataclass(frozen=True)
 class SWParams:
     """SW_Params (SW.jl:3-14)."""
@@ -113,6 +117,16 @@ class SWParams:
     def as_array(self) -> np.ndarray:
         return np.array([self.epsilon, self.sigma, self.a, self.lam, self.gamma,
                          self.cos_theta0, self.A, self.B, self.p, self.q], dtype=np.float64)
+
+    def as_abi_array(self, generated_cos_theta0: Optional[float] = None) -> np.ndarray:
+        """The 11 doubles of the C ABI: the 10 parameters followed by the cos(theta0)
+        of the generated derivative blocks.  By default that is the constant the
+        reference's AutoDiffCalculator bakes in (SW/H3_exec_ijk.jl:39) when the
+        potential uses cos(theta0) = -1/3, and cos_theta0 otherwise."""
+        if generated_cos_theta0 is None:
+            generated_cos_theta0 = (GENERATED_COS_THETA0 if self.cos_theta0 == -1.0 / 3.0
+                                    else self.cos_theta0)
+        return np.append(self.as_array(), generated_cos_theta0)
 
 
 @dataclass(frozen=True)

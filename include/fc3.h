@@ -99,8 +99,11 @@ int fc3_second_order_lj(fc3_handle h, const double* positions, int n_atoms,
 
 /* second_order(sys, StillingerWeberSilicon(), AutoDiffCalculator(tol, r_cut))
  * replaces src/ifc.jl:3-18 -> src/autodiff/second_order_AD.jl:39-111.
- * sw_params: 10 doubles {epsilon, sigma, a, lambda, gamma, cos_theta0, A, B, p, q}
- * (SW.jl:3-14).                                                                 */
+ * sw_params: 11 doubles {epsilon, sigma, a, lambda, gamma, cos_theta0, A, B, p, q,
+ *            cos_theta0_generated}: the SW_Params of SW.jl:3-14 plus the cos(theta0)
+ *            that the reference's generated derivative blocks bake in (-0.333333333333,
+ *            generated_derivatives/SW/H3_exec_ijk.jl:39); the runtime-built iik block
+ *            uses cos_theta0.  Pass cos_theta0 twice for a self-consistent potential. */
 int fc3_second_order_sw(fc3_handle h, const double* positions, int n_atoms,
                         const double box[3], const double* sw_params, double r_cut,
                         double tol, double* ifc2);

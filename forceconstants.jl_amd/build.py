@@ -26,6 +26,8 @@ COMMON = [
     "-ffp-contract=off",
     # packed FP32 VALU instructions stay off on this hardware
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep private arrays in registers: LDS promotion would serialise the prefetch
+    "-mllvm", "-disable-promote-alloca-to-lds",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 

@@ -23,7 +23,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BK = 32;                // K padding granularity of the prepared phi
+constexpr int BK = 96;                // K padding granularity of the prepared phi (6 x 16)
 constexpr int kXcd = 8;               // XCDs on MI355X
 
 __device__ __forceinline__ void split_bf16(float x, __bf16& h, __bf16& l)
@@ -45,8 +45,7 @@ namespace v2 {
 
 constexpr int BM = 256, BN = 256, BK = 16;
 constexpr int NT = 256;
-constexpr int PH = BK + 8;   // bf16 pitch (48 B): conflict-free b128 fragment reads
-constexpr int PF = BK + 4;   // fp32 pitch (80 B)
+constexpr int PF = BK + 4;   // fp32 row pitch (80 B): conflict-free fragment reads
 
 struct G2 {
     const float* A;      // A(m, k) = A[m * sam + k * sak]
@@ -61,19 +60,42 @@ struct G2 {
     long per_xcd;
 };
 
-// A tile (BM x BK fp32) global -> registers, 16 values per thread.
-//   AMC (A is M-contiguous): rows 4 mq + i, k 4 kq + kk       -> v[4 i + kk]
-//   otherwise (K-contiguous): row t, k 4 q + kk                -> v[4 q + kk]
-// With VEC the loads are 16-byte and branch-free: out-of-range rows / k are
-// clamped to a valid address and zeroed afterwards (M % 4 == 0, K % 4 == 0).
-template <bool AMC, bool VEC>
+// BF16 operand tiles in LDS: each row (an A row or a B column) holds BK = 16 bf16,
+// two 16-byte chunks, unpadded.  Blocks of 4 rows fill the 32 banks once; inside a
+// block, row r sits in slot (r & 3) ^ (block & 3) and its chunks are swapped on odd
+// blocks.  This makes the MFMA fragment reads (8 consecutive rows, one chunk each),
+// the row-wise stores (8 rows x one chunk) and the transposing stores of an
+// M-contiguous A (4 rows that are 4 apart x 4 k-quads) all bank-conflict-free.
+// Returns the element offset of chunk c of row r.
+__device__ __forceinline__ int swz(int r, int c)
+{
+    const int b = r >> 2;
+    return 64 * b + 16 * ((r & 3) ^ (b & 3)) + 8 * (c ^ (b & 1));
+}
+
+// ---- global -> registers -------------------------------------------------------
+// A tile (BM x BK fp32), 16 values per thread:
+//   AMC (M-contiguous): thread (mq = t >> 2, kq = t & 3) holds rows 4 mq + i and
+//                       k = 4 kq + kk as v[4 i + kk];
+//   K-contiguous:       thread t holds row t, k = 4 q + kk as v[4 q + kk].
+// FAST: the tile is inside M and K needs no tail, so no checks.  Otherwise the
+// loads are clamped to valid addresses and zeroed (M % 4 == 0, K % 4 == 0 when
+// VEC) or fully scalar.
+template <bool AMC, bool VEC, bool FAST>
 __device__ __forceinline__ void load_a2(const G2& g, int m0, int k0, float (&v)[16])
 {
     const int t = threadIdx.x;
     if constexpr (AMC) {
-        const int mq = t & 63, kq = t >> 6;
+        const int mq = t >> 2, kq = t & 3;
         const int mr = m0 + 4 * mq, kc = k0 + 4 * kq;
-        if constexpr (VEC) {
+        if constexpr (FAST) {
+            const float* p = g.A + (long)kc * g.sak + mr;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const float4 x = *reinterpret_cast<const float4*>(p + (long)kk * g.sak);
+                v[0 + kk] = x.x; v[4 + kk] = x.y; v[8 + kk] = x.z; v[12 + kk] = x.w;
+            }
+        } else if constexpr (VEC) {
             const bool vm = mr < g.M;
             const int mcl = vm ? mr : g.M - 4;
 #pragma unroll
@@ -98,7 +120,14 @@ __device__ __forceinline__ void load_a2(const G2& g, int m0, int k0, float (&v)[
         }
     } else {
         const int m = m0 + t;
-        if constexpr (VEC) {
+        if constexpr (FAST) {
+            const float* p = g.A + (long)m * g.sam + k0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 x = *reinterpret_cast<const float4*>(p + 4 * q);
+                v[4 * q + 0] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+            }
+        } else if constexpr (VEC) {
             const bool vm = m < g.M;
             const long rowoff = (long)(vm ? m : g.M - 1) * g.sam;
 #pragma unroll
@@ -123,64 +152,259 @@ __device__ __forceinline__ void load_a2(const G2& g, int m0, int k0, float (&v)[
     }
 }
 
-// B tile of the prepared phi: 4 chunks of 16 bytes per thread, held in named
-// registers (an indexed array here would be placed in scratch memory, which makes
-// every load wait).  Always in bounds: phi is zero-padded to the tile grid and k0
-// past the end is clamped by the caller.
-struct BRegs {
-    uint4 r0, r1, r2, r3;
-};
+// B tile of the prepared phi: 4 chunks of 16 bytes per thread in named registers
+// (an indexed array would live in scratch memory and make every load wait).
+// Chunk idx = t + 256 it: BF16X3 -> array idx / 512 (hi, lo), column (idx % 512) / 2,
+// chunk idx % 2;  FP32 -> column idx / 4, quarter idx % 4.  Always in bounds (phi
+// is zero-padded to the tile grid; k0 past the end is clamped by the caller).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int PREC>
-__device__ __forceinline__ uint4 load_b_chunk(const G2& g, int n0, int k0, int idx)
-{
-    if constexpr (PREC == FC3_PREC_BF16X3) {
-        const int arr = idx >> 9, col = (idx & 511) >> 1, half = idx & 1;
-        const __bf16* base = static_cast<const __bf16*>(arr ? g.Blo : g.Bhi);
-        return *reinterpret_cast<const uint4*>(base + (long)(n0 + col) * g.ldb + k0 + 8 * half);
-    } else {
-        const int col = idx >> 2, q = idx & 3;
-        const float* base = static_cast<const float*>(g.Bhi);
-        return *reinterpret_cast<const uint4*>(base + (long)(n0 + col) * g.ldb + k0 + 4 * q);
-    }
-}
+struct BRegs {
+    u32x4 r0, r1, r2, r3;
+};
 
 template <int PREC>
 __device__ __forceinline__ void load_b2(const G2& g, int n0, int k0, BRegs& b)
 {
     const int t = threadIdx.x;
-    b.r0 = load_b_chunk<PREC>(g, n0, k0, t);
-    b.r1 = load_b_chunk<PREC>(g, n0, k0, t + NT);
-    b.r2 = load_b_chunk<PREC>(g, n0, k0, t + 2 * NT);
-    b.r3 = load_b_chunk<PREC>(g, n0, k0, t + 3 * NT);
+    if constexpr (PREC == FC3_PREC_BF16X3) {
+        const int col = t >> 1, c = t & 1;
+        const __bf16* ph = static_cast<const __bf16*>(g.Bhi) + (long)(n0 + col) * g.ldb + k0 + 8 * c;
+        const __bf16* pl = static_cast<const __bf16*>(g.Blo) + (long)(n0 + col) * g.ldb + k0 + 8 * c;
+        const long o = 128L * g.ldb;
+        b.r0 = *reinterpret_cast<const u32x4*>(ph);
+        b.r1 = *reinterpret_cast<const u32x4*>(ph + o);
+        b.r2 = *reinterpret_cast<const u32x4*>(pl);
+        b.r3 = *reinterpret_cast<const u32x4*>(pl + o);
+    } else {
+        const int col = t >> 2, q = t & 3;
+        const float* p = static_cast<const float*>(g.Bhi) + (long)(n0 + col) * g.ldb + k0 + 4 * q;
+        const long o = 64L * g.ldb;
+        b.r0 = *reinterpret_cast<const u32x4*>(p);
+        b.r1 = *reinterpret_cast<const u32x4*>(p + o);
+        b.r2 = *reinterpret_cast<const u32x4*>(p + 2 * o);
+        b.r3 = *reinterpret_cast<const u32x4*>(p + 3 * o);
+    }
 }
 
+// ---- registers -> LDS (BF16X3) -------------------------------------------------
 template <bool AMC>
-__device__ __forceinline__ void store_split(__bf16 (&sAh)[BM][PH], __bf16 (&sAl)[BM][PH],
-                                            const float (&v)[16])
+__device__ __forceinline__ void store_a_split(__bf16* sAh, __bf16* sAl, const float (&v)[16])
 {
     const int t = threadIdx.x;
+    if constexpr (AMC) {
+        const int mq = t >> 2, kq = t & 3;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        bf16x4 h, l;
+        for (int r = 0; r < 4; ++r) {
+            bf16x4 h, l;
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            __bf16 hh, ll;
-            split_bf16(v[4 * r + kk], hh, ll);
-            h[kk] = hh;
-            l[kk] = ll;
+            for (int kk = 0; kk < 4; ++kk) {
+                __bf16 hh, ll;
+                split_bf16(v[4 * r + kk], hh, ll);
+                h[kk] = hh;
+                l[kk] = ll;
+            }
+            const int off = swz(4 * mq + r, kq >> 1) + 4 * (kq & 1);
+            *reinterpret_cast<bf16x4*>(sAh + off) = h;
+            *reinterpret_cast<bf16x4*>(sAl + off) = l;
         }
-        if constexpr (AMC) {
-            const int mq = t & 63, kq = t >> 6;
-            *reinterpret_cast<bf16x4*>(&sAh[4 * mq + r][4 * kq]) = h;
-            *reinterpret_cast<bf16x4*>(&sAl[4 * mq + r][4 * kq]) = l;
-        } else {
-            *reinterpret_cast<bf16x4*>(&sAh[t][4 * r]) = h;
-            *reinterpret_cast<bf16x4*>(&sAl[t][4 * r]) = l;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            bf16x8 h, l;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                __bf16 hh, ll;
+                split_bf16(v[8 * c + kk], hh, ll);
+                h[kk] = hh;
+                l[kk] = ll;
+            }
+            const int off = swz(t, c);
+            *reinterpret_cast<bf16x8*>(sAh + off) = h;
+            *reinterpret_cast<bf16x8*>(sAl + off) = l;
         }
     }
 }
 
+__device__ __forceinline__ void store_b_bf16(__bf16* sBh, __bf16* sBl, const BRegs& b)
+{
+    const int t = threadIdx.x;
+    const int col = t >> 1, c = t & 1;
+    *reinterpret_cast<u32x4*>(sBh + swz(col, c)) = b.r0;
+    *reinterpret_cast<u32x4*>(sBh + swz(col + 128, c)) = b.r1;
+    *reinterpret_cast<u32x4*>(sBl + swz(col, c)) = b.r2;
+    *reinterpret_cast<u32x4*>(sBl + swz(col + 128, c)) = b.r3;
+}
+
+constexpr int TILE_E = BM * BK;   // bf16 elements of one operand tile (BM == BN)
+
+// The K loop, BF16X3.  Two LDS stages and a ring of three register stages for
+// the global prefetch, so a global load has three K steps to arrive.  In step k:
+//   read fragments of step k from stage k % 2
+//   MFMA (al bh), (ah bl)
+//   store registers (data of step k+1, loaded in step k-2) -> stage (k+1) % 2
+//   global load of step k+3 into the registers just freed
+//   MFMA (ah bh); barrier
+// The loop is unrolled by 6 so that stage and register indices are static; K is
+// padded to a multiple of 6 BK (zero-padded phi, A zeroed by its bounds check),
+// and prefetches past the end are clamped to the last step and never consumed.
+constexpr int KSTEP_MULT = 6;
+
+template <bool AMC, bool VEC, bool FAST>
+__device__ __forceinline__ void kloop_bf16(const G2& g, int m0, int n0, __bf16 (*s)[4][TILE_E],
+                                           f32x16 (&acc)[4][4])
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm0 = (wave & 1) * 128, wn0 = (wave >> 1) * 128;
+    const int nk = KSTEP_MULT * ((g.K + KSTEP_MULT * BK - 1) / (KSTEP_MULT * BK));
+    const int klast = (nk - 1) * BK;
+    float av0[16], av1[16], av2[16];
+    BRegs bv0, bv1, bv2;
+    int offa[4], offb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        offa[i] = swz(wm0 + i * 32 + (lane & 31), lane >> 5);
+        offb[i] = swz(wn0 + i * 32 + (lane & 31), lane >> 5);
+    }
+#define FC3_KN(x) ((x) < klast ? (x) : klast)
+    // Issue order inside one step (one wave per SIMD issues at most one instruction
+    // per slot, so MFMAs must be spread between the other instructions): the first
+    // fragments, then each remaining fragment read, split VALU, LDS store and global
+    // load placed between MFMAs.
+#define FC3_SGB(mask, n) __builtin_amdgcn_sched_group_barrier(mask, n, 0);
+#define FC3_INTERLEAVE()                                                                      \
+    FC3_SGB(0x100, 6)                                                                         \
+    _Pragma("unroll") for (int q = 0; q < 10; ++q) { FC3_SGB(0x008, 1) FC3_SGB(0x100, 1) }     \
+    _Pragma("unroll") for (int q = 0; q < 12; ++q) {                                          \
+        FC3_SGB(0x008, 1) FC3_SGB(0x002, 4) FC3_SGB(0x200, 1) }                               \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                           \
+        FC3_SGB(0x008, 1) FC3_SGB(0x020, 1) FC3_SGB(0x002, 3) }                               \
+    _Pragma("unroll") for (int q = 0; q < 18; ++q) { FC3_SGB(0x008, 1) FC3_SGB(0x002, 2) }
+#define FC3_MMA(X, Y)                                                                         \
+    _Pragma("unroll") for (int i = 0; i < 4; ++i)                                             \
+        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                         \
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[i], Y[j], acc[i][j], 0, 0, 0);
+#define FC3_STEP(cur, avs, bvs, knext)                                                        \
+    {                                                                                         \
+        bf16x8 ah[4], al[4], bh[4], bl[4];                                                    \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
+            ah[i] = *reinterpret_cast<const bf16x8*>(&s[cur][0][offa[i]]);                    \
+            al[i] = *reinterpret_cast<const bf16x8*>(&s[cur][1][offa[i]]);                    \
+            bh[i] = *reinterpret_cast<const bf16x8*>(&s[cur][2][offb[i]]);                    \
+            bl[i] = *reinterpret_cast<const bf16x8*>(&s[cur][3][offb[i]]);                    \
+        }                                                                                     \
+        FC3_MMA(al, bh)                                                                       \
+        FC3_MMA(ah, bl)                                                                       \
+        store_a_split<AMC>(s[(cur) ^ 1][0], s[(cur) ^ 1][1], avs);                           \
+        store_b_bf16(s[(cur) ^ 1][2], s[(cur) ^ 1][3], bvs);                                 \
+        load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(knext), avs);                                  \
+        load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(knext), bvs);                                  \
+        FC3_MMA(ah, bh)                                                                       \
+        FC3_INTERLEAVE()                                                                      \
+        __syncthreads();                                                                      \
+    }
+    // prologue: stage 0 <- step 0; registers: ring slot r holds step 1 + ((k - 1 - r) mod 3)
+    load_a2<AMC, VEC, FAST>(g, m0, 0, av0);
+    load_b2<FC3_PREC_BF16X3>(g, n0, 0, bv0);
+    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(BK), av1);
+    load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(BK), bv1);
+    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(2 * BK), av2);
+    load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(2 * BK), bv2);
+    store_a_split<AMC>(s[0][0], s[0][1], av0);
+    store_b_bf16(s[0][2], s[0][3], bv0);
+    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(3 * BK), av0);
+    load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(3 * BK), bv0);
+    __syncthreads();
+    // now av1 = step 1, av2 = step 2, av0 = step 3
+    for (int kt = 0; kt < nk; kt += KSTEP_MULT) {
+        FC3_STEP(0, av1, bv1, (kt + 4) * BK)   // k = kt    : stage 1 <- k+1, fetch k+4
+        FC3_STEP(1, av2, bv2, (kt + 5) * BK)   // k = kt + 1: stage 0 <- k+1 (= kt+2)
+        FC3_STEP(0, av0, bv0, (kt + 6) * BK)   // k = kt + 2
+        FC3_STEP(1, av1, bv1, (kt + 7) * BK)   // k = kt + 3
+        FC3_STEP(0, av2, bv2, (kt + 8) * BK)   // k = kt + 4
+        FC3_STEP(1, av0, bv0, (kt + 9) * BK)   // k = kt + 5
+    }
+#undef FC3_STEP
+#undef FC3_MMA
+#undef FC3_INTERLEAVE
+#undef FC3_SGB
+#undef FC3_KN
+}
+
+// The K loop, FP32 (same pipeline; padded row-major LDS tiles).
+template <bool AMC, bool VEC, bool FAST>
+__device__ __forceinline__ void kloop_fp32(const G2& g, int m0, int n0, float (*sA)[BM][PF],
+                                           float (*sB)[BN][PF], f32x16 (&acc)[4][4])
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm0 = (wave & 1) * 128, wn0 = (wave >> 1) * 128;
+    const int nk = 2 * ((g.K + 2 * BK - 1) / (2 * BK));
+    const int klast = (nk - 1) * BK;
+    float av0[16], av1[16];
+    BRegs bv0, bv1;
+    // MFMA step s, lane group h = lane / 32 uses k = 8 h + s (8 contiguous floats)
+    const int kof = (lane >> 5) * 8;
+#define FC3_KN(x) ((x) < klast ? (x) : klast)
+#define FC3_STORE(st, avs, bvs)                                                               \
+    {                                                                                         \
+        _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                       \
+            const float4 x = make_float4(avs[4 * r], avs[4 * r + 1], avs[4 * r + 2], avs[4 * r + 3]); \
+            if (AMC) *reinterpret_cast<float4*>(&sA[st][4 * (t >> 2) + r][4 * (t & 3)]) = x;  \
+            else *reinterpret_cast<float4*>(&sA[st][t][4 * r]) = x;                           \
+        }                                                                                     \
+        *reinterpret_cast<u32x4*>(&sB[st][t >> 2][4 * (t & 3)]) = bvs.r0;                     \
+        *reinterpret_cast<u32x4*>(&sB[st][(t >> 2) + 64][4 * (t & 3)]) = bvs.r1;              \
+        *reinterpret_cast<u32x4*>(&sB[st][(t >> 2) + 128][4 * (t & 3)]) = bvs.r2;             \
+        *reinterpret_cast<u32x4*>(&sB[st][(t >> 2) + 192][4 * (t & 3)]) = bvs.r3;             \
+    }
+#define FC3_STEPF(cur, avs, bvs, knext)                                                       \
+    {                                                                                         \
+        float fa[4][8], fb[4][8];                                                             \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
+            const int r = wm0 + i * 32 + (lane & 31);                                         \
+            const float4 x0 = *reinterpret_cast<const float4*>(&sA[cur][r][kof]);             \
+            const float4 x1 = *reinterpret_cast<const float4*>(&sA[cur][r][kof + 4]);         \
+            fa[i][0] = x0.x; fa[i][1] = x0.y; fa[i][2] = x0.z; fa[i][3] = x0.w;               \
+            fa[i][4] = x1.x; fa[i][5] = x1.y; fa[i][6] = x1.z; fa[i][7] = x1.w;               \
+            const int c = wn0 + i * 32 + (lane & 31);                                         \
+            const float4 y0 = *reinterpret_cast<const float4*>(&sB[cur][c][kof]);             \
+            const float4 y1 = *reinterpret_cast<const float4*>(&sB[cur][c][kof + 4]);         \
+            fb[i][0] = y0.x; fb[i][1] = y0.y; fb[i][2] = y0.z; fb[i][3] = y0.w;               \
+            fb[i][4] = y1.x; fb[i][5] = y1.y; fb[i][6] = y1.z; fb[i][7] = y1.w;               \
+        }                                                                                     \
+        _Pragma("unroll") for (int s2 = 0; s2 < 4; ++s2)                                      \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
+                _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s2], fb[j][s2], acc[i][j], 0, 0, 0); \
+        FC3_STORE((cur) ^ 1, avs, bvs)                                                        \
+        load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(knext), avs);                                  \
+        load_b2<FC3_PREC_FP32>(g, n0, FC3_KN(knext), bvs);                                    \
+        _Pragma("unroll") for (int s2 = 4; s2 < 8; ++s2)                                      \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
+                _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s2], fb[j][s2], acc[i][j], 0, 0, 0); \
+        __syncthreads();                                                                      \
+    }
+    load_a2<AMC, VEC, FAST>(g, m0, 0, av0);
+    load_b2<FC3_PREC_FP32>(g, n0, 0, bv0);
+    FC3_STORE(0, av0, bv0)
+    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(BK), av1);
+    load_b2<FC3_PREC_FP32>(g, n0, FC3_KN(BK), bv1);
+    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(2 * BK), av0);
+    load_b2<FC3_PREC_FP32>(g, n0, FC3_KN(2 * BK), bv0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt += 2) {
+        FC3_STEPF(0, av1, bv1, (kt + 3) * BK)
+        FC3_STEPF(1, av0, bv0, (kt + 4) * BK)
+    }
+#undef FC3_STEPF
+#undef FC3_STORE
+#undef FC3_KN
+}
+
+// 256 x 256 macro tile, 4 waves (2 x 2), each a 128 x 128 tile of 4 x 4 MFMA 32x32
+// blocks; XCD-aware tile order.  Interior tiles (inside M, K a multiple of 2 BK)
+// take the unchecked load path.
 template <int PREC, bool AMC, bool VEC>
 __global__ void __launch_bounds__(NT, 1) k_gemm2(G2 g)
 {
@@ -200,141 +424,18 @@ __global__ void __launch_bounds__(NT, 1) k_gemm2(G2 g)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    // K is walked in pairs of steps so that the two register stages alternate
-    // without copies (a copy at the loop back-edge would wait on the loads in
-    // flight).  Steps past K read zero-padded phi (padded to a multiple of 2 BK)
-    // and A zeroed by its bounds check, so they contribute 0.  Prefetches beyond
-    // the last step are clamped to it and never consumed.
-    const int nk = 2 * ((g.K + 2 * BK - 1) / (2 * BK));
-    const int klast = (nk - 1) * BK;
-    float av0[16], av1[16];
-    BRegs bv0, bv1;
-#define FC3_KN(x) ((x) < klast ? (x) : klast)
-
     if constexpr (PREC == FC3_PREC_BF16X3) {
-        __shared__ __attribute__((aligned(16))) __bf16 sAh[2][BM][PH];
-        __shared__ __attribute__((aligned(16))) __bf16 sAl[2][BM][PH];
-        __shared__ __attribute__((aligned(16))) __bf16 sBh[2][BN][PH];
-        __shared__ __attribute__((aligned(16))) __bf16 sBl[2][BN][PH];
-        // chunk idx = t + 256 it: array idx / 512 (hi, lo), column (idx % 512) / 2, half idx % 2
-#define FC3_STORE_B(st, bv)                                                                   \
-        {                                                                                     \
-            const int col = (t & 511) >> 1, half = t & 1;                                     \
-            *reinterpret_cast<uint4*>(&sBh[st][col][8 * half]) = bv.r0;                       \
-            *reinterpret_cast<uint4*>(&sBh[st][col + 128][8 * half]) = bv.r1;                 \
-            *reinterpret_cast<uint4*>(&sBl[st][col][8 * half]) = bv.r2;                       \
-            *reinterpret_cast<uint4*>(&sBl[st][col + 128][8 * half]) = bv.r3;                 \
-        }
-        const int kof = (lane >> 5) * 8;
-        // one K step on LDS stage `cur`: 48 MFMAs; meanwhile the registers (avs, bvs)
-        // of the next step go to the other stage and the step after next is fetched
-        // into (avs, bvs).
-#define FC3_STEP(cur, avs, bvs, knext)                                                        \
-        {                                                                                     \
-            bf16x8 ah[4], al[4], bh[4], bl[4];                                                \
-            _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                   \
-                const int r = wm0 + i * 32 + (lane & 31);                                     \
-                ah[i] = *reinterpret_cast<const bf16x8*>(&sAh[cur][r][kof]);                  \
-                al[i] = *reinterpret_cast<const bf16x8*>(&sAl[cur][r][kof]);                  \
-            }                                                                                 \
-            _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                   \
-                const int c = wn0 + j * 32 + (lane & 31);                                     \
-                bh[j] = *reinterpret_cast<const bf16x8*>(&sBh[cur][c][kof]);                  \
-                bl[j] = *reinterpret_cast<const bf16x8*>(&sBl[cur][c][kof]);                  \
-            }                                                                                 \
-            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
-                _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0); \
-            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
-                _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0); \
-            store_split<AMC>(sAh[(cur) ^ 1], sAl[(cur) ^ 1], avs);                           \
-            FC3_STORE_B((cur) ^ 1, bvs)                                                       \
-            load_a2<AMC, VEC>(g, m0, FC3_KN(knext), avs);                                     \
-            load_b2<PREC>(g, n0, FC3_KN(knext), bvs);                                         \
-            _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
-                _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0); \
-            __syncthreads();                                                                  \
-        }
-        load_a2<AMC, VEC>(g, m0, 0, av0);
-        load_b2<PREC>(g, n0, 0, bv0);
-        store_split<AMC>(sAh[0], sAl[0], av0);
-        FC3_STORE_B(0, bv0)
-        load_a2<AMC, VEC>(g, m0, FC3_KN(BK), av1);
-        load_b2<PREC>(g, n0, FC3_KN(BK), bv1);
-        load_a2<AMC, VEC>(g, m0, FC3_KN(2 * BK), av0);
-        load_b2<PREC>(g, n0, FC3_KN(2 * BK), bv0);
-        __syncthreads();
-        for (int kt = 0; kt < nk; kt += 2) {
-            // even step: stage 0; stage 1 <- (av1, bv1); fetch kt+3 into (av1, bv1)
-            FC3_STEP(0, av1, bv1, (kt + 3) * BK)
-            // odd step: stage 1; stage 0 <- (av0, bv0); fetch kt+4 into (av0, bv0)
-            FC3_STEP(1, av0, bv0, (kt + 4) * BK)
-        }
-#undef FC3_STEP
-#undef FC3_STORE_B
+        const bool fast = VEC && (m0 + BM <= g.M) && (g.K % (KSTEP_MULT * BK) == 0);
+        __shared__ __attribute__((aligned(16))) __bf16 s[2][4][TILE_E];
+        if (fast) kloop_bf16<AMC, VEC, VEC>(g, m0, n0, s, acc);
+        else kloop_bf16<AMC, VEC, false>(g, m0, n0, s, acc);
     } else {
+        const bool fast = VEC && (m0 + BM <= g.M) && (g.K % (2 * BK) == 0);
         __shared__ __attribute__((aligned(16))) float sA[2][BM][PF];
         __shared__ __attribute__((aligned(16))) float sB[2][BN][PF];
-#define FC3_STORE_AF(st, avs, bvs)                                                            \
-        _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                       \
-            const float4 x = make_float4(avs[4 * r], avs[4 * r + 1], avs[4 * r + 2], avs[4 * r + 3]); \
-            if (AMC) *reinterpret_cast<float4*>(&sA[st][4 * (t & 63) + r][4 * (t >> 6)]) = x; \
-            else *reinterpret_cast<float4*>(&sA[st][t][4 * r]) = x;                           \
-        }                                                                                     \
-        *reinterpret_cast<uint4*>(&sB[st][t >> 2][4 * (t & 3)]) = bvs.r0;                     \
-        *reinterpret_cast<uint4*>(&sB[st][(t >> 2) + 64][4 * (t & 3)]) = bvs.r1;              \
-        *reinterpret_cast<uint4*>(&sB[st][(t >> 2) + 128][4 * (t & 3)]) = bvs.r2;             \
-        *reinterpret_cast<uint4*>(&sB[st][(t >> 2) + 192][4 * (t & 3)]) = bvs.r3;
-        // MFMA step s, lane group h = lane / 32 uses k = 8 h + s (8 contiguous floats)
-        const int kof = (lane >> 5) * 8;
-#define FC3_STEPF(cur, avs, bvs, knext)                                                       \
-        {                                                                                     \
-            float fa[4][8], fb[4][8];                                                         \
-            _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                   \
-                const int r = wm0 + i * 32 + (lane & 31);                                     \
-                const float4 x0 = *reinterpret_cast<const float4*>(&sA[cur][r][kof]);         \
-                const float4 x1 = *reinterpret_cast<const float4*>(&sA[cur][r][kof + 4]);     \
-                fa[i][0] = x0.x; fa[i][1] = x0.y; fa[i][2] = x0.z; fa[i][3] = x0.w;           \
-                fa[i][4] = x1.x; fa[i][5] = x1.y; fa[i][6] = x1.z; fa[i][7] = x1.w;           \
-            }                                                                                 \
-            _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                   \
-                const int c = wn0 + j * 32 + (lane & 31);                                     \
-                const float4 x0 = *reinterpret_cast<const float4*>(&sB[cur][c][kof]);         \
-                const float4 x1 = *reinterpret_cast<const float4*>(&sB[cur][c][kof + 4]);     \
-                fb[j][0] = x0.x; fb[j][1] = x0.y; fb[j][2] = x0.z; fb[j][3] = x0.w;           \
-                fb[j][4] = x1.x; fb[j][5] = x1.y; fb[j][6] = x1.z; fb[j][7] = x1.w;           \
-            }                                                                                 \
-            _Pragma("unroll") for (int s2 = 0; s2 < 4; ++s2)                                  \
-                _Pragma("unroll") for (int i = 0; i < 4; ++i)                                 \
-                    _Pragma("unroll") for (int j = 0; j < 4; ++j)                             \
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s2], fb[j][s2], acc[i][j], 0, 0, 0); \
-            FC3_STORE_AF((cur) ^ 1, avs, bvs)                                                 \
-            load_a2<AMC, VEC>(g, m0, FC3_KN(knext), avs);                                     \
-            load_b2<PREC>(g, n0, FC3_KN(knext), bvs);                                         \
-            _Pragma("unroll") for (int s2 = 4; s2 < 8; ++s2)                                  \
-                _Pragma("unroll") for (int i = 0; i < 4; ++i)                                 \
-                    _Pragma("unroll") for (int j = 0; j < 4; ++j)                             \
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s2], fb[j][s2], acc[i][j], 0, 0, 0); \
-            __syncthreads();                                                                  \
-        }
-        load_a2<AMC, VEC>(g, m0, 0, av0);
-        load_b2<PREC>(g, n0, 0, bv0);
-        FC3_STORE_AF(0, av0, bv0)
-        load_a2<AMC, VEC>(g, m0, FC3_KN(BK), av1);
-        load_b2<PREC>(g, n0, FC3_KN(BK), bv1);
-        load_a2<AMC, VEC>(g, m0, FC3_KN(2 * BK), av0);
-        load_b2<PREC>(g, n0, FC3_KN(2 * BK), bv0);
-        __syncthreads();
-        for (int kt = 0; kt < nk; kt += 2) {
-            FC3_STEPF(0, av1, bv1, (kt + 3) * BK)
-            FC3_STEPF(1, av0, bv0, (kt + 4) * BK)
-        }
-#undef FC3_STEPF
-#undef FC3_STORE_AF
+        if (fast) kloop_fp32<AMC, VEC, VEC>(g, m0, n0, sA, sB, acc);
+        else kloop_fp32<AMC, VEC, false>(g, m0, n0, sA, sB, acc);
     }
-#undef FC3_KN
 
     // epilogue: register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
 #pragma unroll

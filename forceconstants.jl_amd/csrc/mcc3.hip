@@ -468,6 +468,284 @@ __global__ void __launch_bounds__(NT, 1) k_gemm2(G2 g)
 
 }  // namespace v2
 
+// ---------------------------------------------------------------------------------
+// GEMM v3 (BF16X3 only): both operands arrive already split into bf16 hi/lo planes in
+// a K-blocked layout: K is cut into blocks of BK = 16, and inside a block the rows
+// follow each other at 32 bytes (row r, k-block kb at kb * KS + 16 r).  One K step of
+// a 256-row tile is then 8 KB of contiguous memory, so the K loop is global -> LDS
+// copies (global_load_lds, 16 bytes per lane, each instruction one contiguous KB) with
+// no registers, VALU or LDS stores in between, then fragment reads and MFMAs.  Four
+// LDS stages in a ring; the copies for step k+3 are issued in step k into the stage
+// read in step k-1.  The epilogue writes C as FP32, or split again into bf16 hi/lo
+// planes in the K-blocked layout the next mode product reads.
+namespace v3 {
+
+using v2::BM;
+using v2::BN;
+using v2::BK;
+using v2::NT;
+using v2::TILE_E;
+constexpr int NSTAGE = 4;
+
+// Operand and output addressing (element offsets; lo planes at + a_lo, b_lo, c_lo):
+//   A(m, k) = A[(m / a_R) a_SS + (k / 16) a_KS + 16 (m % a_R) + k % 16]
+//   B(k, c) = B[(k / 16) b_KS + 16 c + k % 16]
+//   C(m, c) = C[(m / c_R) c_SS + ((m % c_R) / 16) c_KS + (m % c_R) % 16 + c c_CS]
+// (c_KS = 16 makes C plain with rows contiguous inside each group of c_R rows.)
+struct G3 {
+    const __bf16* A;
+    long a_lo, a_SS, a_KS;
+    const __bf16* B;
+    long b_lo, b_KS;
+    void* C;
+    long c_SS, c_KS, c_CS, c_lo;
+    int a_R, c_R;
+    int M, N, K;         // K % BK == 0, M % 4 == 0, c_R % 4 == 0
+    int tiles_m, tiles_n;
+    long per_xcd;
+};
+
+// Workgroup barrier for the copy ring: s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding,
+// expcnt unconstrained), then s_barrier.  After it, every wave's copies older than its
+// last N are in LDS and every wave's fragment reads issued before it have returned.
+// The signal fences keep the compiler from moving memory accesses across; unlike
+// __syncthreads() there is no release fence, which would wait for all copies in flight.
+template <int N>
+__device__ __forceinline__ void ring_barrier()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | (0x7 << 4));
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// s_waitcnt vmcnt(0), other counters unconstrained
+__device__ __forceinline__ void wait_vmcnt0() { __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xf << 8)); }
+
+// Operand tile layout in LDS: row r (an A row or a B column) holds BK = 16 bf16 as two
+// 16-byte chunks c = 0, 1; chunk (r, c) sits at 16-byte slot
+//   32 (r >> 4) + 16 c + (r & 15).
+// A 64-lane ds_read_b128 is served 16 lanes per cycle from 64 four-byte banks; the
+// fragment reads give each group of 16 lanes 16 consecutive rows of one chunk, which
+// land in 16 distinct slots modulo 16, i.e. on all 64 banks once.  The copies write
+// 64 consecutive slots per instruction.
+__device__ __forceinline__ int lds_chunk(int r, int c) { return 32 * (r >> 4) + 16 * c + (r & 15); }
+
+__device__ __forceinline__ void lds_copy16(const __bf16* g, __bf16* lds)
+{
+    __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(g), reinterpret_cast<void*>(lds),
+                                     16, 0, 0);
+}
+
+template <bool SPLIT_OUT>
+__global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
+{
+    const long wg = blockIdx.x;
+    const long tile = (wg % kXcd) * g.per_xcd + wg / kXcd;
+    if (tile >= (long)g.tiles_m * g.tiles_n) return;
+    const int tm = (int)(tile / g.tiles_n), tn = (int)(tile - (long)tm * g.tiles_n);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm0 = (wave & 1) * 128, wn0 = (wave >> 1) * 128;
+
+    // One LDS object per stage: the compiler then sees that the fragment reads of one
+    // stage do not alias the copies in flight into the others, and adds no wait.
+    __shared__ __attribute__((aligned(16))) __bf16 s0[4 * TILE_E];
+    __shared__ __attribute__((aligned(16))) __bf16 s1[4 * TILE_E];
+    __shared__ __attribute__((aligned(16))) __bf16 s2[4 * TILE_E];
+    __shared__ __attribute__((aligned(16))) __bf16 s3[4 * TILE_E];
+
+    // Copy plan: wave w fills array w (0 A hi, 1 A lo, 2 B hi, 3 B lo) of a stage with
+    // 8 copies of 64 chunks; chunk P = 64 q + lane of the tile (layout: lds_chunk)
+    // holds row r = 16 (P >> 5) + (P & 15), k-half c = (P >> 4) & 1.  Copy q covers
+    // rows 32 q .. 32 q + 31, one contiguous KB of the K block (one slab permitting).
+    const __bf16* src[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int P = 64 * q + lane;
+        const int r = 16 * (P >> 5) + (P & 15);
+        const int c = (P >> 4) & 1;
+        if (wave < 2) {
+            int m = m0 + r;
+            if (m >= g.M) m = g.M - 1;        // rows past M feed only discarded rows of C
+            const unsigned sl = (unsigned)m / (unsigned)g.a_R;
+            const long arow = (long)sl * g.a_SS + 16L * (m - (int)sl * g.a_R);
+            src[q] = g.A + arow + (wave ? g.a_lo : 0) + 8 * c;
+        } else {
+            src[q] = g.B + (wave == 3 ? g.b_lo : 0) + 16L * (n0 + r) + 8 * c;
+        }
+    }
+    const long kstride = wave < 2 ? g.a_KS : g.b_KS;   // elements per K block
+    const int nk = g.K / BK;
+    const int wofs = wave * TILE_E;
+    // this wave's 8 copies of step k into stage array `st`; steps past the end copy the
+    // last step again (into a stage that is not read any more), so that every step has
+    // the same number of copies in flight
+    auto issue = [&](__bf16* st, int k) {
+        const long kk = (k < nk ? k : nk - 1) * kstride;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lds_copy16(src[q] + kk, st + wofs + 512 * q);
+    };
+
+    f32x16 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    int offa[4], offb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        offa[i] = 8 * lds_chunk(wm0 + i * 32 + (lane & 31), lane >> 5);
+        offb[i] = 8 * lds_chunk(wn0 + i * 32 + (lane & 31), lane >> 5);
+    }
+
+#define FC3_MMA3(X, Y)                                                                      \
+    _Pragma("unroll") for (int i = 0; i < 4; ++i)                                           \
+        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                       \
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[i], Y[j], acc[i][j], 0, 0, 0);
+    // Step K reads stage SRD and refills stage SWR (read in step K-1, retired by the
+    // barrier that ended it) with step K+3.  The closing barrier leaves the copies of
+    // steps K+2 and K+3 in flight and those of K+1 in LDS.
+#define FC3_STEP3(K, SRD, SWR)                                                              \
+    {                                                                                       \
+        bf16x8 ah[4], al[4], bh[4], bl[4];                                                  \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                     \
+            ah[i] = *reinterpret_cast<const bf16x8*>(SRD + offa[i]);                        \
+            al[i] = *reinterpret_cast<const bf16x8*>(SRD + TILE_E + offa[i]);               \
+            bh[i] = *reinterpret_cast<const bf16x8*>(SRD + 2 * TILE_E + offb[i]);           \
+            bl[i] = *reinterpret_cast<const bf16x8*>(SRD + 3 * TILE_E + offb[i]);           \
+        }                                                                                   \
+        FC3_MMA3(al, bh)                                                                    \
+        issue(SWR, (K) + 3);                                                                \
+        FC3_MMA3(ah, bl)                                                                    \
+        FC3_MMA3(ah, bh)                                                                    \
+        ring_barrier<16>();                                                                 \
+    }
+
+    // prologue: steps 0..2 in flight, step 0 landed
+    issue(s0, 0);
+    issue(s1, 1);
+    issue(s2, 2);
+    ring_barrier<16>();
+    int k = 0;
+    for (; k + 4 <= nk; k += 4) {
+        FC3_STEP3(k, s0, s3)
+        FC3_STEP3(k + 1, s1, s0)
+        FC3_STEP3(k + 2, s2, s1)
+        FC3_STEP3(k + 3, s3, s2)
+    }
+    if (k < nk) FC3_STEP3(k, s0, s3)
+    if (k + 1 < nk) FC3_STEP3(k + 1, s1, s0)
+    if (k + 2 < nk) FC3_STEP3(k + 2, s2, s1)
+#undef FC3_STEP3
+#undef FC3_MMA3
+    wait_vmcnt0();   // no copy may still be landing when the workgroup's LDS is released
+
+    // epilogue: register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn0 + j * 32 + (lane & 31);
+            if (col >= g.N) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = m0 + wm0 + i * 32 + q * 8 + (lane >> 5) * 4;
+                if (row >= g.M) continue;   // M % 4 == 0: the 4 rows are all valid
+                const unsigned sl = (unsigned)row / (unsigned)g.c_R;
+                const int rr = row - (int)sl * g.c_R;   // rr % 4 == 0: the 4 rows share a K block
+                const long off = (long)sl * g.c_SS + (long)(rr >> 4) * g.c_KS + (rr & 15) +
+                                 (long)col * g.c_CS;
+                const float x0 = acc[i][j][4 * q], x1 = acc[i][j][4 * q + 1];
+                const float x2 = acc[i][j][4 * q + 2], x3 = acc[i][j][4 * q + 3];
+                if constexpr (SPLIT_OUT) {
+                    __bf16* C = static_cast<__bf16*>(g.C);
+                    bf16x4 h, l;
+                    __bf16 hh, ll;
+                    split_bf16(x0, hh, ll); h[0] = hh; l[0] = ll;
+                    split_bf16(x1, hh, ll); h[1] = hh; l[1] = ll;
+                    split_bf16(x2, hh, ll); h[2] = hh; l[2] = ll;
+                    split_bf16(x3, hh, ll); h[3] = hh; l[3] = ll;
+                    *reinterpret_cast<bf16x4*>(C + off) = h;
+                    *reinterpret_cast<bf16x4*>(C + off + g.c_lo) = l;
+                } else {
+                    float* C = static_cast<float*>(g.C);
+                    *reinterpret_cast<float4*>(C + off) = make_float4(x0, x1, x2, x3);
+                }
+            }
+        }
+}
+
+// Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K, K % 16 == 0) -> the
+// K-blocked A operand: X[(k / 16) 16 rows + 16 (r - r0) + k % 16] (hi), lo at
+// + rows K.  64 rows x 64 k per block through LDS; each thread then writes 4
+// consecutive k of one row, so a K block's 64 rows go out as 2 KB contiguous.
+__global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict__ psi, long n2,
+                                                          int K, long r0, long rows,
+                                                          __bf16* __restrict__ X)
+{
+    __shared__ float tile[64][65];
+    const long rb = (long)blockIdx.x * 64;
+    const int kb = blockIdx.y * 64;
+    const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
+    for (int kk = ty; kk < 64; kk += 4) {
+        const int k = kb + kk;
+        const long r = rb + tx;
+        tile[kk][tx] = (k < K && r < rows) ? psi[r0 + r + n2 * k] : 0.0f;
+    }
+    __syncthreads();
+    const int rr = t >> 2, quad = t & 3;
+    const long r = rb + rr;
+    if (r >= rows) return;
+    const long lo_off = rows * K;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int k = kb + 16 * b + 4 * quad;
+        if (k >= K) break;
+        bf16x4 h, l;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            __bf16 hh, ll;
+            split_bf16(tile[16 * b + 4 * quad + e][rr], hh, ll);
+            h[e] = hh;
+            l[e] = ll;
+        }
+        const long off = (long)(k >> 4) * 16 * rows + 16 * r + (k & 15);
+        *reinterpret_cast<bf16x4*>(X + off) = h;
+        *reinterpret_cast<bf16x4*>(X + lo_off + off) = l;
+    }
+}
+
+// cnt slabs T_l (n x n, T_l[i + n j] at src[l n^2 + i + n j]) -> the K-blocked A operand
+// of the mode-1 product A(l n + j, i) = T_l[i + n j]: hi at
+// Y[l 2n^2 + (i / 16) 16 n + 16 j + i % 16], lo at + n^2.  n % 16 == 0.
+__global__ void __launch_bounds__(256) k_split_slabs(const float* __restrict__ src, int n,
+                                                      long cnt, __bf16* __restrict__ Y)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // group of 4 values
+    const long n2 = (long)n * n;
+    const long per = n2 / 4;
+    if (q >= per * cnt) return;
+    const long l = q / per, e = 4 * (q - l * per);
+    const int j = (int)(e / n), i = (int)(e - (long)j * n);
+    const float4 x = *reinterpret_cast<const float4*>(src + l * n2 + e);
+    bf16x4 h, lo;
+    __bf16 a, b;
+    split_bf16(x.x, a, b); h[0] = a; lo[0] = b;
+    split_bf16(x.y, a, b); h[1] = a; lo[1] = b;
+    split_bf16(x.z, a, b); h[2] = a; lo[2] = b;
+    split_bf16(x.w, a, b); h[3] = a; lo[3] = b;
+    const long off = l * 2 * n2 + (long)(i >> 4) * 16 * n + 16L * j + (i & 15);
+    *reinterpret_cast<bf16x4*>(Y + off) = h;
+    *reinterpret_cast<bf16x4*>(Y + off + n2) = lo;
+}
+
+}  // namespace v3
+
 // phi (n x n, col-major, ld n) -> zero-padded (Kpad x Npad, ld Kpad): fp32, or bf16 hi/lo.
 __global__ void k_prep_phi(const float* __restrict__ phi, int n, int Kpad, int Npad, int prec,
                            float* __restrict__ pf, __bf16* __restrict__ ph, __bf16* __restrict__ pl)
@@ -588,6 +866,201 @@ int modes12(hipStream_t s, int prec, float* T, int n, int l0, int cnt, const Phi
     return FC3_OK;
 }
 
+
+// ---------------------------------------------------------------------------------
+// BF16X3 with pre-split operands (v3 kernels), used when n % 16 == 0.  Every operand
+// is in the K-blocked layout of v3::G3:
+//   phi:    B(k, c) = phi[k + n c], columns zero-padded to a multiple of BN.
+//   mode 3: Psi row blocks (rows (i, j) = i + n j) are split-transposed into X, and the
+//           GEMM writes U_l(i, j) = U[(i, j), l] split, already as the mode-1 A operand
+//           A(l n + j, i): slab l hi at bf16 offset l 2n^2, lo at +n^2.  That is
+//           exactly the byte range of K3 slab l.
+//   mode 1: S_l = U_l^T phi, written split as the mode-2 A operand (same layout).
+//   mode 2: K3_l = S_l^T phi, FP32, over the bytes of U_l (already consumed).
+bool v3_applies(int n, int prec) { return prec == FC3_PREC_BF16X3 && n % v3::BK == 0 && n >= 16; }
+bool aligned16(const void* p) { return (uintptr_t)p % 16 == 0; }
+
+struct Split3 {
+    const __bf16* b;
+    long lo, ks;
+};
+
+// phi -> K-blocked B operand: hi at P[(k / 16) 16 Npad + 16 c + k % 16], lo at + n Npad.
+__global__ void k_prep_phi3(const float* __restrict__ phi, int n, int Npad, __bf16* __restrict__ P)
+{
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long e = (long)n * Npad;
+    if (o >= e) return;
+    const long kb = o / (16L * Npad), rem = o - kb * 16L * Npad;
+    const int c = (int)(rem >> 4), k = (int)(16 * kb + (rem & 15));
+    const float x = c < n ? phi[k + (long)c * n] : 0.0f;
+    __bf16 h, l;
+    split_bf16(x, h, l);
+    P[o] = h;
+    P[e + o] = l;
+}
+
+size_t phi3_bytes(int n) { return Carve::need((size_t)2 * 2 * round_up(n, v3::BN) * n); }
+
+int prep_phi3(hipStream_t s, const float* phi, int n, Carve& cv, Split3& out)
+{
+    const int Npad = round_up(n, v3::BN);
+    const long e = (long)n * Npad;
+    __bf16* p = cv.take<__bf16>(2 * e);
+    out.b = p;
+    out.lo = e;
+    out.ks = 16L * Npad;
+    hipLaunchKernelGGL(k_prep_phi3, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, phi, n,
+                       Npad, p);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+// Output addressing of v3::G3 (see there); lo plane at + lo for split outputs.
+struct OutMap {
+    void* C;
+    long R, SS, KS, CS, lo;
+};
+
+// C(m, c) = sum_k A(m, k) phi(k, c); A(m, k) = A[(m / a_R) a_SS + (k / 16) a_KS + 16 (m % a_R)
+// + k % 16], lo at + a_lo.
+int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_SS, long a_KS,
+          const Split3& B, const OutMap& o, bool split_out, long M, int N, int K)
+{
+    if (M >= (1L << 31) || a_R >= (1L << 31) || o.R >= (1L << 31))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
+    const bool ok = K % v3::BK == 0 && M % 4 == 0 && M > 0 && a_lo % 8 == 0 && a_SS % 8 == 0 &&
+                    a_KS % 8 == 0 && B.lo % 8 == 0 && B.ks % 8 == 0 && o.R % 4 == 0 &&
+                    o.SS % 4 == 0 && o.KS % 4 == 0 && o.CS % 4 == 0 && o.lo % 4 == 0 &&
+                    aligned16(A) && aligned16(B.b) && aligned16(o.C);
+    if (!ok) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: v3 GEMM operand layout");
+    v3::G3 g;
+    g.A = A; g.a_lo = a_lo; g.a_SS = a_SS; g.a_KS = a_KS; g.a_R = (int)a_R;
+    g.B = B.b; g.b_lo = B.lo; g.b_KS = B.ks;
+    g.C = o.C; g.c_SS = o.SS; g.c_KS = o.KS; g.c_CS = o.CS; g.c_lo = o.lo; g.c_R = (int)o.R;
+    g.M = (int)M; g.N = N; g.K = K;
+    g.tiles_m = (int)((M + v3::BM - 1) / v3::BM);
+    g.tiles_n = (N + v3::BN - 1) / v3::BN;
+    const long tiles = (long)g.tiles_m * g.tiles_n;
+    g.per_xcd = (tiles + kXcd - 1) / kXcd;
+    const long grid = g.per_xcd * kXcd;
+    if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
+    ProfScope prof(kind, s);
+    if (split_out)
+        hipLaunchKernelGGL(v3::k_gemm3<true>, dim3((unsigned)grid), dim3(v3::NT), 0, s, g);
+    else
+        hipLaunchKernelGGL(v3::k_gemm3<false>, dim3((unsigned)grid), dim3(v3::NT), 0, s, g);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+// Mode-3 row block: a multiple of n rows (whole j columns of Psi(i, j, :)), enough
+// workgroups to fill the chip, X scratch bounded to 2 GiB.
+long mode3_rows(int n)
+{
+    const long n2 = (long)n * n;
+    const long tiles_n = (n + v3::BN - 1) / v3::BN;
+    const long target = (8192 + tiles_n - 1) / tiles_n * v3::BM;
+    long R = (target + n - 1) / n * n;
+    const long cap = ((1L << 31) / (4L * n)) / n * n;
+    if (R > cap) R = cap;
+    if (R < n) R = n;
+    if (R > n2) R = n2;
+    return R;
+}
+
+// Psi rows [r0, r0 + rows) -> X (split, K-blocked) -> C(m, l) for those rows.
+int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, const Split3& B,
+                __bf16* X, const OutMap& o, bool split_out)
+{
+    const long n2 = (long)n * n;
+    dim3 gt((unsigned)((rows + 63) / 64), (unsigned)((n + 63) / 64));
+    hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, r0, rows, X);
+    FC3_HIP(hipGetLastError());
+    return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * n, rows, 0, 16 * rows, B, o, split_out,
+                 rows, n, n);
+}
+
+// Modes 1 and 2 on slabs [l0, l0 + cnt): U holds them as the mode-1 A operand (slab
+// stride 2n^2 bf16, lo at +n^2, K-blocked); S is scratch of the same layout; T
+// receives the FP32 slabs K3_l[b + n c].
+int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, __bf16* S,
+               float* T)
+{
+    const long n2 = (long)n * n;
+    // A(l n + j, i) = U_l(i, j);  C(l n + j, b) = S_l(j, b) -> mode-2 A(l n + b, j)
+    const OutMap oS{S, n, 2 * n2, 16L * n, 16, n2};
+    FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, n2, n, 2 * n2, 16L * n, B, oS, true,
+                  (long)cnt * n, n, n));
+    // A(l n + b, j) = S_l(j, b);  C(l n + b, c) = K3_l[b + n c]
+    const OutMap oT{T, n, n2, 16, n, 0};
+    return gemm3(s, FC3_KERNEL_GEMM_MODE12, S, n2, n, 2 * n2, 16L * n, B, oT, false,
+                 (long)cnt * n, n, n);
+}
+
+size_t mcc3_v3_workspace_bytes(int n, bool inplace)
+{
+    const long n2 = (long)n * n;
+    const size_t slabs = Carve::need(sizeof(float) * n2 * slab_batch(n));
+    return phi3_bytes(n) + slabs * (inplace ? 2 : 1) +
+           Carve::need(sizeof(float) * (size_t)mode3_rows(n) * n) + 4096;
+}
+
+int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3, void* ws,
+            size_t ws_bytes)
+{
+    Carve cv(ws, ws_bytes);
+    Split3 B;
+    FC3_TRY(prep_phi3(s, phi, n, cv, B));
+    const long n2 = (long)n * n;
+    const int nb = slab_batch(n);
+    const long R = mode3_rows(n);
+    __bf16* S = cv.take<__bf16>((size_t)2 * n2 * nb);
+    __bf16* X = cv.take<__bf16>((size_t)2 * R * n);
+    __bf16* U = reinterpret_cast<__bf16*>(k3);
+    for (long r0 = 0; r0 < n2; r0 += R) {
+        const long rows = (n2 - r0) < R ? (n2 - r0) : R;
+        // C((i, j) - r0, l) -> U_l(i, j) = U[l 2n^2 + (i / 16) 16 n + 16 j + i % 16]; a row
+        // block starts at a multiple of n (R is), so j = (m + r0) / n with m = row
+        const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * n2, n2};
+        FC3_TRY(mode3_block(s, psi, n, r0, rows, B, X, oU, true));
+    }
+    for (int l0 = 0; l0 < n; l0 += nb) {
+        const int cnt = (n - l0) < nb ? (n - l0) : nb;
+        FC3_TRY(modes12_v3(s, U + 2 * n2 * l0, n, cnt, B, S, k3 + n2 * l0));
+    }
+    return FC3_OK;
+}
+
+int mcc3_v3_inplace(hipStream_t s, float* psi, const float* phi, int n, void* ws, size_t ws_bytes)
+{
+    Carve cv(ws, ws_bytes);
+    Split3 B;
+    FC3_TRY(prep_phi3(s, phi, n, cv, B));
+    const long n2 = (long)n * n;
+    const int nb = slab_batch(n);
+    const long R = mode3_rows(n);
+    __bf16* Y = cv.take<__bf16>((size_t)2 * n2 * nb);
+    __bf16* S = cv.take<__bf16>((size_t)2 * n2 * nb);
+    __bf16* X = cv.take<__bf16>((size_t)2 * R * n);
+    // modes 1 and 2 on the slabs of Psi (they commute with mode 3)
+    for (int l0 = 0; l0 < n; l0 += nb) {
+        const int cnt = (n - l0) < nb ? (n - l0) : nb;
+        const long q = n2 / 4 * cnt;
+        hipLaunchKernelGGL(v3::k_split_slabs, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
+                           psi + n2 * l0, n, (long)cnt, Y);
+        FC3_HIP(hipGetLastError());
+        FC3_TRY(modes12_v3(s, Y, n, cnt, B, S, psi + n2 * l0));
+    }
+    // mode 3 by row blocks: each block is copied (split) to X before the GEMM overwrites it
+    for (long r0 = 0; r0 < n2; r0 += R) {
+        const long rows = (n2 - r0) < R ? (n2 - r0) : R;
+        const OutMap oP{psi + r0, rows, 0, 16, n2, 0};
+        FC3_TRY(mode3_block(s, psi, n, r0, rows, B, X, oP, false));
+    }
+    return FC3_OK;
+}
+
 }  // namespace
 
 size_t mcc3_workspace_bytes(int n, int precision, bool inplace)
@@ -599,12 +1072,16 @@ size_t mcc3_workspace_bytes(int n, int precision, bool inplace)
         const long R = 1L << 16;
         b += Carve::need(sizeof(float) * (size_t)(R < n2 ? R : n2) * n);
     }
+    // the pre-split path is taken only for 16-byte aligned tensors; size for either path
+    if (v3_applies(n, precision)) b = std::max(b, mcc3_v3_workspace_bytes(n, inplace));
     return b;
 }
 
 int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int precision, float* k3,
              void* ws, size_t ws_bytes)
 {
+    if (v3_applies(n, precision) && aligned16(psi) && aligned16(k3))
+        return mcc3_v3(s, psi, phi, n, k3, ws, ws_bytes);
     Carve cv(ws, ws_bytes);
     PhiPrep B;
     FC3_TRY(prep_phi(s, phi, n, precision, cv, B));
@@ -623,6 +1100,8 @@ int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int preci
 int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int precision, void* ws,
                      size_t ws_bytes)
 {
+    if (v3_applies(n, precision) && aligned16(psi))
+        return mcc3_v3_inplace(s, psi, phi, n, ws, ws_bytes);
     Carve cv(ws, ws_bytes);
     PhiPrep B;
     FC3_TRY(prep_phi(s, phi, n, precision, cv, B));

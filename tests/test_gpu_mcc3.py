@@ -2,8 +2,12 @@
 
 Bars (relative Frobenius error vs FP64): FP32 <= 2e-6 (plain FP32 accumulation
 over n-term sums), BF16X3 <= 1e-5.  Exact invariants: phi = I gives K3 = Psi and a
-permutation phi permutes Psi (FP32).
+permutation phi permutes Psi (FP32).  Sizes cover both BF16X3 kernels (n % 16 == 0
+takes the pre-split path), single and partial tiles, and, at n = 1536, several
+mode-3 row blocks against a closed form (Psi a sum of rank-1 terms).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -30,7 +34,7 @@ def random_case(n, seed):
     return np.asfortranarray(psi.astype(np.float32)), np.asfortranarray(phi.astype(np.float32))
 
 
-@pytest.mark.parametrize("n", [96, 200, 99, 256])
+@pytest.mark.parametrize("n", [16, 48, 96, 200, 99, 256, 272])
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_mcc3_random(ctx, n, prec):
     psi, phi = random_case(n, 100 + n)
@@ -44,13 +48,13 @@ def test_mcc3_random(ctx, n, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-def test_mcc3_inplace_and_blocked(ctx, prec):
-    n = 96
+@pytest.mark.parametrize("n", [96, 112])
+def test_mcc3_inplace_and_blocked(ctx, prec, n):
     psi, phi = random_case(n, 7)
     ref = O.mcc3_fp64(psi, phi)
     dphi = fc3.DeviceArray.from_host(phi, ctx=ctx)
     dpsi = fc3.DeviceArray.from_host(psi, ctx=ctx)
-    blocked = fc3.mcc3(dpsi, dphi, 32, precision=prec)      # host result, block_size = 32
+    blocked = fc3.mcc3(dpsi, dphi, 16, precision=prec)      # host result, block_size = 16
     assert isinstance(blocked, np.ndarray)
     assert rel_fro(blocked, ref) <= TOL[prec]
     fc3.mcc3_(dpsi, dphi, precision=prec)                   # mcc3!: overwrites Psi
@@ -102,3 +106,46 @@ def test_mcc3_lj_fcc222_modes(ctx, prec):
     err = np.linalg.norm((k3.astype(np.float64) - ref).ravel())
     print(f"fcc222 {prec}: |K3 - K3_64| = {err:.3e}, rel {rel_fro(k3, ref):.3e}", flush=True)
     assert err <= 1e-6 or rel_fro(k3, ref) <= TOL[prec]
+
+
+@pytest.mark.parametrize("inplace", [False, True])
+def test_mcc3_bf16x3_rank3_large(ctx, inplace):
+    """n = 1536: Psi = sum_t a_t (x) b_t (x) c_t, so K3 = sum_t (phi^T a_t) (x) (phi^T b_t)
+    (x) (phi^T c_t) in closed form; Psi is uploaded slab by slab.  The bar is the
+    whole-tensor one, 1e-5 relative Frobenius, estimated on 14 output slabs (the two
+    ends and 12 drawn at random): sqrt(sum_l |K3_l - ref_l|^2 / sum_l |ref_l|^2)."""
+    n, T = 1536, 3
+    rng = np.random.default_rng(11)
+    a, b, c = (rng.standard_normal((T, n)) for _ in range(3))
+    phi, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    phi32 = phi.astype(np.float32)
+    psi = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    slab_bytes = 4 * n * n
+    for k in range(n):
+        slab = np.asfortranarray(np.einsum("ti,tj,t->ij", a, b, c[:, k]).astype(np.float32))
+        fc3._lib.check(ctx.lib.fc3_memcpy_h2d(ctx.h, ctypes.c_void_p(psi.ptr.value + slab_bytes * k),
+                                              slab.ctypes.data_as(ctypes.c_void_p), slab_bytes))
+    dphi = fc3.DeviceArray.from_host(phi32, ctx=ctx)
+    if inplace:
+        fc3.mcc3_(psi, dphi, precision="bf16x3")
+        out = psi
+    else:
+        out = fc3.mcc3(psi, dphi, precision="bf16x3")
+    p64 = phi32.astype(np.float64)
+    A, B, C = a @ p64, b @ p64, c @ p64   # (T, n): phi^T a_t as rows
+    slabs = sorted({0, n - 1, *np.random.default_rng(12).choice(n, 12, replace=False).tolist()})
+    num = den = 0.0
+    for l in slabs:
+        got = np.empty((n, n), dtype=np.float32, order="F")
+        fc3._lib.check(ctx.lib.fc3_memcpy_d2h(ctx.h, got.ctypes.data_as(ctypes.c_void_p),
+                                              ctypes.c_void_p(out.ptr.value + slab_bytes * l),
+                                              slab_bytes))
+        ref = np.einsum("ti,tj,t->ij", A, B, C[:, l])
+        e2 = float(np.sum((got.astype(np.float64) - ref) ** 2))
+        r2 = float(np.sum(ref * ref))
+        num += e2
+        den += r2
+        print(f"rank3 n={n} inplace={inplace} slab {l}: rel err {np.sqrt(e2 / r2):.3e}", flush=True)
+    err = np.sqrt(num / den)
+    print(f"rank3 n={n} inplace={inplace}: rel fro err over {len(slabs)} slabs {err:.3e}", flush=True)
+    assert err <= 1e-5

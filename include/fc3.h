@@ -145,7 +145,8 @@ int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precis
              float* k3);
 
 /* mcc3!(Psi, phi) replaces src/mcc.jl:60-80: the result overwrites psi (device),
- * using O(n^2) workspace instead of the reference's two extra n^3 tensors.       */
+ * using a bounded workspace (at most about 4 GiB, O(n^2) slabs plus O(n) rows of
+ * Psi) instead of the reference's two extra n^3 tensors.                         */
 int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int precision);
 
 /* mcc3(Psi, phi, block_size) replaces src/mcc.jl:89-120: same result as fc3_mcc3,

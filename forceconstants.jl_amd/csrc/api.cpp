@@ -180,6 +180,7 @@ int fc3_destroy(fc3_handle h)
     if (!h) return FC3_OK;
     hipSetDevice(h->device);
     hipStreamSynchronize(h->stream);
+    release_blas(h);
     if (h->ws) hipFree(h->ws);
     for (auto& r : h->prof.records) {
         hipEventDestroy(r.start);
@@ -460,6 +461,64 @@ int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n, in
     hipStreamSynchronize(h->stream);
     hipFree(k3);
     return st;
+}
+
+int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const float* dk3, int n,
+                      int precision, float* dpsi, float* dphi)
+{
+    FC3_TRY(check_handle(h));
+    if (!phi || !dk3) return set_error(FC3_ERR_ARGUMENT, "phi, dk3 must be non-null");
+    if (!dpsi && !dphi) return set_error(FC3_ERR_ARGUMENT, "dpsi and dphi are both null");
+    if (dphi && !psi) return set_error(FC3_ERR_ARGUMENT, "psi must be non-null for dphi");
+    if (n < 1) return set_error(FC3_ERR_ARGUMENT, "n must be >= 1");
+    if (precision != FC3_PREC_FP32 && precision != FC3_PREC_BF16X3)
+        return set_error(FC3_ERR_ARGUMENT, "unknown precision");
+    if ((long long)n * n >= (1LL << 31))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: n^2 must be < 2^31");
+    if (dpsi && ((const void*)dpsi == (const void*)dk3 || (const void*)dpsi == (const void*)psi))
+        return set_error(FC3_ERR_ARGUMENT, "dpsi must not alias psi or dk3");
+    FC3_TRY(ensure_workspace(h, mcc3_backward_workspace_bytes(n, precision, dphi != nullptr)));
+    return mcc3_backward(h, psi, phi, dk3, n, precision, dpsi, dphi);
+}
+
+int fc3_third_order_lj_param_backward(fc3_handle h, const double* positions, int n_atoms,
+                                      const double box[3], double sigma, double epsilon,
+                                      double pot_r_cut, double r_cut, double tol,
+                                      const double* masses, const float* dpsi32,
+                                      const double* dpsi64, double grad[2])
+{
+    FC3_TRY(check_handle(h));
+    FC3_TRY(check_system(positions, n_atoms, box));
+    for (int c = 0; c < 3; ++c)
+        if (!(pot_r_cut < box[c])) return set_error(FC3_ERR_CUTOFF, "Cutoff larger than L/2");
+    if (!grad) return set_error(FC3_ERR_ARGUMENT, "grad must be non-null");
+    if ((dpsi32 == nullptr) == (dpsi64 == nullptr))
+        return set_error(FC3_ERR_ARGUMENT, "exactly one of dpsi32, dpsi64 must be non-null");
+    if (dpsi64 && masses)
+        return set_error(FC3_ERR_ARGUMENT, "dpsi64 is the cotangent of the unweighted Psi: masses must be null");
+    if (!(sigma > 0.0)) return set_error(FC3_ERR_ARGUMENT, "sigma must be positive");
+    if (masses)
+        for (int i = 0; i < n_atoms; ++i)
+            if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    const int N = n_atoms;
+    Prepared P;
+    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
+        const size_t blocks = ((size_t)N * cap + 255) / 256;
+        return sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * 27 + 2 * blocks + 2) +
+               4096;
+    }, P));
+    const size_t slots = (size_t)N * P.nl.cap;
+    double* E3 = reinterpret_cast<double*>(P.extra);
+    double* E2 = E3 + slots * 27;
+    double* S3 = E2 + slots * 9;
+    double* out = S3 + (size_t)N * 27;
+    double* partial = out + 2;
+    FC3_TRY(lj_pair_blocks(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, S3, E2, nullptr));
+    FC3_TRY(lj_param_grad(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, S3, tol,
+                          masses ? P.masses : nullptr, dpsi32, dpsi64, partial, out));
+    FC3_HIP(hipMemcpyAsync(grad, out, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    FC3_HIP(hipStreamSynchronize(h->stream));
+    return FC3_OK;
 }
 
 }  // extern "C"

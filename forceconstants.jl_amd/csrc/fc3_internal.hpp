@@ -46,6 +46,7 @@ struct Profiler {
 
 struct fc3_context {
     fc3::Profiler prof;
+    void* blas = nullptr;   // rocBLAS handle, created on first use (adjoint.hip)
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -108,6 +109,8 @@ int lj_pair_blocks(hipStream_t s, const double* d_pos, const double box[3], doub
                    double* E2, double* S2);
 int lj_write_ifc2(hipStream_t s, const NeighborList& nl, const double* E2, const double* S2,
                   double tol, double* ifc2);
+// sigma^12 and sigma^6 as the reference's Float64^Int evaluates them.
+void lj_sigma_powers(double sigma, double* s12, double* s6);
 
 // Generic dense third-order writer over per-pair blocks (shared by LJ and SW):
 // see the kernel comment in ifc3_writer.hip.
@@ -135,5 +138,19 @@ int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int preci
              float* k3, void* ws, size_t ws_bytes);
 int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int precision,
                      void* ws, size_t ws_bytes);
+
+// ---- adjoints (adjoint.hip) ---------------------------------------------------------
+void release_blas(fc3_context* h);
+size_t mcc3_backward_workspace_bytes(int n, int precision, bool need_dphi);
+// dpsi (n^3) and/or dphi (n x n), device FP32; uses h's workspace (sized by the above).
+int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const float* G, int n,
+                  int precision, float* dpsi, float* dphi);
+// dL/dsigma, dL/depsilon of <dpsi, Psi> into d_out[0..1] (device); partial: 2 doubles per
+// 256 (atom, slot) pairs.  dpsi32 is the cotangent of the mass-weighted FP32 Psi
+// (d_masses as in the forward call), dpsi64 that of the unweighted FP64 Psi.
+int lj_param_grad(hipStream_t s, const double* d_pos, const double box[3], double sigma,
+                  double epsilon, const NeighborList& nl, const double* E3, const double* S3,
+                  double tol, const double* d_masses, const float* dpsi32, const double* dpsi64,
+                  double* partial, double* d_out);
 
 }  // namespace fc3

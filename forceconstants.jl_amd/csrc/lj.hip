@@ -105,15 +105,9 @@ k_lj_write_ifc2(const int* __restrict__ count, const int* __restrict__ nbr, int 
 
 }  // namespace
 
-int lj_pair_blocks(hipStream_t s, const double* d_pos, const double box[3], double sigma,
-                   double epsilon, const NeighborList& nl, double* E3, double* S3, double* E2,
-                   double* S2)
+void lj_sigma_powers(double sigma, double* s12, double* s6)
 {
-    // sigma^12 and sigma^6 as Julia evaluates them (compensated Float64^Int); the
-    // host computes them with the same device routine for bit agreement.
-    const int N = nl.n_atoms;
-    const long tot = (long)N * nl.cap;
-    // powers of sigma: evaluated on the host with the identical algorithm
+    // Julia's Float64^Int: compensated power by squaring (same algorithm as geom::ipow)
     auto hpow = [](double x, int n) {
         double yh = 1.0, yl = 0.0, xh = x, xl = 0.0;
         while (n > 1) {
@@ -134,7 +128,20 @@ int lj_pair_blocks(hipStream_t s, const double* d_pos, const double box[3], doub
         pl = pl + (yh * xl + yl * xh);
         return ph + pl;
     };
-    const double s12 = hpow(sigma, 12), s6 = hpow(sigma, 6);
+    *s12 = hpow(sigma, 12);
+    *s6 = hpow(sigma, 6);
+}
+
+int lj_pair_blocks(hipStream_t s, const double* d_pos, const double box[3], double sigma,
+                   double epsilon, const NeighborList& nl, double* E3, double* S3, double* E2,
+                   double* S2)
+{
+    // sigma^12 and sigma^6 as Julia evaluates them (compensated Float64^Int); the
+    // host computes them with the same algorithm as the device for bit agreement.
+    const int N = nl.n_atoms;
+    const long tot = (long)N * nl.cap;
+    double s12 = 0.0, s6 = 0.0;
+    lj_sigma_powers(sigma, &s12, &s6);
     ProfScope prof(FC3_KERNEL_PAIR_BLOCKS, s);
     hipLaunchKernelGGL(k_lj_pair_blocks, dim3((tot + 255) / 256), dim3(256), 0, s, d_pos, box[0],
                        box[1], box[2], s12, s6, epsilon, nl.count, nl.nbr, N, nl.cap, E3, E2);

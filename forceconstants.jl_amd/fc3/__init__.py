@@ -6,14 +6,16 @@ libfc3.so.  See api.py for the reference-named entry points.
 """
 from . import systems
 from .api import (Context, DenseForceConstants, DeviceArray, dynamical_matrix, get_modes,
-                  mass_weight_third_order_, mcc3, mcc3_, second_order, second_order_device,
-                  third_order, third_order_device)
+                  mass_weight_third_order_, mcc3, mcc3_, mcc3_backward, mcc3_rrule,
+                  second_order, second_order_device, third_order, third_order_device,
+                  third_order_param_backward)
 from .systems import (AnalyticalCalculator, AutoDiffCalculator, LJ, StillingerWeberSilicon,
                       SuperCellSystem, SWParams, diamond, fcc)
 
 __all__ = [
     "Context", "DenseForceConstants", "DeviceArray", "dynamical_matrix", "get_modes",
-    "mass_weight_third_order_", "mcc3", "mcc3_", "second_order", "second_order_device",
+    "mass_weight_third_order_", "mcc3", "mcc3_", "mcc3_backward", "mcc3_rrule",
+    "third_order_param_backward", "second_order", "second_order_device",
     "third_order", "third_order_device", "systems", "AnalyticalCalculator",
     "AutoDiffCalculator", "LJ", "StillingerWeberSilicon", "SuperCellSystem", "SWParams",
     "diamond", "fcc",

@@ -63,6 +63,10 @@ SIGNATURES = {
     "fc3_mcc3": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _vp]),
     "fc3_mcc3_inplace": (_c_int, [_handle, _vp, _vp, _c_int, _c_int]),
     "fc3_mcc3_blocked": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _c_int, _fp]),
+    "fc3_mcc3_backward": (_c_int, [_handle, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp]),
+    "fc3_third_order_lj_param_backward": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double,
+                                                   _c_double, _c_double, _c_double, _dp, _vp, _vp,
+                                                   _dp]),
 }
 
 _lib = None

@@ -279,3 +279,61 @@ def mcc3_(psi: DeviceArray, phi: DeviceArray, precision="fp32") -> DeviceArray:
         raise AssertionError("mcc3!: Psi must be n x n x n and phi n x n")
     check(ctx.lib.fc3_mcc3_inplace(ctx.h, psi.ptr, phi.ptr, n, _prec(precision)))
     return psi
+
+
+# ---- adjoints (reverse mode; the targets of the Julia rrules) -------------------------
+
+def mcc3_backward(psi: DeviceArray, phi: DeviceArray, dk3: DeviceArray, precision="fp32",
+                  dpsi: bool = True, dphi: bool = True):
+    """Pullback of K3 = mcc3(Psi, phi) for the cotangent dk3 = dL/dK3 (src/mcc.jl:15-17):
+    returns (dL/dPsi or None, dL/dphi or None) as device arrays."""
+    ctx = psi.ctx
+    n = phi.shape[0]
+    if psi.shape != (n, n, n) or dk3.shape != (n, n, n) or phi.shape != (n, n):
+        raise AssertionError("mcc3 backward: Psi, dK3 must be n x n x n and phi n x n")
+    gp = DeviceArray((n, n, n), np.float32, ctx) if dpsi else None
+    gf = DeviceArray((n, n), np.float32, ctx) if dphi else None
+    check(ctx.lib.fc3_mcc3_backward(ctx.h, psi.ptr, phi.ptr, dk3.ptr, n, _prec(precision),
+                                    gp.ptr if gp else None, gf.ptr if gf else None))
+    return gp, gf
+
+
+def mcc3_rrule(psi: DeviceArray, phi: DeviceArray, precision="fp32"):
+    """(K3, pullback) as a ChainRulesCore rrule returns them; pullback(dk3) ->
+    (dL/dPsi, dL/dphi)."""
+    k3 = mcc3(psi, phi, precision=precision)
+
+    def pullback(dk3: DeviceArray):
+        return mcc3_backward(psi, phi, dk3, precision=precision)
+
+    return k3, pullback
+
+
+def third_order_param_backward(sys: SuperCellSystem, pot, calc, dpsi: DeviceArray,
+                               masses=None, ctx: Optional[Context] = None):
+    """dL/d(potential parameters) for the cotangent dpsi of third_order's output.
+
+    dpsi FP32: cotangent of the FP32 tensor third_order_device(..., psi32=...) writes
+    (mass weighted when `masses` is given); dpsi FP64: of the unweighted FP64 tensor.
+    Returns {"sigma": ..., "epsilon": ...} for LJ."""
+    ctx = ctx or dpsi.ctx
+    if not isinstance(pot, LJ):
+        raise NotImplementedError("parameter adjoint implemented for LJ")
+    pos, L = _sys_args(sys)
+    n = sys.n_dof
+    if dpsi.shape != (n, n, n):
+        raise AssertionError("dpsi must be 3N x 3N x 3N")
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    if dpsi.dtype == np.float32:
+        d32, d64 = dpsi.ptr, None
+    elif dpsi.dtype == np.float64:
+        if m is not None:
+            raise AssertionError("an FP64 cotangent refers to the unweighted tensor: masses must be None")
+        d32, d64 = None, dpsi.ptr
+    else:
+        raise TypeError("dpsi must be float32 or float64")
+    g = np.zeros(2, dtype=np.float64)
+    check(ctx.lib.fc3_third_order_lj_param_backward(
+        ctx.h, _ptr(pos), sys.n_atoms, _ptr(L), pot.sigma, pot.epsilon, pot.r_cut, calc.r_cut,
+        calc.tol, _ptr(m) if m is not None else None, d32, d64, _ptr(g)))
+    return {"sigma": float(g[0]), "epsilon": float(g[1])}

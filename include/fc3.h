@@ -155,6 +155,30 @@ int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int prec
 int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n,
                      int block_size, int precision, float* k3_host);
 
+/* ---- adjoints (no counterpart in the reference; its rrule targets) ----------- */
+/* Reverse mode of mcc3 (src/mcc.jl:15-17) for a cotangent dk3 = dL/dK3 (device n^3):
+ *   dpsi = dL/dPsi = dk3 x1 phi x2 phi x3 phi            (device n^3, or NULL)
+ *   dphi = dL/dphi, the three mode terms                  (device n x n, or NULL; needs psi)
+ * psi, phi, dk3 device FP32.  With dphi the work is FP32 (18 n^4 flops, two n^3
+ * buffers of workspace); dpsi alone runs the mcc3 kernels at `precision`.          */
+int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const float* dk3,
+                      int n, int precision, float* dpsi, float* dphi);
+
+/* Reverse mode of third_order(sys, LJ(sigma, epsilon, pot_r_cut), AnalyticalCalculator)
+ * (src/ifc.jl:20-35 -> third_order_analytical.jl:4-69, asr.jl:19-40,
+ * helper_funcs.jl:20-28) with respect to the potential parameters:
+ *   grad[0] = dL/dsigma, grad[1] = dL/depsilon   (host)
+ * for the cotangent of exactly one output of fc3_third_order_lj:
+ *   dpsi32: of the FP32 psi32 (mass weighted when masses != NULL), device n^3, or
+ *   dpsi64: of the FP64 psi64 (masses must be NULL), device n^3.
+ * Pair blocks are recomputed from the positions; entries zeroed by the tolerance
+ * contribute nothing.                                                              */
+int fc3_third_order_lj_param_backward(fc3_handle h, const double* positions, int n_atoms,
+                                      const double box[3], double sigma, double epsilon,
+                                      double pot_r_cut, double r_cut, double tol,
+                                      const double* masses, const float* dpsi32,
+                                      const double* dpsi64, double grad[2]);
+
 #ifdef __cplusplus
 }
 #endif

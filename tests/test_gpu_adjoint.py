@@ -1,0 +1,161 @@
+"""Adjoints on the GPU: mcc3 backward and the LJ parameter backward of third_order.
+
+Pins (none retypes the adjoint formulas):
+  * mcc3 dL/dPsi: the adjoint identity <G, mcc3(dPsi_dir, phi)> = <dL/dPsi, dPsi_dir> for
+    random directions, with the forward evaluated by the FP64 oracle contraction.
+  * mcc3 dL/dphi: central differences in FP64 of L(phi) = <G, mcc3(Psi, phi)> along
+    random directions (L is a cubic polynomial in phi, so the truncation error is
+    O(h^2) relative; h = 1e-4).
+  * LJ dL/dsigma: central differences (relative step 1e-6, as the C5 configuration asks)
+    of F(sigma) = <dPsi, Psi(sigma)> with Psi from the FP64 oracle restatement of the
+    reference loop; dL/depsilon: Psi is linear in epsilon, so dF/depsilon = F / epsilon
+    exactly.
+  * The whole chain L = sum w K3^2 (w ~ U(0,1) seeded) through mass weighting and mcc3,
+    against FP64 central differences of the oracle chain.
+Bars: FP64 paths 1e-6 relative (FD noise), FP32 paths 2e-5 relative.
+"""
+import numpy as np
+import pytest
+
+import fc3
+from fc3 import systems as S
+from oracle import fc3_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LJ = S.ARGON_LJ
+
+
+def _rand(shape, seed, dtype=np.float64):
+    return np.asfortranarray(np.random.default_rng(seed).standard_normal(shape).astype(dtype))
+
+
+def _dot(a, b):
+    return float(np.sum(np.asarray(a, dtype=np.float64) * np.asarray(b, dtype=np.float64)))
+
+
+@pytest.mark.parametrize("n", [48, 96, 100])
+def test_mcc3_backward_dpsi_adjoint_identity(ctx, n):
+    psi = _rand((n, n, n), 1, np.float32)
+    phi, _ = np.linalg.qr(np.random.default_rng(2).standard_normal((n, n)))
+    phi = np.asfortranarray(phi.astype(np.float32))
+    G = _rand((n, n, n), 3, np.float32)
+    d = fc3.DeviceArray.from_host
+    dpsi, dphi = fc3.mcc3_backward(d(psi, ctx=ctx), d(phi, ctx=ctx), d(G, ctx=ctx))
+    gp = dpsi.to_host()
+    for seed in (4, 5):
+        X = _rand((n, n, n), seed)
+        lhs = _dot(G, O.mcc3_fp64(X, phi))
+        rhs = _dot(gp, X)
+        print(f"n={n} <G, mcc3(X)> = {lhs:.9e}, <dPsi, X> = {rhs:.9e}", flush=True)
+        assert abs(lhs - rhs) <= 2e-5 * np.sqrt(_dot(G, G) * _dot(O.mcc3_fp64(X, phi), O.mcc3_fp64(X, phi)))
+    # dL/dPsi alone (the mcc3-kernel path) agrees with the path that also forms dL/dphi
+    for prec in ("fp32", "bf16x3"):
+        only, none = fc3.mcc3_backward(d(psi, ctx=ctx), d(phi, ctx=ctx), d(G, ctx=ctx),
+                                       precision=prec, dphi=False)
+        assert none is None
+        rel = np.linalg.norm((only.to_host().astype(np.float64) - gp).ravel()) / np.linalg.norm(gp)
+        print(f"n={n} dPsi via mcc3 kernels ({prec}) vs rocBLAS path: rel {rel:.2e}", flush=True)
+        assert rel <= (2e-6 if prec == "fp32" else 1e-5)
+
+
+@pytest.mark.parametrize("n", [48, 96])
+def test_mcc3_backward_dphi_central_difference(ctx, n):
+    psi = _rand((n, n, n), 11, np.float32)
+    phi = _rand((n, n), 12, np.float32) / np.sqrt(n)
+    phi = np.asfortranarray(phi.astype(np.float32))
+    G = _rand((n, n, n), 13, np.float32)
+    d = fc3.DeviceArray.from_host
+    _, dphi = fc3.mcc3_backward(d(psi, ctx=ctx), d(phi, ctx=ctx), d(G, ctx=ctx), dpsi=False)
+    gf = dphi.to_host().astype(np.float64)
+    p64, f64, g64 = (a.astype(np.float64) for a in (psi, phi, G))
+    h = 1e-4
+    for seed in (14, 15, 16):
+        D = _rand((n, n), seed)
+        fd = (_dot(g64, O.mcc3_fp64(p64, f64 + h * D)) - _dot(g64, O.mcc3_fp64(p64, f64 - h * D))) / (2 * h)
+        ad = _dot(gf, D)
+        print(f"n={n} dir {seed}: FD {fd:.9e} adjoint {ad:.9e} rel {abs(fd - ad) / abs(fd):.2e}", flush=True)
+        assert abs(fd - ad) <= 2e-5 * np.linalg.norm(gf) * np.linalg.norm(D)
+
+
+def _lj_sys(cells, su, seed):
+    return S.fcc(S.ARGON_A, cells, S.ARGON_MASS, su, seed=seed)
+
+
+@pytest.mark.parametrize("cells,su,seed,tol", [((2, 2, 2), 0.05, 21, 0.0), ((2, 2, 2), 0.05, 22, 1e-12),
+                                               ((3, 3, 3), 0.03, 23, 1e-12)])
+def test_lj_param_backward_fp64(ctx, cells, su, seed, tol):
+    sysd = _lj_sys(cells, su, seed)
+    calc = S.AnalyticalCalculator(tol, LJ.r_cut)
+    n = sysd.n_dof
+    dpsi = _rand((n, n, n), seed + 100)
+    g = fc3.third_order_param_backward(sysd, LJ, calc, fc3.DeviceArray.from_host(dpsi, ctx=ctx))
+    F = lambda pot: _dot(dpsi, O.lj_third_order(sysd, pot, calc))
+    F0 = F(LJ)
+    # epsilon: Psi is linear in epsilon
+    print(f"{cells}: dF/deps {g['epsilon']:.12e} vs F/eps {F0 / LJ.epsilon:.12e}", flush=True)
+    assert abs(g["epsilon"] - F0 / LJ.epsilon) <= 1e-10 * abs(F0 / LJ.epsilon)
+    # sigma: central difference, relative step 1e-6
+    hs = 1e-6 * LJ.sigma
+    fd = (F(S.LJ(LJ.sigma + hs, LJ.epsilon, LJ.r_cut)) - F(S.LJ(LJ.sigma - hs, LJ.epsilon, LJ.r_cut))) / (2 * hs)
+    print(f"{cells}: dF/dsigma adjoint {g['sigma']:.12e} FD {fd:.12e} rel {abs(fd - g['sigma']) / abs(fd):.2e}",
+          flush=True)
+    assert abs(fd - g["sigma"]) <= 1e-6 * abs(fd)
+
+
+def test_lj_param_backward_fp32_mass_weighted(ctx):
+    sysd = _lj_sys((2, 2, 2), 0.05, 31)
+    calc = S.AnalyticalCalculator(1e-12, LJ.r_cut)
+    n = sysd.n_dof
+    m = np.random.default_rng(32).uniform(20.0, 40.0, size=sysd.n_atoms)
+    d32 = _rand((n, n, n), 33, np.float32)
+    g = fc3.third_order_param_backward(sysd, LJ, calc, fc3.DeviceArray.from_host(d32, ctx=ctx), masses=m)
+
+    def F(pot):
+        psi = O.lj_third_order(sysd, pot, calc)
+        O.mass_weight_third_order(psi, m)
+        return _dot(d32, psi)
+
+    F0 = F(LJ)
+    assert abs(g["epsilon"] - F0 / LJ.epsilon) <= 1e-10 * abs(F0 / LJ.epsilon)
+    hs = 1e-6 * LJ.sigma
+    fd = (F(S.LJ(LJ.sigma + hs, LJ.epsilon, LJ.r_cut)) - F(S.LJ(LJ.sigma - hs, LJ.epsilon, LJ.r_cut))) / (2 * hs)
+    print(f"mass weighted: adjoint {g['sigma']:.12e} FD {fd:.12e}", flush=True)
+    assert abs(fd - g["sigma"]) <= 1e-6 * abs(fd)
+
+
+def test_lj_chain_sum_w_k3_squared(ctx):
+    """L = sum w K3^2, K3 = mcc3(mass-weighted FP32 Psi(sigma, epsilon), phi), phi fixed:
+    the GPU chain (forward, K3 cotangent 2 w K3, mcc3 backward, parameter backward)
+    against FP64 central differences of the oracle chain."""
+    sysd = _lj_sys((2, 2, 2), 0.05, 41)
+    calc = S.AnalyticalCalculator(1e-12, LJ.r_cut)
+    n = sysd.n_dof
+    f2 = O.lj_second_order(sysd, LJ, calc)
+    _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    w = np.asfortranarray(np.random.default_rng(42).uniform(0.0, 1.0, size=(n, n, n)))
+    # GPU forward
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, LJ, calc, psi32=psi32, masses=sysd.masses, ctx=ctx)
+    dphi = fc3.DeviceArray.from_host(phi32, ctx=ctx)
+    k3 = fc3.mcc3(psi32, dphi).to_host().astype(np.float64)
+    G = fc3.DeviceArray.from_host(2.0 * w * k3, dtype=np.float32, ctx=ctx)
+    dpsi, _ = fc3.mcc3_backward(psi32, dphi, G, dphi=False)
+    g = fc3.third_order_param_backward(sysd, LJ, calc, dpsi, masses=sysd.masses)
+
+    def L(pot):
+        psi = O.lj_third_order(sysd, pot, calc)
+        O.mass_weight_third_order(psi, sysd.masses)
+        K = O.mcc3_fp64(psi, phi32)
+        return float(np.sum(w * K * K))
+
+    for name, plus, minus, h in (
+            ("sigma", S.LJ(LJ.sigma * (1 + 1e-6), LJ.epsilon, LJ.r_cut),
+             S.LJ(LJ.sigma * (1 - 1e-6), LJ.epsilon, LJ.r_cut), LJ.sigma * 1e-6),
+            ("epsilon", S.LJ(LJ.sigma, LJ.epsilon * (1 + 1e-6), LJ.r_cut),
+             S.LJ(LJ.sigma, LJ.epsilon * (1 - 1e-6), LJ.r_cut), LJ.epsilon * 1e-6)):
+        fd = (L(plus) - L(minus)) / (2 * h)
+        print(f"chain dL/d{name}: adjoint {g[name]:.9e} FD {fd:.9e} rel {abs(fd - g[name]) / abs(fd):.2e}",
+              flush=True)
+        assert abs(fd - g[name]) <= 2e-4 * abs(fd)

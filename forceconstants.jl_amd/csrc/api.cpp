@@ -181,6 +181,7 @@ int fc3_destroy(fc3_handle h)
     hipSetDevice(h->device);
     hipStreamSynchronize(h->stream);
     release_blas(h);
+    release_batch(h);
     if (h->ws) hipFree(h->ws);
     for (auto& r : h->prof.records) {
         hipEventDestroy(r.start);

@@ -1,0 +1,389 @@
+// Batched snapshots: IFC3 assembly (LJ, mass-weighted FP32) followed by mcc3 for many
+// displaced supercells of uneven size, issued as one captured HIP graph.
+//
+// The reference handles one system per call (src/ifc.jl:20-35, src/mcc.jl:29-51); a
+// trajectory of snapshots is a loop of third_order + mcc3 calls.  Here one call takes
+// the whole list:
+//   plan     (host, synchronising): upload positions and masses, count neighbours per
+//            snapshot, size one workspace for all snapshots' neighbour lists and pair
+//            blocks plus one shared Psi buffer and one mcc3 workspace (largest snapshot).
+//   tune     (optional): for every snapshot size not seen before, time each applicable
+//            mcc3 configuration (kernel family x launch size) on that size's first
+//            snapshot and keep the fastest; the table can be saved to and loaded from a
+//            text file.
+//   capture  every snapshot's kernels (neighbour fill, pair and self blocks, the dense
+//            writer into the shared Psi, the mode-product GEMMs into the snapshot's K3)
+//            into one graph, instantiated once and cached for the same plan.
+//   launch   the graph: one launch for the whole batch.
+// The graph holds no allocation, memset or host synchronisation, so replays are valid
+// as long as the plan (sizes, pointers, configurations) is unchanged; a call whose plan
+// differs captures a new graph.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "fc3_internal.hpp"
+
+namespace fc3 {
+
+struct BatchState {
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    std::vector<char> key;          // plan the cached graph was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t nodes = 0;
+    struct Tuned {
+        Mcc3Config cfg;
+        float ms = 0.0f;
+    };
+    std::map<std::pair<int, int>, Tuned> tuned;   // (n, precision) -> configuration
+};
+
+void release_batch(fc3_context* h)
+{
+    if (!h || !h->batch) return;
+    BatchState* b = static_cast<BatchState*>(h->batch);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->ws) hipFree(b->ws);
+    delete b;
+    h->batch = nullptr;
+}
+
+static BatchState* state(fc3_context* h)
+{
+    if (!h->batch) h->batch = new BatchState();
+    return static_cast<BatchState*>(h->batch);
+}
+
+namespace {
+
+struct SnapPlan {
+    int N = 0, cap = 0;
+    long n = 0;
+    size_t off_pos = 0, off_m = 0, off_count = 0, off_nbr = 0, off_rev = 0, off_E3 = 0, off_E2 = 0,
+           off_S3 = 0;
+    Mcc3Config cfg;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+template <class T>
+void put(std::vector<char>& k, const T& v)
+{
+    const char* p = reinterpret_cast<const char*>(&v);
+    k.insert(k.end(), p, p + sizeof(T));
+}
+
+struct Params {
+    double sigma, epsilon, pot_r_cut, r_cut, tol;
+    int precision;
+};
+
+// Kernels of one snapshot on stream s (no synchronisation, no allocation).
+int snapshot_work(hipStream_t s, char* base, const SnapPlan& P, const fc3_snapshot& sn,
+                  const Params& prm, float* psi32, void* mws, size_t mws_bytes)
+{
+    NeighborList nl;
+    nl.n_atoms = P.N;
+    nl.cap = P.cap;
+    nl.count = reinterpret_cast<int*>(base + P.off_count);
+    nl.nbr = reinterpret_cast<int*>(base + P.off_nbr);
+    nl.rev = reinterpret_cast<int*>(base + P.off_rev);
+    const double* pos = reinterpret_cast<const double*>(base + P.off_pos);
+    const double* m = sn.masses ? reinterpret_cast<const double*>(base + P.off_m) : nullptr;
+    double* E3 = reinterpret_cast<double*>(base + P.off_E3);
+    double* E2 = reinterpret_cast<double*>(base + P.off_E2);
+    double* S3 = reinterpret_cast<double*>(base + P.off_S3);
+    FC3_TRY(neighbor_fill(s, pos, P.N, sn.box, prm.r_cut, nl));
+    FC3_TRY(lj_pair_blocks(s, pos, sn.box, prm.sigma, prm.epsilon, nl, E3, S3, E2, nullptr));
+    Ifc3Blocks b;
+    b.E = E3;
+    b.S = S3;
+    FC3_TRY(write_ifc3_pairwise(s, nl, b, prm.tol, m, nullptr, psi32));
+    return mcc3_run(s, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &P.cfg);
+}
+
+}  // namespace
+
+static int tune_size(fc3_context* h, BatchState* B, char* base, const SnapPlan& P,
+                     const fc3_snapshot& sn, const Params& prm, float* psi32, void* mws,
+                     size_t mws_bytes)
+{
+    // assemble Psi once, then time each configuration's mcc3 (after one warm-up run)
+    SnapPlan Q = P;
+    Q.cfg = Mcc3Config{};
+    FC3_TRY(snapshot_work(h->stream, base, Q, sn, prm, psi32, mws, mws_bytes));
+    std::vector<Mcc3Config> cands;
+    if (mcc3_variant_applies((int)P.n, prm.precision, 3))
+        for (long t : {8192L, 16384L, 32768L}) cands.push_back(Mcc3Config{3, t});
+    cands.push_back(Mcc3Config{2, 0});
+    BatchState::Tuned best;
+    best.ms = -1.0f;
+    for (const Mcc3Config& c : cands) {
+        if (mcc3_workspace_bytes((int)P.n, prm.precision, false, &c) > mws_bytes) continue;
+        FC3_TRY(mcc3_run(h->stream, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &c));
+        FC3_HIP(hipEventRecord(h->ev_start, h->stream));
+        FC3_TRY(mcc3_run(h->stream, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &c));
+        FC3_HIP(hipEventRecord(h->ev_stop, h->stream));
+        FC3_HIP(hipEventSynchronize(h->ev_stop));
+        float ms = 0.0f;
+        FC3_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+        if (best.ms < 0.0f || ms < best.ms) {
+            best.ms = ms;
+            best.cfg = c;
+        }
+    }
+    B->tuned[{(int)P.n, prm.precision}] = best;
+    return FC3_OK;
+}
+
+}  // namespace fc3
+
+using namespace fc3;
+
+extern "C" {
+
+int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int count,
+                                  double sigma, double epsilon, double pot_r_cut, double r_cut,
+                                  double tol, int precision, int flags)
+{
+    if (!h) return set_error(FC3_ERR_ARGUMENT, "null handle");
+    FC3_HIP(hipSetDevice(h->device));
+    set_current(h);
+    if (!snaps || count < 1) return set_error(FC3_ERR_ARGUMENT, "snaps must hold count >= 1 entries");
+    if (precision != FC3_PREC_FP32 && precision != FC3_PREC_BF16X3)
+        return set_error(FC3_ERR_ARGUMENT, "unknown precision");
+    const Params prm{sigma, epsilon, pot_r_cut, r_cut, tol, precision};
+    for (int i = 0; i < count; ++i) {
+        const fc3_snapshot& s = snaps[i];
+        if (!s.positions || !s.phi || !s.k3 || s.n_atoms < 1)
+            return set_error(FC3_ERR_ARGUMENT, "snapshot " + std::to_string(i) +
+                                                   ": positions, phi, k3 must be non-null, n_atoms >= 1");
+        if (3LL * s.n_atoms > 65535) return set_error(FC3_ERR_UNSUPPORTED, "3N must be <= 65535");
+        for (int c = 0; c < 3; ++c)
+            if (!(pot_r_cut < s.box[c]))
+                return set_error(FC3_ERR_CUTOFF, "Cutoff larger than L/2");
+        if (s.masses)
+            for (int a = 0; a < s.n_atoms; ++a)
+                if (!(s.masses[a] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    }
+    BatchState* B = state(h);
+    hipStream_t st = h->stream;
+
+    // ---- plan: layout of the per-snapshot region (positions, masses, counts first) ----
+    std::vector<SnapPlan> P(count);
+    size_t head = 0;
+    long max_n = 0;
+    for (int i = 0; i < count; ++i) {
+        SnapPlan& p = P[i];
+        p.N = snaps[i].n_atoms;
+        p.n = 3L * p.N;
+        max_n = std::max(max_n, p.n);
+        p.off_pos = head;
+        head = align256(head + sizeof(double) * 3 * p.N);
+        p.off_m = head;
+        head = align256(head + sizeof(double) * p.N);
+        p.off_count = head;
+        head = align256(head + sizeof(int) * (p.N + 1));
+    }
+    // neighbour counts in the handle's workspace (positions uploaded there first)
+    FC3_TRY(ensure_workspace(h, head + 256));
+    {
+        char* w = static_cast<char*>(h->ws);
+        for (int i = 0; i < count; ++i)
+            FC3_HIP(hipMemcpyAsync(w + P[i].off_pos, snaps[i].positions,
+                                   sizeof(double) * 3 * P[i].N, hipMemcpyHostToDevice, st));
+        for (int i = 0; i < count; ++i) {
+            int hmax = 0;
+            int* cnt = reinterpret_cast<int*>(w + P[i].off_count);
+            FC3_TRY(neighbor_count(st, reinterpret_cast<double*>(w + P[i].off_pos), P[i].N,
+                                   snaps[i].box, r_cut, cnt, cnt + P[i].N, &hmax));
+            P[i].cap = hmax > 0 ? hmax : 1;
+        }
+    }
+    size_t off = head;
+    size_t max_mws = 0;
+    for (int i = 0; i < count; ++i) {
+        SnapPlan& p = P[i];
+        const size_t slots = (size_t)p.N * p.cap;
+        p.off_nbr = off;
+        off = align256(off + sizeof(int) * slots);
+        p.off_rev = off;
+        off = align256(off + sizeof(int) * slots);
+        p.off_E3 = off;
+        off = align256(off + sizeof(double) * 27 * slots);
+        p.off_E2 = off;
+        off = align256(off + sizeof(double) * 9 * slots);
+        p.off_S3 = off;
+        off = align256(off + sizeof(double) * 27 * p.N);
+    }
+    // configurations: tuned ones where known (tuning needs the buffers, so it runs below)
+    const bool tune = flags & FC3_BATCH_AUTOTUNE;
+    for (int i = 0; i < count; ++i) {
+        auto it = B->tuned.find({(int)P[i].n, precision});
+        if (it != B->tuned.end()) P[i].cfg = it->second.cfg;
+        // size the mcc3 workspace for every candidate a tuning run may try
+        for (int v : {2, 3})
+            for (long t : {8192L, 16384L, 32768L}) {
+                Mcc3Config c{v, t};
+                if (!mcc3_variant_applies((int)P[i].n, precision, v)) continue;
+                max_mws = std::max(max_mws, mcc3_workspace_bytes((int)P[i].n, precision, false, &c));
+            }
+    }
+    const size_t off_psi = off;
+    off = align256(off + sizeof(float) * (size_t)max_n * max_n * max_n);
+    const size_t off_mws = off;
+    off = align256(off + max_mws);
+    if (off > B->ws_bytes) {
+        if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+        if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
+        B->key.clear();
+        if (B->ws) { FC3_HIP(hipStreamSynchronize(st)); FC3_HIP(hipFree(B->ws)); B->ws = nullptr; B->ws_bytes = 0; }
+        const size_t want = (off + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+        hipError_t e = hipMalloc(&B->ws, want);
+        if (e != hipSuccess) {
+            B->ws = nullptr;
+            return set_error(FC3_ERR_OUT_OF_MEMORY, "batch workspace of " + std::to_string(want) +
+                                                        " bytes: " + hipGetErrorString(e));
+        }
+        B->ws_bytes = want;
+    }
+    char* base = static_cast<char*>(B->ws);
+    // inputs and neighbour counts into the batch workspace (outside the graph)
+    for (int i = 0; i < count; ++i) {
+        const SnapPlan& p = P[i];
+        FC3_HIP(hipMemcpyAsync(base + p.off_pos, snaps[i].positions, sizeof(double) * 3 * p.N,
+                               hipMemcpyHostToDevice, st));
+        if (snaps[i].masses)
+            FC3_HIP(hipMemcpyAsync(base + p.off_m, snaps[i].masses, sizeof(double) * p.N,
+                                   hipMemcpyHostToDevice, st));
+        int hmax = 0;
+        int* cnt = reinterpret_cast<int*>(base + p.off_count);
+        FC3_TRY(neighbor_count(st, reinterpret_cast<double*>(base + p.off_pos), p.N, snaps[i].box,
+                               r_cut, cnt, cnt + p.N, &hmax));
+        if (hmax > p.cap) return set_error(FC3_ERR_CAPACITY, "neighbour count changed between passes");
+    }
+    float* psi32 = reinterpret_cast<float*>(base + off_psi);
+    void* mws = base + off_mws;
+
+    // ---- tune sizes not seen before ----
+    const bool prof_was = h->prof.enabled;
+    if (tune) {
+        h->prof.enabled = false;
+        for (int i = 0; i < count; ++i) {
+            if (B->tuned.count({(int)P[i].n, precision})) continue;
+            FC3_TRY(tune_size(h, B, base, P[i], snaps[i], prm, psi32, mws, max_mws));
+        }
+        h->prof.enabled = prof_was;
+        for (int i = 0; i < count; ++i) P[i].cfg = B->tuned[{(int)P[i].n, precision}].cfg;
+    }
+
+    if (flags & FC3_BATCH_NO_GRAPH) {
+        for (int i = 0; i < count; ++i)
+            FC3_TRY(snapshot_work(st, base, P[i], snaps[i], prm, psi32, mws, max_mws));
+        return FC3_OK;
+    }
+
+    // ---- graph: reuse when the plan is unchanged, else capture ----
+    std::vector<char> key;
+    put(key, B->ws);
+    put(key, prm.sigma);
+    put(key, prm.epsilon);
+    put(key, prm.pot_r_cut);
+    put(key, prm.r_cut);
+    put(key, prm.tol);
+    put(key, prm.precision);
+    put(key, count);
+    for (int i = 0; i < count; ++i) {
+        put(key, P[i].N);
+        put(key, P[i].cap);
+        put(key, P[i].cfg.variant);
+        put(key, P[i].cfg.tiles);
+        put(key, snaps[i].phi);
+        put(key, snaps[i].k3);
+        put(key, snaps[i].masses != nullptr);
+        put(key, snaps[i].box);
+    }
+    if (!B->exec || key != B->key) {
+        if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+        if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
+        h->prof.enabled = false;   // no event records inside the capture
+        FC3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        int rc = FC3_OK;
+        for (int i = 0; i < count && rc == FC3_OK; ++i)
+            rc = snapshot_work(st, base, P[i], snaps[i], prm, psi32, mws, max_mws);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        h->prof.enabled = prof_was;
+        if (rc != FC3_OK) { if (g) hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return set_error(FC3_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        B->graph = g;
+        FC3_HIP(hipGraphGetNodes(g, nullptr, &B->nodes));
+        FC3_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+        B->key = key;
+    }
+    FC3_HIP(hipGraphLaunch(B->exec, st));
+    return FC3_OK;
+}
+
+int fc3_batch_graph_nodes(fc3_handle h, size_t* nodes)
+{
+    if (!h || !nodes) return set_error(FC3_ERR_ARGUMENT, "null argument");
+    *nodes = h->batch ? static_cast<BatchState*>(h->batch)->nodes : 0;
+    return FC3_OK;
+}
+
+int fc3_autotune_query(fc3_handle h, int n, int precision, int* variant, long* tiles, float* ms)
+{
+    if (!h || !variant || !tiles || !ms) return set_error(FC3_ERR_ARGUMENT, "null argument");
+    BatchState* B = state(h);
+    auto it = B->tuned.find({n, precision});
+    if (it == B->tuned.end()) return set_error(FC3_ERR_ARGUMENT, "size not tuned");
+    *variant = it->second.cfg.variant;
+    *tiles = it->second.cfg.tiles;
+    *ms = it->second.ms;
+    return FC3_OK;
+}
+
+int fc3_autotune_save(fc3_handle h, const char* path)
+{
+    if (!h || !path) return set_error(FC3_ERR_ARGUMENT, "null argument");
+    BatchState* B = state(h);
+    FILE* f = std::fopen(path, "w");
+    if (!f) return set_error(FC3_ERR_ARGUMENT, std::string("cannot write ") + path);
+    std::fprintf(f, "# n precision variant tiles ms\n");
+    for (const auto& kv : B->tuned)
+        std::fprintf(f, "%d %d %d %ld %.6f\n", kv.first.first, kv.first.second,
+                     kv.second.cfg.variant, kv.second.cfg.tiles, kv.second.ms);
+    std::fclose(f);
+    return FC3_OK;
+}
+
+int fc3_autotune_load(fc3_handle h, const char* path)
+{
+    if (!h || !path) return set_error(FC3_ERR_ARGUMENT, "null argument");
+    BatchState* B = state(h);
+    FILE* f = std::fopen(path, "r");
+    if (!f) return set_error(FC3_ERR_ARGUMENT, std::string("cannot read ") + path);
+    char line[256];
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        int n, prec, var;
+        long tiles;
+        float ms;
+        if (std::sscanf(line, "%d %d %d %ld %f", &n, &prec, &var, &tiles, &ms) != 5) continue;
+        if (!mcc3_variant_applies(n, prec, var)) continue;
+        BatchState::Tuned t;
+        t.cfg = Mcc3Config{var, tiles};
+        t.ms = ms;
+        B->tuned[{n, prec}] = t;
+    }
+    std::fclose(f);
+    return FC3_OK;
+}
+
+}  // extern "C"

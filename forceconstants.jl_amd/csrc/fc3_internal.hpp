@@ -47,6 +47,7 @@ struct Profiler {
 struct fc3_context {
     fc3::Profiler prof;
     void* blas = nullptr;   // rocBLAS handle, created on first use (adjoint.hip)
+    void* batch = nullptr;  // batched-snapshot state: workspace, graph, tuning (batch.cpp)
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -133,14 +134,23 @@ int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const d
                 double tol, const double* d_masses, double* psi64, float* psi32, double* ifc2);
 
 // ---- mcc3 (mcc3.hip) --------------------------------------------------------------
-size_t mcc3_workspace_bytes(int n, int precision, bool inplace);
+// Kernel family and launch sizing of one mcc3: variant 2 = FP32-operand kernels (any n),
+// 3 = pre-split BF16x3 kernels (BF16X3 with n % 16 == 0), 0 = default (3 when it
+// applies); tiles = target workgroups per GEMM launch (0 = default).
+struct Mcc3Config {
+    int variant = 0;
+    long tiles = 0;
+};
+bool mcc3_variant_applies(int n, int precision, int variant);
+size_t mcc3_workspace_bytes(int n, int precision, bool inplace, const Mcc3Config* cfg = nullptr);
 int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int precision,
-             float* k3, void* ws, size_t ws_bytes);
+             float* k3, void* ws, size_t ws_bytes, const Mcc3Config* cfg = nullptr);
 int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int precision,
                      void* ws, size_t ws_bytes);
 
 // ---- adjoints (adjoint.hip) ---------------------------------------------------------
 void release_blas(fc3_context* h);
+void release_batch(fc3_context* h);
 size_t mcc3_backward_workspace_bytes(int n, int precision, bool need_dphi);
 // dpsi (n^3) and/or dphi (n x n), device FP32; uses h's workspace (sized by the above).
 int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const float* G, int n,

@@ -14,6 +14,8 @@
 //   BF16X3:  a*b ~= ah*bh + ah*bl + al*bh with ah = bf16_rne(a), al = bf16_rne(a - ah),
 //            on v_mfma_f32_32x32x16_bf16, FP32 accumulation.  Intermediates (U, S)
 //            are FP32 and are split again when they are read.
+#include <algorithm>
+
 #include "fc3_internal.hpp"
 
 namespace fc3 {
@@ -954,19 +956,35 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
     return FC3_OK;
 }
 
-// Mode-3 row block: a multiple of n rows (whole j columns of Psi(i, j, :)), enough
-// workgroups to fill the chip, X scratch bounded to 2 GiB.
-long mode3_rows(int n)
+// Launch sizing for the pre-split path.  A launch needs about 16k workgroups (64 per
+// CU) to run at full rate: smaller ones lose 5-15% to ramp-up and tail.  Batches are
+// equalised so that the last one is not a small remainder.
+constexpr long kTilesPerLaunch = 16384;   // default; the batch autotuner may choose others
+constexpr size_t kScratchSlabs = size_t(4) << 30, kScratchRows = size_t(8) << 30;
+constexpr size_t kScratchSlabsInplace = size_t(1) << 30, kScratchRowsInplace = size_t(2) << 30;
+
+// Slabs per mode-1/2 launch; each slab takes 4 n^2 bytes of split scratch.
+int slab_batch3(int n, size_t cap_bytes, long tiles_per_launch)
 {
-    const long n2 = (long)n * n;
     const long tiles_n = (n + v3::BN - 1) / v3::BN;
-    const long target = (8192 + tiles_n - 1) / tiles_n * v3::BM;
-    long R = (target + n - 1) / n * n;
-    const long cap = ((1L << 31) / (4L * n)) / n * n;
-    if (R > cap) R = cap;
-    if (R < n) R = n;
-    if (R > n2) R = n2;
-    return R;
+    long b = (tiles_per_launch * v3::BM + tiles_n * n - 1) / (tiles_n * n);
+    const long cap = (long)(cap_bytes / (4 * (size_t)n * n));
+    b = std::max(1L, std::min({b, cap, (long)n}));
+    const long batches = (n + b - 1) / b;
+    return (int)((n + batches - 1) / batches);
+}
+
+// Mode-3 row block: whole columns j of Psi(i, j, :) (a multiple of n rows); the split
+// copy X takes 4 n bytes per row.
+long mode3_rows(int n, size_t cap_bytes, long tiles_per_launch)
+{
+    const long tiles_n = (n + v3::BN - 1) / v3::BN;
+    const long target = (tiles_per_launch + tiles_n - 1) / tiles_n * v3::BM;
+    long u = (target + n - 1) / n;                                   // columns j per block
+    const long cap = (long)(cap_bytes / (4 * (size_t)n)) / n;
+    u = std::max(1L, std::min({u, cap, (long)n}));
+    const long blocks = (n + u - 1) / u;
+    return (n + blocks - 1) / blocks * (long)n;
 }
 
 // Psi rows [r0, r0 + rows) -> X (split, K-blocked) -> C(m, l) for those rows.
@@ -998,23 +1016,25 @@ int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, 
                  (long)cnt * n, n, n);
 }
 
-size_t mcc3_v3_workspace_bytes(int n, bool inplace)
+size_t mcc3_v3_workspace_bytes(int n, bool inplace, long tiles)
 {
     const long n2 = (long)n * n;
-    const size_t slabs = Carve::need(sizeof(float) * n2 * slab_batch(n));
-    return phi3_bytes(n) + slabs * (inplace ? 2 : 1) +
-           Carve::need(sizeof(float) * (size_t)mode3_rows(n) * n) + 4096;
+    const int nb = slab_batch3(n, inplace ? kScratchSlabsInplace : kScratchSlabs, tiles);
+    const long R = mode3_rows(n, inplace ? kScratchRowsInplace : kScratchRows, tiles);
+    const size_t slabs = Carve::need(sizeof(float) * n2 * nb);
+    return phi3_bytes(n) + slabs * (inplace ? 2 : 1) + Carve::need(sizeof(float) * (size_t)R * n) +
+           4096;
 }
 
 int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3, void* ws,
-            size_t ws_bytes)
+            size_t ws_bytes, long tiles)
 {
     Carve cv(ws, ws_bytes);
     Split3 B;
     FC3_TRY(prep_phi3(s, phi, n, cv, B));
     const long n2 = (long)n * n;
-    const int nb = slab_batch(n);
-    const long R = mode3_rows(n);
+    const int nb = slab_batch3(n, kScratchSlabs, tiles);
+    const long R = mode3_rows(n, kScratchRows, tiles);
     __bf16* S = cv.take<__bf16>((size_t)2 * n2 * nb);
     __bf16* X = cv.take<__bf16>((size_t)2 * R * n);
     __bf16* U = reinterpret_cast<__bf16*>(k3);
@@ -1038,8 +1058,8 @@ int mcc3_v3_inplace(hipStream_t s, float* psi, const float* phi, int n, void* ws
     Split3 B;
     FC3_TRY(prep_phi3(s, phi, n, cv, B));
     const long n2 = (long)n * n;
-    const int nb = slab_batch(n);
-    const long R = mode3_rows(n);
+    const int nb = slab_batch3(n, kScratchSlabsInplace, kTilesPerLaunch);
+    const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
     __bf16* Y = cv.take<__bf16>((size_t)2 * n2 * nb);
     __bf16* S = cv.take<__bf16>((size_t)2 * n2 * nb);
     __bf16* X = cv.take<__bf16>((size_t)2 * R * n);
@@ -1063,9 +1083,10 @@ int mcc3_v3_inplace(hipStream_t s, float* psi, const float* phi, int n, void* ws
 
 }  // namespace
 
-size_t mcc3_workspace_bytes(int n, int precision, bool inplace)
+size_t mcc3_workspace_bytes(int n, int precision, bool inplace, const Mcc3Config* cfg)
 {
     const long n2 = (long)n * n;
+    const long tiles = cfg && cfg->tiles > 0 ? cfg->tiles : kTilesPerLaunch;
     size_t b = phi_bytes(n, precision) + Carve::need(sizeof(float) * n2 * slab_batch(n)) + 4096;
     if (inplace) {
         // mode-3 row-block scratch: R rows x n
@@ -1073,15 +1094,22 @@ size_t mcc3_workspace_bytes(int n, int precision, bool inplace)
         b += Carve::need(sizeof(float) * (size_t)(R < n2 ? R : n2) * n);
     }
     // the pre-split path is taken only for 16-byte aligned tensors; size for either path
-    if (v3_applies(n, precision)) b = std::max(b, mcc3_v3_workspace_bytes(n, inplace));
+    if (v3_applies(n, precision)) b = std::max(b, mcc3_v3_workspace_bytes(n, inplace, tiles));
     return b;
 }
 
-int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int precision, float* k3,
-             void* ws, size_t ws_bytes)
+bool mcc3_variant_applies(int n, int precision, int variant)
 {
-    if (v3_applies(n, precision) && aligned16(psi) && aligned16(k3))
-        return mcc3_v3(s, psi, phi, n, k3, ws, ws_bytes);
+    return variant == 2 || (variant == 3 && v3_applies(n, precision));
+}
+
+int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int precision, float* k3,
+             void* ws, size_t ws_bytes, const Mcc3Config* cfg)
+{
+    const int variant = cfg && cfg->variant ? cfg->variant : 3;
+    const long tiles = cfg && cfg->tiles > 0 ? cfg->tiles : kTilesPerLaunch;
+    if (variant == 3 && v3_applies(n, precision) && aligned16(psi) && aligned16(k3))
+        return mcc3_v3(s, psi, phi, n, k3, ws, ws_bytes, tiles);
     Carve cv(ws, ws_bytes);
     PhiPrep B;
     FC3_TRY(prep_phi(s, phi, n, precision, cv, B));
@@ -1101,7 +1129,7 @@ int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int pre
                      size_t ws_bytes)
 {
     if (v3_applies(n, precision) && aligned16(psi))
-        return mcc3_v3_inplace(s, psi, phi, n, ws, ws_bytes);
+        return mcc3_v3_inplace(s, psi, phi, n, ws, ws_bytes);   // default launch sizing
     Carve cv(ws, ws_bytes);
     PhiPrep B;
     FC3_TRY(prep_phi(s, phi, n, precision, cv, B));

@@ -5,7 +5,8 @@ mass_weight_third_order!, mcc3) backed by hand-written HIP kernels for gfx950 in
 libfc3.so.  See api.py for the reference-named entry points.
 """
 from . import systems
-from .api import (Context, DenseForceConstants, DeviceArray, dynamical_matrix, get_modes,
+from .api import (autotune_load, autotune_query, autotune_save, batch_graph_nodes,
+                  batch_third_order_mcc3, Context, DenseForceConstants, DeviceArray, dynamical_matrix, get_modes,
                   mass_weight_third_order_, mcc3, mcc3_, mcc3_backward, mcc3_rrule,
                   second_order, second_order_device, third_order, third_order_device,
                   third_order_param_backward)
@@ -13,6 +14,8 @@ from .systems import (AnalyticalCalculator, AutoDiffCalculator, LJ, StillingerWe
                       SuperCellSystem, SWParams, diamond, fcc)
 
 __all__ = [
+    "autotune_load", "autotune_query", "autotune_save", "batch_graph_nodes",
+    "batch_third_order_mcc3",
     "Context", "DenseForceConstants", "DeviceArray", "dynamical_matrix", "get_modes",
     "mass_weight_third_order_", "mcc3", "mcc3_", "mcc3_backward", "mcc3_rrule",
     "third_order_param_backward", "second_order", "second_order_device",

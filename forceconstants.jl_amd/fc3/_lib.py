@@ -33,6 +33,16 @@ _fp = ctypes.POINTER(ctypes.c_float)
 _handle = ctypes.c_void_p
 
 # name -> (restype, argtypes); kept in the order of include/fc3.h
+class Snapshot(ctypes.Structure):
+    """fc3_snapshot of fc3.h"""
+    _fields_ = [("positions", ctypes.POINTER(ctypes.c_double)), ("n_atoms", ctypes.c_int),
+                ("box", ctypes.c_double * 3), ("masses", ctypes.POINTER(ctypes.c_double)),
+                ("phi", ctypes.c_void_p), ("k3", ctypes.c_void_p)]
+
+
+FC3_BATCH_AUTOTUNE = 1
+FC3_BATCH_NO_GRAPH = 2
+
 SIGNATURES = {
     "fc3_version": (_c_int, []),
     "fc3_last_error": (ctypes.c_char_p, []),
@@ -63,6 +73,14 @@ SIGNATURES = {
     "fc3_mcc3": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _vp]),
     "fc3_mcc3_inplace": (_c_int, [_handle, _vp, _vp, _c_int, _c_int]),
     "fc3_mcc3_blocked": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _c_int, _fp]),
+    "fc3_batch_third_order_mcc3_lj": (_c_int, [_handle, ctypes.POINTER(Snapshot), _c_int, _c_double,
+                                               _c_double, _c_double, _c_double, _c_double, _c_int,
+                                               _c_int]),
+    "fc3_batch_graph_nodes": (_c_int, [_handle, ctypes.POINTER(_c_size)]),
+    "fc3_autotune_query": (_c_int, [_handle, _c_int, _c_int, ctypes.POINTER(_c_int),
+                                    ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_float)]),
+    "fc3_autotune_save": (_c_int, [_handle, ctypes.c_char_p]),
+    "fc3_autotune_load": (_c_int, [_handle, ctypes.c_char_p]),
     "fc3_mcc3_backward": (_c_int, [_handle, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp]),
     "fc3_third_order_lj_param_backward": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double,
                                                    _c_double, _c_double, _c_double, _dp, _vp, _vp,

@@ -337,3 +337,68 @@ def third_order_param_backward(sys: SuperCellSystem, pot, calc, dpsi: DeviceArra
         ctx.h, _ptr(pos), sys.n_atoms, _ptr(L), pot.sigma, pot.epsilon, pot.r_cut, calc.r_cut,
         calc.tol, _ptr(m) if m is not None else None, d32, d64, _ptr(g)))
     return {"sigma": float(g[0]), "epsilon": float(g[1])}
+
+
+# ---- batched snapshots ------------------------------------------------------------------
+
+def batch_third_order_mcc3(snapshots, pot, calc, precision="bf16x3", autotune: bool = True,
+                           graph: bool = True, ctx: Optional[Context] = None):
+    """For each (sys, phi, k3) in `snapshots`: Psi = third_order(sys, pot, calc) mass
+    weighted to FP32 with sys.masses, then k3 <- mcc3(Psi, phi), all in one graph launch.
+    phi, k3: device FP32 arrays (n x n, n x n x n).  Returns once the work is enqueued."""
+    if not isinstance(pot, LJ):
+        raise NotImplementedError("batched assembly implemented for LJ")
+    snaps = list(snapshots)
+    if not snaps:
+        raise AssertionError("empty batch")
+    ctx = ctx or snaps[0][1].ctx
+    arr = (_lib.Snapshot * len(snaps))()
+    keep = []
+    for i, (sys, phi, k3) in enumerate(snaps):
+        n = sys.n_dof
+        if phi.shape != (n, n) or k3.shape != (n, n, n):
+            raise AssertionError(f"snapshot {i}: phi must be n x n and k3 n x n x n (n = {n})")
+        if phi.dtype != np.float32 or k3.dtype != np.float32:
+            raise TypeError("phi and k3 must be float32 device arrays")
+        pos, L = _sys_args(sys)
+        m = np.ascontiguousarray(sys.masses, dtype=np.float64)
+        keep += [pos, m]
+        arr[i].positions = _ptr(pos)
+        arr[i].n_atoms = sys.n_atoms
+        for c in range(3):
+            arr[i].box[c] = float(L[c])
+        arr[i].masses = _ptr(m)
+        arr[i].phi = phi.ptr
+        arr[i].k3 = k3.ptr
+    flags = (_lib.FC3_BATCH_AUTOTUNE if autotune else 0) | (0 if graph else _lib.FC3_BATCH_NO_GRAPH)
+    check(ctx.lib.fc3_batch_third_order_mcc3_lj(ctx.h, arr, len(snaps), pot.sigma, pot.epsilon,
+                                                pot.r_cut, calc.r_cut, calc.tol, _prec(precision),
+                                                flags))
+    del keep
+
+
+def batch_graph_nodes(ctx: Optional[Context] = None) -> int:
+    ctx = ctx or Context.default()
+    v = ctypes.c_size_t()
+    check(ctx.lib.fc3_batch_graph_nodes(ctx.h, ctypes.byref(v)))
+    return int(v.value)
+
+
+def autotune_query(n: int, precision="bf16x3", ctx: Optional[Context] = None):
+    """(variant, tiles, ms) chosen for size n: variant 2 = FP32-operand kernels, 3 =
+    pre-split BF16x3 kernels."""
+    ctx = ctx or Context.default()
+    v, t, ms = ctypes.c_int(), ctypes.c_long(), ctypes.c_float()
+    check(ctx.lib.fc3_autotune_query(ctx.h, n, _prec(precision), ctypes.byref(v), ctypes.byref(t),
+                                     ctypes.byref(ms)))
+    return int(v.value), int(t.value), float(ms.value)
+
+
+def autotune_save(path: str, ctx: Optional[Context] = None):
+    ctx = ctx or Context.default()
+    check(ctx.lib.fc3_autotune_save(ctx.h, path.encode()))
+
+
+def autotune_load(path: str, ctx: Optional[Context] = None):
+    ctx = ctx or Context.default()
+    check(ctx.lib.fc3_autotune_load(ctx.h, path.encode()))

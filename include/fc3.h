@@ -155,6 +155,41 @@ int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int prec
 int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n,
                      int block_size, int precision, float* k3_host);
 
+/* ---- batched snapshots (no counterpart in the reference, which handles one
+ *      system per call: a loop of third_order (src/ifc.jl:20-35) and mcc3
+ *      (src/mcc.jl:29-51) over displaced snapshots) ---------------------------- */
+typedef struct {
+    const double* positions;   /* host, 3 * n_atoms doubles, Angstrom */
+    int n_atoms;
+    double box[3];             /* cubic/orthorhombic box, Angstrom */
+    const double* masses;      /* host, n_atoms (mass-weighted Psi), or NULL */
+    const float* phi;          /* device, n x n FP32 modes (n = 3 n_atoms) */
+    float* k3;                 /* device, n^3 FP32 output */
+} fc3_snapshot;
+
+enum {
+    FC3_BATCH_AUTOTUNE = 1,    /* time the mcc3 configurations of sizes not seen before */
+    FC3_BATCH_NO_GRAPH = 2     /* issue the kernels directly instead of one graph launch */
+};
+
+/* For every snapshot: Psi = third_order(LJ, AnalyticalCalculator) mass weighted to FP32
+ * (as fc3_third_order_lj's psi32), then k3 = mcc3(Psi, phi) at `precision`.  All
+ * snapshots run from one captured graph (cached for an unchanged plan: same sizes,
+ * pointers and configurations), with one shared Psi buffer of the largest size.
+ * Returns once the work is enqueued.                                              */
+int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int count,
+                                  double sigma, double epsilon, double pot_r_cut,
+                                  double r_cut, double tol, int precision, int flags);
+/* Nodes of the last captured batch graph. */
+int fc3_batch_graph_nodes(fc3_handle h, size_t* nodes);
+/* Tuning table: the chosen mcc3 configuration of size n (variant 2 = FP32-operand
+ * kernels, 3 = pre-split BF16x3 kernels; tiles = workgroups per GEMM launch) and its
+ * time; save/load as text lines "n precision variant tiles ms".                   */
+int fc3_autotune_query(fc3_handle h, int n, int precision, int* variant, long* tiles,
+                       float* ms);
+int fc3_autotune_save(fc3_handle h, const char* path);
+int fc3_autotune_load(fc3_handle h, const char* path);
+
 /* ---- adjoints (no counterpart in the reference; its rrule targets) ----------- */
 /* Reverse mode of mcc3 (src/mcc.jl:15-17) for a cotangent dk3 = dL/dK3 (device n^3):
  *   dpsi = dL/dPsi = dk3 x1 phi x2 phi x3 phi            (device n^3, or NULL)

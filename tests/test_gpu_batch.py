@@ -1,0 +1,96 @@
+"""Batched snapshots (IFC3 assembly + mcc3 in one graph launch) on the GPU.
+
+Each snapshot's K3 is checked against the FP64 host contraction of the oracle's
+mass-weighted Psi for the same positions (relative Frobenius error <= 1e-5 for BF16x3,
+2e-6 for FP32).  The graph is checked against the same kernels issued directly
+(bitwise equal) and against its own replay (bitwise equal); the tuning table is
+checked to survive a save/load round trip.
+"""
+import numpy as np
+import pytest
+
+import fc3
+from fc3 import systems as S
+from oracle import fc3_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LJ = S.ARGON_LJ
+CALC = S.AnalyticalCalculator(1e-12, LJ.r_cut)
+TOL = {"fp32": 2e-6, "bf16x3": 1e-5}
+
+
+def _snapshots(ctx, spec):
+    """spec: list of (cells, seed); sigma_u ~ U[0.02, 0.10] from the seed."""
+    out = []
+    for cells, seed in spec:
+        su = float(np.random.default_rng(seed).uniform(0.02, 0.10))
+        sysd = S.fcc(S.ARGON_A, cells, S.ARGON_MASS, su, seed=seed)
+        f2 = O.lj_second_order(sysd, LJ, CALC)
+        _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+        phi32 = np.asfortranarray(phi.astype(np.float32))
+        n = sysd.n_dof
+        out.append((sysd, fc3.DeviceArray.from_host(phi32, ctx=ctx),
+                    fc3.DeviceArray((n, n, n), np.float32, ctx), phi32))
+    return out
+
+
+def _ref(sysd, phi32):
+    psi = O.lj_third_order(sysd, LJ, CALC)
+    O.mass_weight_third_order(psi, sysd.masses)
+    return O.mcc3_fp64(psi.astype(np.float32), phi32)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a.astype(np.float64) - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+def test_batch_uneven_sizes_match_fp64(ctx, prec):
+    spec = [((2, 2, 2), 1), ((3, 3, 3), 2), ((2, 2, 2), 3), ((3, 3, 3), 5)]
+    snaps = _snapshots(ctx, spec)
+    fc3.batch_third_order_mcc3([s[:3] for s in snaps], LJ, CALC, precision=prec, ctx=ctx)
+    ctx.synchronize()
+    print(f"{prec}: graph nodes {fc3.batch_graph_nodes(ctx)}", flush=True)
+    for (sysd, _, k3, phi32), (cells, seed) in zip(snaps, spec):
+        err = _rel(k3.to_host(), _ref(sysd, phi32))
+        print(f"{prec} snapshot {cells} seed {seed}: rel fro err {err:.3e}", flush=True)
+        assert err <= TOL[prec]
+
+
+def test_batch_graph_replay_and_direct_launches_agree(ctx):
+    spec = [((2, 2, 2), 11), ((3, 3, 3), 12), ((4, 4, 4), 13)]
+    snaps = _snapshots(ctx, spec)
+    args = [s[:3] for s in snaps]
+    fc3.batch_third_order_mcc3(args, LJ, CALC, ctx=ctx)            # tunes, captures, launches
+    ctx.synchronize()
+    first = [s[2].to_host() for s in snaps]
+    nodes = fc3.batch_graph_nodes(ctx)
+    for s in snaps:
+        s[2].to_host()
+    fc3.batch_third_order_mcc3(args, LJ, CALC, ctx=ctx)            # same plan: replay
+    ctx.synchronize()
+    assert fc3.batch_graph_nodes(ctx) == nodes
+    for a, s in zip(first, snaps):
+        assert np.array_equal(a, s[2].to_host())
+    fc3.batch_third_order_mcc3(args, LJ, CALC, graph=False, ctx=ctx)   # direct launches
+    ctx.synchronize()
+    for a, s in zip(first, snaps):
+        assert np.array_equal(a, s[2].to_host())
+
+
+def test_autotune_table_roundtrip(ctx, tmp_path):
+    snaps = _snapshots(ctx, [((4, 4, 4), 21)])
+    fc3.batch_third_order_mcc3([s[:3] for s in snaps], LJ, CALC, ctx=ctx)
+    ctx.synchronize()
+    v, t, ms = fc3.autotune_query(768, "bf16x3", ctx=ctx)
+    print(f"n=768 bf16x3: variant {v} tiles {t} {ms:.3f} ms", flush=True)
+    assert v in (2, 3) and ms > 0
+    path = str(tmp_path / "tune.txt")
+    fc3.autotune_save(path, ctx=ctx)
+    other = fc3.Context(ctx.device)
+    try:
+        fc3.autotune_load(path, ctx=other)
+        assert fc3.autotune_query(768, "bf16x3", ctx=other)[:2] == (v, t)
+    finally:
+        other.close()

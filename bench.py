@@ -108,6 +108,82 @@ def cpu_baseline_assembly():
             "sample": f"LJ FCC 4x4x4 ({n} DoF) third_order, dense FP64, {dt:.1f} s"}
 
 
+def batched_snapshots(ctx, per_size=32, replays=3):
+    """The batched configuration: 3 x `per_size` LJ argon snapshots of FCC 3x3x3,
+    4x4x4 and 5x5x5 (324, 768, 1500 DoF), sigma_u ~ U[0.02, 0.10] A per snapshot
+    (seeded), phi per snapshot from the host FP64 eigendecomposition of its dynamical
+    matrix; assembly + BF16x3 mcc3 of all snapshots in one graph launch.  K3 outputs
+    rotate through two buffers per size (the whole set would need 494 GB); the error
+    is checked on sampled slabs of the last two snapshots of each size."""
+    import fc3
+    from fc3 import systems as S
+    from oracle import fc3_oracle as O
+    calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
+    rng = np.random.default_rng(96)
+    out_bufs, snaps, meta = {}, [], []
+    t0 = time.perf_counter()
+    for cells in (3, 4, 5):
+        for j in range(per_size):
+            seed = int(rng.integers(1 << 30))
+            su = float(np.random.default_rng(seed).uniform(0.02, 0.10))
+            sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, su, seed=seed)
+            n = sysd.n_dof
+            _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
+            phi32 = np.asfortranarray(phi.astype(np.float32))
+            if n not in out_bufs:
+                out_bufs[n] = [fc3.DeviceArray((n, n, n), np.float32, ctx) for _ in range(2)]
+            snaps.append((sysd, fc3.DeviceArray.from_host(phi32, ctx=ctx), out_bufs[n][j % 2]))
+            meta.append((sysd, phi32, seed, su))
+        log(f"batched: phi for {per_size} snapshots of FCC {cells}^3 ready "
+            f"({time.perf_counter() - t0:.0f} s)")
+    fc3.batch_third_order_mcc3(snaps, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)  # tune + capture
+    ctx.synchronize()
+    nodes = fc3.batch_graph_nodes(ctx)
+    ctx.timer_start()
+    for _ in range(replays):
+        fc3.batch_third_order_mcc3(snaps, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)
+    t = ctx.timer_stop() / replays
+    nsnap = len(snaps)
+    flop = sum(6.0 * (3 * m[0].n_atoms) ** 4 for m in meta)
+    tune = {}
+    for n in sorted(out_bufs):
+        v, tiles, ms = fc3.autotune_query(n, "bf16x3", ctx=ctx)
+        tune[str(n)] = {"variant": v, "tiles": tiles, "mcc3_ms": ms}
+    # error of the surviving outputs (last two per size) on sampled slabs
+    import ctypes
+    from fc3._lib import check
+    errs = []
+    for i in range(nsnap - 1, -1, -1):
+        sysd, phi32, seed, su = meta[i]
+        n = sysd.n_dof
+        if sum(1 for e in errs if e["n"] == n) >= 2:
+            continue
+        coo = O.lj_third_order_coo(sysd, S.ARGON_LJ, calc)
+        v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
+        k3 = snaps[i][2]
+        num = den = 0.0
+        buf = np.empty((n, n), dtype=np.float32, order="F")
+        for l in (0, n // 2, n - 1):
+            ref = O.mcc3_fp64_slab((coo[0], coo[1], coo[2], v32), phi32.astype(np.float64), l)
+            check(ctx.lib.fc3_memcpy_d2h(ctx.h, buf.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_void_p(k3.ptr.value + 4 * n * n * l), 4 * n * n))
+            d = buf.astype(np.float64) - ref
+            num += float((d * d).sum())
+            den += float((ref * ref).sum())
+        errs.append({"n": n, "seed": seed, "sigma_u": su, "rel_fro_err": (num / den) ** 0.5})
+    for bufs in out_bufs.values():
+        for b in bufs:
+            b.free()
+    res = {"snapshots": nsnap, "sizes_dof": sorted(out_bufs), "ms_per_batch": t,
+           "snapshots_per_s": nsnap / (t * 1e-3), "K3_TFLOP_per_s": flop / (t * 1e-3) / 1e12,
+           "graph_nodes": nodes, "replays_timed": replays, "autotuned": tune,
+           "errors_sampled": errs,
+           "note": "assembly (FP64 blocks, mass-weighted FP32 Psi) + BF16x3 mcc3 per snapshot, "
+                   "one graph launch per batch"}
+    log(f"batched: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +192,8 @@ def main():
     ap.add_argument("--cells", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp32-steps", type=int, default=1)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-snapshot section (96 snapshots of 324/768/1500 DoF)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -238,6 +316,10 @@ def main():
                                    if "fp32" in results else None),
         "assembly": asm,
     }
+    d32.free()
+    k3.free()
+    if not a.no_batched:
+        out["batched"] = batched_snapshots(ctx)
     if not a.no_cpu_baseline:
         log("cpu baselines")
         out["cpu_baseline"] = cpu_baseline_mcc3()

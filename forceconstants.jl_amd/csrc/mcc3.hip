@@ -682,12 +682,13 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
         }
 }
 
-// Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K, K % 16 == 0) -> the
-// K-blocked A operand: X[(k / 16) 16 rows + 16 (r - r0) + k % 16] (hi), lo at
-// + rows K.  64 rows x 64 k per block through LDS; each thread then writes 4
-// consecutive k of one row, so a K block's 64 rows go out as 2 KB contiguous.
+// Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K) -> the K-blocked A
+// operand with K padded to Kp (a multiple of 16, zeros for k >= K):
+// X[(k / 16) 16 rows + 16 (r - r0) + k % 16] (hi), lo at + rows Kp.  64 rows x 64 k per
+// block through LDS; each thread then writes 4 consecutive k of one row, so a K block's
+// 64 rows go out as 2 KB contiguous.
 __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict__ psi, long n2,
-                                                          int K, long r0, long rows,
+                                                          int K, int Kp, long r0, long rows,
                                                           __bf16* __restrict__ X)
 {
     __shared__ float tile[64][65];
@@ -703,11 +704,11 @@ __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict
     const int rr = t >> 2, quad = t & 3;
     const long r = rb + rr;
     if (r >= rows) return;
-    const long lo_off = rows * K;
+    const long lo_off = rows * Kp;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const int k = kb + 16 * b + 4 * quad;
-        if (k >= K) break;
+        if (k >= Kp) break;
         bf16x4 h, l;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -724,8 +725,9 @@ __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict
 
 // cnt slabs T_l (n x n, T_l[i + n j] at src[l n^2 + i + n j]) -> the K-blocked A operand
 // of the mode-1 product A(l n + j, i) = T_l[i + n j]: hi at
-// Y[l 2n^2 + (i / 16) 16 n + 16 j + i % 16], lo at + n^2.  n % 16 == 0.
-__global__ void __launch_bounds__(256) k_split_slabs(const float* __restrict__ src, int n,
+// Y[l 2P + (i / 16) 16 n + 16 j + i % 16], lo at + P, with P = n Kp (Kp = n rounded up
+// to 16; the padding k >= n is not written).  n % 4 == 0.
+__global__ void __launch_bounds__(256) k_split_slabs(const float* __restrict__ src, int n, int Kp,
                                                       long cnt, __bf16* __restrict__ Y)
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // group of 4 values
@@ -741,9 +743,25 @@ __global__ void __launch_bounds__(256) k_split_slabs(const float* __restrict__ s
     split_bf16(x.y, a, b); h[1] = a; lo[1] = b;
     split_bf16(x.z, a, b); h[2] = a; lo[2] = b;
     split_bf16(x.w, a, b); h[3] = a; lo[3] = b;
-    const long off = l * 2 * n2 + (long)(i >> 4) * 16 * n + 16L * j + (i & 15);
+    const long P = (long)n * Kp;
+    const long off = l * 2 * P + (long)(i >> 4) * 16 * n + 16L * j + (i & 15);
     *reinterpret_cast<bf16x4*>(Y + off) = h;
-    *reinterpret_cast<bf16x4*>(Y + off + n2) = lo;
+    *reinterpret_cast<bf16x4*>(Y + off + P) = lo;
+}
+
+// Zeroes the last K block (block_elems bf16 at block_off) of both planes of `slabs` slabs
+// of a K-blocked operand: the rows' K padding, which no producer writes.
+__global__ void __launch_bounds__(256) k_zero_kblock(__bf16* __restrict__ base, long slabs,
+                                                      long slab_stride, long plane_off,
+                                                      long block_off, long block_elems)
+{
+    const long per = 2 * block_elems;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < slabs * per;
+         t += (long)gridDim.x * blockDim.x) {
+        const long sl = t / per, r = t - sl * per;
+        const long p = r / block_elems, e = r - p * block_elems;
+        base[sl * slab_stride + p * plane_off + block_off + e] = __bf16(0.0f);
+    }
 }
 
 }  // namespace v3
@@ -870,50 +888,70 @@ int modes12(hipStream_t s, int prec, float* T, int n, int l0, int cnt, const Phi
 
 
 // ---------------------------------------------------------------------------------
-// BF16X3 with pre-split operands (v3 kernels), used when n % 16 == 0.  Every operand
-// is in the K-blocked layout of v3::G3:
-//   phi:    B(k, c) = phi[k + n c], columns zero-padded to a multiple of BN.
+// BF16X3 with pre-split operands (v3 kernels), used when n % 4 == 0.  The contraction
+// length is padded to Kp = n rounded up to 16 with zeros.  Every operand is in the
+// K-blocked layout of v3::G3; a slab of a mode-1/2 operand holds P = n Kp bf16 per plane:
+//   phi:    B(k, c) = phi[k + n c], Kp rows, columns zero-padded to a multiple of BN.
 //   mode 3: Psi row blocks (rows (i, j) = i + n j) are split-transposed into X, and the
 //           GEMM writes U_l(i, j) = U[(i, j), l] split, already as the mode-1 A operand
-//           A(l n + j, i): slab l hi at bf16 offset l 2n^2, lo at +n^2.  That is
-//           exactly the byte range of K3 slab l.
+//           A(l n + j, i): slab l hi at bf16 offset l 2P, lo at +P.  For n % 16 == 0
+//           (P = n^2) that is exactly the byte range of K3 slab l and U lives there;
+//           otherwise U has its own scratch.
 //   mode 1: S_l = U_l^T phi, written split as the mode-2 A operand (same layout).
-//   mode 2: K3_l = S_l^T phi, FP32, over the bytes of U_l (already consumed).
-bool v3_applies(int n, int prec) { return prec == FC3_PREC_BF16X3 && n % v3::BK == 0 && n >= 16; }
+//   mode 2: K3_l = S_l^T phi, FP32 (over the bytes of U_l when U lives in K3).
+bool v3_applies(int n, int prec) { return prec == FC3_PREC_BF16X3 && n % 4 == 0 && n >= 16; }
 bool aligned16(const void* p) { return (uintptr_t)p % 16 == 0; }
+inline int padk(int n) { return round_up(n, v3::BK); }
 
 struct Split3 {
     const __bf16* b;
     long lo, ks;
 };
 
-// phi -> K-blocked B operand: hi at P[(k / 16) 16 Npad + 16 c + k % 16], lo at + n Npad.
-__global__ void k_prep_phi3(const float* __restrict__ phi, int n, int Npad, __bf16* __restrict__ P)
+// phi -> K-blocked B operand: hi at P[(k / 16) 16 Npad + 16 c + k % 16], lo at + Kp Npad;
+// zero for k >= n or c >= n.
+__global__ void k_prep_phi3(const float* __restrict__ phi, int n, int Kp, int Npad,
+                            __bf16* __restrict__ P)
 {
     const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long e = (long)n * Npad;
+    const long e = (long)Kp * Npad;
     if (o >= e) return;
     const long kb = o / (16L * Npad), rem = o - kb * 16L * Npad;
     const int c = (int)(rem >> 4), k = (int)(16 * kb + (rem & 15));
-    const float x = c < n ? phi[k + (long)c * n] : 0.0f;
+    const float x = (c < n && k < n) ? phi[k + (long)c * n] : 0.0f;
     __bf16 h, l;
     split_bf16(x, h, l);
     P[o] = h;
     P[e + o] = l;
 }
 
-size_t phi3_bytes(int n) { return Carve::need((size_t)2 * 2 * round_up(n, v3::BN) * n); }
+size_t phi3_bytes(int n) { return Carve::need((size_t)2 * 2 * round_up(n, v3::BN) * padk(n)); }
 
 int prep_phi3(hipStream_t s, const float* phi, int n, Carve& cv, Split3& out)
 {
-    const int Npad = round_up(n, v3::BN);
-    const long e = (long)n * Npad;
+    const int Npad = round_up(n, v3::BN), Kp = padk(n);
+    const long e = (long)Kp * Npad;
     __bf16* p = cv.take<__bf16>(2 * e);
     out.b = p;
     out.lo = e;
     out.ks = 16L * Npad;
-    hipLaunchKernelGGL(k_prep_phi3, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, phi, n,
+    hipLaunchKernelGGL(k_prep_phi3, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, phi, n, Kp,
                        Npad, p);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+// Zero the K padding of `slabs` slabs of a K-blocked mode-1/2 operand (no-op if n % 16 == 0).
+int zero_kpad(hipStream_t s, __bf16* base, int n, long slabs)
+{
+    const int Kp = padk(n);
+    if (Kp == n || slabs <= 0) return FC3_OK;
+    const long P = (long)n * Kp;
+    const long blk = 16L * n;
+    const long tot = slabs * 2 * blk;
+    const unsigned grid = (unsigned)std::min<long>((tot + 255) / 256, 65536);
+    hipLaunchKernelGGL(v3::k_zero_kblock, dim3(grid), dim3(256), 0, s, base, slabs, 2 * P, P,
+                       (long)(Kp / 16 - 1) * blk, blk);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
@@ -963,67 +1001,71 @@ constexpr long kTilesPerLaunch = 16384;   // default; the batch autotuner may ch
 constexpr size_t kScratchSlabs = size_t(4) << 30, kScratchRows = size_t(8) << 30;
 constexpr size_t kScratchSlabsInplace = size_t(1) << 30, kScratchRowsInplace = size_t(2) << 30;
 
-// Slabs per mode-1/2 launch; each slab takes 4 n^2 bytes of split scratch.
+// Slabs per mode-1/2 launch; each slab takes 4 n Kp bytes of split scratch.
 int slab_batch3(int n, size_t cap_bytes, long tiles_per_launch)
 {
     const long tiles_n = (n + v3::BN - 1) / v3::BN;
     long b = (tiles_per_launch * v3::BM + tiles_n * n - 1) / (tiles_n * n);
-    const long cap = (long)(cap_bytes / (4 * (size_t)n * n));
+    const long cap = (long)(cap_bytes / (4 * (size_t)n * padk(n)));
     b = std::max(1L, std::min({b, cap, (long)n}));
     const long batches = (n + b - 1) / b;
     return (int)((n + batches - 1) / batches);
 }
 
 // Mode-3 row block: whole columns j of Psi(i, j, :) (a multiple of n rows); the split
-// copy X takes 4 n bytes per row.
+// copy X takes 4 Kp bytes per row.
 long mode3_rows(int n, size_t cap_bytes, long tiles_per_launch)
 {
     const long tiles_n = (n + v3::BN - 1) / v3::BN;
     const long target = (tiles_per_launch + tiles_n - 1) / tiles_n * v3::BM;
     long u = (target + n - 1) / n;                                   // columns j per block
-    const long cap = (long)(cap_bytes / (4 * (size_t)n)) / n;
+    const long cap = (long)(cap_bytes / (4 * (size_t)padk(n))) / n;
     u = std::max(1L, std::min({u, cap, (long)n}));
     const long blocks = (n + u - 1) / u;
     return (n + blocks - 1) / blocks * (long)n;
 }
 
-// Psi rows [r0, r0 + rows) -> X (split, K-blocked) -> C(m, l) for those rows.
+// Psi rows [r0, r0 + rows) -> X (split, K-blocked, K padded) -> C(m, l) for those rows.
 int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, const Split3& B,
                 __bf16* X, const OutMap& o, bool split_out)
 {
     const long n2 = (long)n * n;
-    dim3 gt((unsigned)((rows + 63) / 64), (unsigned)((n + 63) / 64));
-    hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, r0, rows, X);
+    const int Kp = padk(n);
+    dim3 gt((unsigned)((rows + 63) / 64), (unsigned)((Kp + 63) / 64));
+    hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows, X);
     FC3_HIP(hipGetLastError());
-    return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * n, rows, 0, 16 * rows, B, o, split_out,
-                 rows, n, n);
+    return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, o, split_out,
+                 rows, n, Kp);
 }
 
 // Modes 1 and 2 on slabs [l0, l0 + cnt): U holds them as the mode-1 A operand (slab
-// stride 2n^2 bf16, lo at +n^2, K-blocked); S is scratch of the same layout; T
+// stride 2P bf16 with P = n Kp, lo at +P, K-blocked); S is scratch of the same layout; T
 // receives the FP32 slabs K3_l[b + n c].
 int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, __bf16* S,
                float* T)
 {
     const long n2 = (long)n * n;
+    const int Kp = padk(n);
+    const long P = (long)n * Kp;
     // A(l n + j, i) = U_l(i, j);  C(l n + j, b) = S_l(j, b) -> mode-2 A(l n + b, j)
-    const OutMap oS{S, n, 2 * n2, 16L * n, 16, n2};
-    FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, n2, n, 2 * n2, 16L * n, B, oS, true,
-                  (long)cnt * n, n, n));
+    const OutMap oS{S, n, 2 * P, 16L * n, 16, P};
+    FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, n, 2 * P, 16L * n, B, oS, true, (long)cnt * n,
+                  n, Kp));
     // A(l n + b, j) = S_l(j, b);  C(l n + b, c) = K3_l[b + n c]
     const OutMap oT{T, n, n2, 16, n, 0};
-    return gemm3(s, FC3_KERNEL_GEMM_MODE12, S, n2, n, 2 * n2, 16L * n, B, oT, false,
-                 (long)cnt * n, n, n);
+    return gemm3(s, FC3_KERNEL_GEMM_MODE12, S, P, n, 2 * P, 16L * n, B, oT, false, (long)cnt * n,
+                 n, Kp);
 }
 
 size_t mcc3_v3_workspace_bytes(int n, bool inplace, long tiles)
 {
-    const long n2 = (long)n * n;
+    const long P = (long)n * padk(n);
     const int nb = slab_batch3(n, inplace ? kScratchSlabsInplace : kScratchSlabs, tiles);
     const long R = mode3_rows(n, inplace ? kScratchRowsInplace : kScratchRows, tiles);
-    const size_t slabs = Carve::need(sizeof(float) * n2 * nb);
-    return phi3_bytes(n) + slabs * (inplace ? 2 : 1) + Carve::need(sizeof(float) * (size_t)R * n) +
-           4096;
+    const size_t slabs = Carve::need(2 * sizeof(__bf16) * P * nb);
+    const size_t u = (!inplace && padk(n) != n) ? Carve::need(2 * sizeof(__bf16) * P * n) : 0;
+    return phi3_bytes(n) + slabs * (inplace ? 2 : 1) + u +
+           Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n)) + 4096;
 }
 
 int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3, void* ws,
@@ -1033,21 +1075,27 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
     Split3 B;
     FC3_TRY(prep_phi3(s, phi, n, cv, B));
     const long n2 = (long)n * n;
+    const int Kp = padk(n);
+    const long P = (long)n * Kp;
     const int nb = slab_batch3(n, kScratchSlabs, tiles);
     const long R = mode3_rows(n, kScratchRows, tiles);
-    __bf16* S = cv.take<__bf16>((size_t)2 * n2 * nb);
-    __bf16* X = cv.take<__bf16>((size_t)2 * R * n);
-    __bf16* U = reinterpret_cast<__bf16*>(k3);
+    __bf16* S = cv.take<__bf16>((size_t)2 * P * nb);
+    __bf16* X = cv.take<__bf16>((size_t)2 * R * Kp);
+    // U lives in the K3 buffer when its slabs have K3's size, else in scratch
+    __bf16* U = Kp == n ? reinterpret_cast<__bf16*>(k3) : cv.take<__bf16>((size_t)2 * P * n);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 workspace too small");
+    FC3_TRY(zero_kpad(s, U, n, n));
+    FC3_TRY(zero_kpad(s, S, n, nb));
     for (long r0 = 0; r0 < n2; r0 += R) {
         const long rows = (n2 - r0) < R ? (n2 - r0) : R;
-        // C((i, j) - r0, l) -> U_l(i, j) = U[l 2n^2 + (i / 16) 16 n + 16 j + i % 16]; a row
+        // C((i, j) - r0, l) -> U_l(i, j) = U[l 2P + (i / 16) 16 n + 16 j + i % 16]; a row
         // block starts at a multiple of n (R is), so j = (m + r0) / n with m = row
-        const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * n2, n2};
+        const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * P, P};
         FC3_TRY(mode3_block(s, psi, n, r0, rows, B, X, oU, true));
     }
     for (int l0 = 0; l0 < n; l0 += nb) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        FC3_TRY(modes12_v3(s, U + 2 * n2 * l0, n, cnt, B, S, k3 + n2 * l0));
+        FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0));
     }
     return FC3_OK;
 }
@@ -1058,17 +1106,22 @@ int mcc3_v3_inplace(hipStream_t s, float* psi, const float* phi, int n, void* ws
     Split3 B;
     FC3_TRY(prep_phi3(s, phi, n, cv, B));
     const long n2 = (long)n * n;
+    const int Kp = padk(n);
+    const long P = (long)n * Kp;
     const int nb = slab_batch3(n, kScratchSlabsInplace, kTilesPerLaunch);
     const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
-    __bf16* Y = cv.take<__bf16>((size_t)2 * n2 * nb);
-    __bf16* S = cv.take<__bf16>((size_t)2 * n2 * nb);
-    __bf16* X = cv.take<__bf16>((size_t)2 * R * n);
+    __bf16* Y = cv.take<__bf16>((size_t)2 * P * nb);
+    __bf16* S = cv.take<__bf16>((size_t)2 * P * nb);
+    __bf16* X = cv.take<__bf16>((size_t)2 * R * Kp);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 workspace too small");
+    FC3_TRY(zero_kpad(s, Y, n, nb));
+    FC3_TRY(zero_kpad(s, S, n, nb));
     // modes 1 and 2 on the slabs of Psi (they commute with mode 3)
     for (int l0 = 0; l0 < n; l0 += nb) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
         const long q = n2 / 4 * cnt;
         hipLaunchKernelGGL(v3::k_split_slabs, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
-                           psi + n2 * l0, n, (long)cnt, Y);
+                           psi + n2 * l0, n, Kp, (long)cnt, Y);
         FC3_HIP(hipGetLastError());
         FC3_TRY(modes12_v3(s, Y, n, cnt, B, S, psi + n2 * l0));
     }

@@ -3,7 +3,7 @@
 Bars (relative Frobenius error vs FP64): FP32 <= 2e-6 (plain FP32 accumulation
 over n-term sums), BF16X3 <= 1e-5.  Exact invariants: phi = I gives K3 = Psi and a
 permutation phi permutes Psi (FP32).  Sizes cover both BF16X3 kernels (n % 16 == 0
-takes the pre-split path), single and partial tiles, and, at n = 1536, several
+takes the pre-split path, with K padded to a multiple of 16 when n % 16 != 0), single and partial tiles, and, at n = 1536, several
 mode-3 row blocks against a closed form (Psi a sum of rank-1 terms).
 """
 import ctypes
@@ -34,7 +34,7 @@ def random_case(n, seed):
     return np.asfortranarray(psi.astype(np.float32)), np.asfortranarray(phi.astype(np.float32))
 
 
-@pytest.mark.parametrize("n", [16, 48, 96, 200, 99, 256, 272])
+@pytest.mark.parametrize("n", [16, 20, 48, 96, 200, 99, 256, 272, 324])
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_mcc3_random(ctx, n, prec):
     psi, phi = random_case(n, 100 + n)
@@ -48,13 +48,13 @@ def test_mcc3_random(ctx, n, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-@pytest.mark.parametrize("n", [96, 112])
+@pytest.mark.parametrize("n", [96, 100, 112])
 def test_mcc3_inplace_and_blocked(ctx, prec, n):
     psi, phi = random_case(n, 7)
     ref = O.mcc3_fp64(psi, phi)
     dphi = fc3.DeviceArray.from_host(phi, ctx=ctx)
     dpsi = fc3.DeviceArray.from_host(psi, ctx=ctx)
-    blocked = fc3.mcc3(dpsi, dphi, 16, precision=prec)      # host result, block_size = 16
+    blocked = fc3.mcc3(dpsi, dphi, 4, precision=prec)       # host result, block_size = 4
     assert isinstance(blocked, np.ndarray)
     assert rel_fro(blocked, ref) <= TOL[prec]
     fc3.mcc3_(dpsi, dphi, precision=prec)                   # mcc3!: overwrites Psi

@@ -1,3 +1,4 @@
+#include <algorithm>
 // C-ABI entry points (include/fc3.h): argument validation with the reference's
 // assertion messages, workspace management and dispatch to the HIP kernels.
 #include <cmath>
@@ -518,6 +519,49 @@ int fc3_third_order_lj_param_backward(fc3_handle h, const double* positions, int
     FC3_TRY(lj_param_grad(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, S3, tol,
                           masses ? P.masses : nullptr, dpsi32, dpsi64, partial, out));
     FC3_HIP(hipMemcpyAsync(grad, out, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    FC3_HIP(hipStreamSynchronize(h->stream));
+    return FC3_OK;
+}
+
+int fc3_third_order_sw_param_backward(fc3_handle h, const double* positions, int n_atoms,
+                                      const double box[3], const double* sw_params, double r_cut,
+                                      double tol, const double* masses, const float* dpsi32,
+                                      const double* dpsi64, double grad[4])
+{
+    FC3_TRY(check_handle(h));
+    FC3_TRY(check_system(positions, n_atoms, box));
+    if (!sw_params || !grad) return set_error(FC3_ERR_ARGUMENT, "sw_params, grad must be non-null");
+    if (!(r_cut <= sw_params[2] * sw_params[1]))
+        return set_error(FC3_ERR_ARGUMENT,
+                         "For SW silicon force constant cutoff must be less than potential cutoff");
+    if ((dpsi32 == nullptr) == (dpsi64 == nullptr))
+        return set_error(FC3_ERR_ARGUMENT, "exactly one of dpsi32, dpsi64 must be non-null");
+    if (dpsi64 && masses)
+        return set_error(FC3_ERR_ARGUMENT, "dpsi64 is the cotangent of the unweighted Psi: masses must be null");
+    if (!(sw_params[6] != 0.0) || !(sw_params[3] != 0.0) || !(sw_params[4] != 0.0))
+        return set_error(FC3_ERR_ARGUMENT, "A, lambda and gamma must be nonzero");
+    if (masses)
+        for (int i = 0; i < n_atoms; ++i)
+            if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    const int N = n_atoms;
+    const long n = 3L * N;
+    const size_t fwd = tol > 0.0 ? sizeof(double) * (size_t)n * n * n : 0;
+    Prepared P;
+    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
+        return std::max(sw_workspace_bytes(N, cap, true), sw_param_grad_bytes(N, cap)) +
+               Carve::need(fwd) + Carve::need(4 * sizeof(double)) + 4096;
+    }, P));
+    Carve cv(P.extra, P.extra_bytes);
+    double* out = cv.take<double>(4);
+    double* psi_fwd = fwd ? cv.take<double>((size_t)n * n * n) : nullptr;
+    char* rest = cv.take<char>(0);
+    const size_t rest_bytes = P.extra_bytes - (size_t)(rest - P.extra);
+    if (psi_fwd)   // the forward tensor, for the tolerance mask
+        FC3_TRY(sw_assemble(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl,
+                            rest, rest_bytes, tol, nullptr, psi_fwd, nullptr, nullptr));
+    FC3_TRY(sw_param_grad(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl, tol,
+                          psi_fwd, masses ? P.masses : nullptr, dpsi32, dpsi64, rest, out));
+    FC3_HIP(hipMemcpyAsync(grad, out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     FC3_HIP(hipStreamSynchronize(h->stream));
     return FC3_OK;
 }

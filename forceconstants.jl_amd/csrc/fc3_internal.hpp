@@ -132,6 +132,14 @@ size_t sw_workspace_bytes(int N, int cap, bool third);
 int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const double* params,
                 double c0_fd, double c0_iik, const NeighborList& nl, void* ws, size_t ws_bytes,
                 double tol, const double* d_masses, double* psi64, float* psi32, double* ifc2);
+// dL/d(A, B, lambda, gamma) into d_out[0..3] (device) for the cotangent dpsi32 (of the
+// mass-weighted FP32 Psi, d_masses as in the forward call) or dpsi64 (of the FP64 Psi);
+// psi_fwd: the forward FP64 Psi, read for the tolerance mask when tol > 0.
+size_t sw_param_grad_bytes(int N, int cap);
+int sw_param_grad(hipStream_t s, const double* d_pos, const double box[3], const double* params,
+                  double c0_fd, double c0_iik, const NeighborList& nl, double tol,
+                  const double* psi_fwd, const double* d_masses, const float* dpsi32,
+                  const double* dpsi64, void* ws, double* d_out);
 
 // ---- mcc3 (mcc3.hip) --------------------------------------------------------------
 // Kernel family and launch sizing of one mcc3: variant 2 = FP32-operand kernels (any n),

@@ -487,6 +487,165 @@ int reduce_blocks(hipStream_t s, long R, int N, u64* key, double* val, Carve& cv
     return FC3_OK;
 }
 
+// ---- parameter adjoint ----------------------------------------------------------
+// Cotangent of one placed record value at (3A+x1, 3B+x2, 3C+x3): the reference's ASR!
+// overwrites every (A, B, A) block, so a record with C == A contributes nothing, and a
+// record with C != A reaches its own position and, with a minus sign, the ASR entry
+// (3A+x1, 3B+x2, 3A+x3).  apply_tols! zeroes entries (and their derivatives) with
+// |value| < tol, read from the forward tensor psi_fwd when tol > 0.
+template <typename DT>
+struct GradCtx {
+    const DT* dpsi;
+    const double* psi_fwd;   // forward FP64 Psi (mask), or null when tol == 0
+    double tol;
+    const double* masses;    // mass factors for a cotangent of the mass-weighted Psi, or null
+    long n;
+};
+
+template <typename DT>
+__device__ double pos_cot(const GradCtx<DT>& g, int A, int B, int C, int x1, int x2, int x3)
+{
+    const long p = (3L * A + x1) + g.n * (3L * B + x2) + g.n * g.n * (3L * C + x3);
+    if (g.psi_fwd && !(fabs(g.psi_fwd[p]) >= g.tol)) return 0.0;
+    double w = (double)g.dpsi[p];
+    if (g.masses) w /= sqrt((g.masses[A] * g.masses[B]) * g.masses[C]);
+    return w;
+}
+
+// Adds to W[src index] the cotangent of the placement (A, B, C) with permutation perm
+// (permutedims semantics of permute27) and sign sgn.
+template <typename DT>
+__device__ void place_cot(const GradCtx<DT>& g, int A, int B, int C, int p1, int p2, int p3,
+                          double sgn, double* W)
+{
+    if (C == A) return;
+    const int pm[3] = {p1 - 1, p2 - 1, p3 - 1};
+    for (int x3 = 0; x3 < 3; ++x3)
+        for (int x2 = 0; x2 < 3; ++x2)
+            for (int x1 = 0; x1 < 3; ++x1) {
+                int jj[3];
+                jj[pm[0]] = x1;
+                jj[pm[1]] = x2;
+                jj[pm[2]] = x3;
+                const double w = pos_cot(g, A, B, C, x1, x2, x3) - pos_cot(g, A, B, A, x1, x2, x3);
+                W[jj[0] + 3 * jj[1] + 9 * jj[2]] += sgn * w;
+            }
+}
+
+// One thread per (centre i, slot s), the record loop of k_sw_emit: accumulates
+// dL/d(A, B, lambda, gamma) into per-block partials (4 doubles per block).
+template <typename DT>
+__global__ void __launch_bounds__(64) k_sw_param_grad(EmitCtx c, GradCtx<DT> g,
+                                                      double* __restrict__ partial)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double gA = 0.0, gB = 0.0, gl = 0.0, gg = 0.0;
+    if (t < c.N * c.cap) {
+        const int i = t / c.cap, s = t - i * c.cap;
+        const int d = c.count[i];
+        if (s < d) {
+            const int* nb = c.nbr + (size_t)i * c.cap;
+            const int j = nb[s];
+            const double ri[3] = {c.pos[3 * i], c.pos[3 * i + 1], c.pos[3 * i + 2]};
+            double rij[3] = {ri[0] - c.pos[3 * j], ri[1] - c.pos[3 * j + 1],
+                             ri[2] - c.pos[3 * j + 2]};
+            geom::mirror(rij, c.L);
+            double W[27];
+            if (j > i) {
+                for (int e = 0; e < 27; ++e) W[e] = 0.0;
+                const int ks[6][3] = {{i, i, j}, {i, j, i}, {j, i, i}, {j, j, i}, {j, i, j}, {i, j, j}};
+                for (int q = 0; q < 6; ++q)
+                    place_cot(g, ks[q][0], ks[q][1], ks[q][2], 1, 2, 3, q < 3 ? -1.0 : 1.0, W);
+                double blk[27], b1[27], b0[27];
+                sw::pair_blocks(rij, c.P, nullptr, blk);
+                sw::Params P1 = c.P, P0 = c.P;
+                P1.B = 1.0;
+                P0.B = 0.0;
+                sw::pair_blocks(rij, P1, nullptr, b1);   // phi2 is affine in B
+                sw::pair_blocks(rij, P0, nullptr, b0);
+                for (int e = 0; e < 27; ++e) {
+                    gA += (blk[e] / c.P.A) * W[e];
+                    gB += (b1[e] - b0[e]) * W[e];
+                }
+            }
+            for (int tt = s + 1; tt < d; ++tt) {
+                const int k = nb[tt];
+                double rik[3] = {ri[0] - c.pos[3 * k], ri[1] - c.pos[3 * k + 1],
+                                 ri[2] - c.pos[3 * k + 2]};
+                geom::mirror(rik, c.L);
+                double u[3], v[3];
+                for (int q = 0; q < 3; ++q) {
+                    const double nj = ri[q] - rij[q], nk = ri[q] - rik[q];
+                    u[q] = ri[q] - nj;
+                    v[q] = ri[q] - nk;
+                }
+                const double p = (u[0] * u[0] + u[1] * u[1]) + u[2] * u[2];
+                const double q2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+                const double sdot = (u[0] * v[0] + u[1] * v[1]) + u[2] * v[2];
+                const double le = c.P.lambda * c.P.eps, gs = c.P.gamma * c.P.sigma,
+                             rc = c.P.a * c.P.sigma;
+                double Hfd[4][4][4], dHfd[4][4][4], Hik[4][4][4], dHik[4][4][4];
+                sw::h_partials(p, q2, sdot, c.c0_fd, le, gs, rc, Hfd, dHfd, c.P.gamma);
+                sw::h_partials(p, q2, sdot, c.c0_iik, le, gs, rc, Hik, dHik, c.P.gamma);
+                sw::TForms tf;
+                sw::t_forms(u, v, tf);
+                struct Pl { int A, B, C, p1, p2, p3; };
+                // the seven blocks of k_sw_emit with their placements
+                auto block = [&](const double (*H)[4][4], const double (*dH)[4][4], int G1, int G2,
+                                 int G3, const Pl* pl, int np) {
+                    for (int e = 0; e < 27; ++e) W[e] = 0.0;
+                    for (int q = 0; q < np; ++q)
+                        place_cot(g, pl[q].A, pl[q].B, pl[q].C, pl[q].p1, pl[q].p2, pl[q].p3, 1.0, W);
+                    double b[27], db[27];
+                    sw::block3(H, tf, G1, G2, G3, b);
+                    sw::block3(dH, tf, G1, G2, G3, db);
+                    for (int e = 0; e < 27; ++e) {
+                        gl += (b[e] / c.P.lambda) * W[e];
+                        gg += db[e] * W[e];
+                    }
+                };
+                const Pl iij[3] = {{i, i, j, 1, 2, 3}, {i, j, i, 1, 3, 2}, {j, i, i, 3, 2, 1}};
+                const Pl ijj[3] = {{i, j, j, 1, 2, 3}, {j, i, j, 2, 1, 3}, {j, j, i, 3, 2, 1}};
+                const Pl iik[3] = {{i, i, k, 1, 2, 3}, {i, k, i, 1, 3, 2}, {k, i, i, 3, 2, 1}};
+                const Pl ikk[3] = {{i, k, k, 1, 2, 3}, {k, i, k, 2, 1, 3}, {k, k, i, 3, 2, 1}};
+                const Pl jjk[3] = {{j, j, k, 1, 2, 3}, {j, k, j, 1, 3, 2}, {k, j, j, 3, 2, 1}};
+                const Pl jkk[3] = {{j, k, k, 1, 2, 3}, {k, j, k, 2, 1, 3}, {k, k, j, 3, 2, 1}};
+                const Pl ijk[6] = {{i, j, k, 1, 2, 3}, {i, k, j, 1, 3, 2}, {j, i, k, 2, 1, 3},
+                                   {j, k, i, 2, 3, 1}, {k, i, j, 3, 1, 2}, {k, j, i, 3, 2, 1}};
+                block(Hfd, dHfd, 0, 0, 1, iij, 3);
+                block(Hfd, dHfd, 0, 1, 1, ijj, 3);
+                block(Hik, dHik, 0, 0, 2, iik, 3);
+                block(Hfd, dHfd, 0, 2, 2, ikk, 3);
+                block(Hfd, dHfd, 1, 1, 2, jjk, 3);
+                block(Hfd, dHfd, 1, 2, 2, jkk, 3);
+                block(Hfd, dHfd, 0, 1, 2, ijk, 6);
+            }
+        }
+    }
+    __shared__ double red[4][64];
+    red[0][threadIdx.x] = gA;
+    red[1][threadIdx.x] = gB;
+    red[2][threadIdx.x] = gl;
+    red[3][threadIdx.x] = gg;
+    __syncthreads();
+    for (int h = 32; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) partial[4 * blockIdx.x + q] = red[q][0];
+}
+
+__global__ void k_sum_partials4(const double* __restrict__ partial, int cnt, double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < cnt; ++i)
+        for (int q = 0; q < 4; ++q) a[q] += partial[4 * i + q];
+    for (int q = 0; q < 4; ++q) out[q] = a[q];
+}
+
 }  // namespace
 
 size_t sw_workspace_bytes(int N, int cap, bool third)
@@ -601,6 +760,52 @@ int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const d
                            N, tol, ifc2);
         FC3_HIP(hipGetLastError());
     }
+    return FC3_OK;
+}
+
+}  // namespace fc3
+
+namespace fc3 {
+
+size_t sw_param_grad_bytes(int N, int cap)
+{
+    const long blocks_ = ((long)N * cap + 63) / 64;
+    return Carve::need(sizeof(double) * (4 * blocks_ + 4));
+}
+
+int sw_param_grad(hipStream_t s, const double* d_pos, const double box[3], const double* params,
+                  double c0_fd, double c0_iik, const NeighborList& nl, double tol,
+                  const double* psi_fwd, const double* d_masses, const float* dpsi32,
+                  const double* dpsi64, void* ws, double* d_out)
+{
+    const int N = nl.n_atoms;
+    const long nblk = ((long)N * nl.cap + 63) / 64;
+    Carve cv(ws, sw_param_grad_bytes(N, nl.cap));
+    double* partial = cv.take<double>(4 * nblk);
+    EmitCtx c;
+    c.pos = d_pos;
+    c.count = nl.count;
+    c.nbr = nl.nbr;
+    c.N = N;
+    c.cap = nl.cap;
+    c.L[0] = box[0]; c.L[1] = box[1]; c.L[2] = box[2];
+    c.P = sw::Params{params[0], params[1], params[2], params[3], params[4],
+                     params[5], params[6], params[7], params[8], params[9]};
+    c.c0_fd = c0_fd;
+    c.c0_iik = c0_iik;
+    c.off3 = nullptr; c.off2 = nullptr;
+    c.key3 = nullptr; c.val3 = nullptr; c.key2 = nullptr; c.val2 = nullptr;
+    const long n = 3L * N;
+    if (dpsi32) {
+        GradCtx<float> g{dpsi32, tol > 0.0 ? psi_fwd : nullptr, tol, d_masses, n};
+        hipLaunchKernelGGL(k_sw_param_grad<float>, dim3((unsigned)nblk), dim3(64), 0, s, c, g, partial);
+    } else {
+        GradCtx<double> g{dpsi64, tol > 0.0 ? psi_fwd : nullptr, tol, nullptr, n};
+        hipLaunchKernelGGL(k_sw_param_grad<double>, dim3((unsigned)nblk), dim3(64), 0, s, c, g, partial);
+    }
+    FC3_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials4, dim3(1), dim3(64), 0, s, partial, (int)nblk, d_out);
+    FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
 

@@ -30,8 +30,11 @@ struct Params {
     double eps, sigma, a, lambda, gamma, c0, A, B, p, q;
 };
 
-// Derivatives of x -> exp(gs / (sqrt(x) - rc)) with respect to x, orders 0..3.
-__device__ __forceinline__ void exp_radial_sq(double x, double gs, double rc, double e[4])
+// Derivatives of x -> exp(gs / (sqrt(x) - rc)) with respect to x, orders 0..3; with de,
+// also their derivatives with respect to gamma (gs = gamma sigma: the exponent and all
+// its x-derivatives are linear in gs, so d/dgamma multiplies each by 1/gamma).
+__device__ __forceinline__ void exp_radial_sq(double x, double gs, double rc, double e[4],
+                                              double* de = nullptr, double gamma = 1.0)
 {
     const double rho = sqrt(x);
     const double d = rho - rc;
@@ -45,12 +48,21 @@ __device__ __forceinline__ void exp_radial_sq(double x, double gs, double rc, do
     e[1] = e0 * f1;
     e[2] = e0 * (f2 + f1 * f1);
     e[3] = e0 * (f3 + 3.0 * f1 * f2 + f1 * f1 * f1);
+    if (de) {
+        const double g0 = gs / d, ig = 1.0 / gamma;
+        de[0] = e0 * g0 * ig;
+        de[1] = e0 * f1 * (g0 + 1.0) * ig;
+        de[2] = e0 * (g0 * (f2 + f1 * f1) + f2 + 2.0 * f1 * f1) * ig;
+        de[3] = e0 * (g0 * (f3 + 3.0 * f1 * f2 + f1 * f1 * f1) + f3 + 6.0 * f1 * f2 +
+                      3.0 * f1 * f1 * f1) * ig;
+    }
 }
 
 // Partials of H(p, q, s) = le (C - c0)^2 e(p) e(q): Hd[i][j][k] = d^(i+j+k) H / dp^i dq^j ds^k
-// for i + j + k <= 3.
+// for i + j + k <= 3; with dHg, also their derivatives with respect to gamma.
 __device__ inline void h_partials(double p, double q, double s, double c0, double le, double gs,
-                                  double rc, double Hd[4][4][4])
+                                  double rc, double Hd[4][4][4],
+                                  double (*dHg)[4][4] = nullptr, double gamma = 1.0)
 {
     const double a = sqrt(p), b = sqrt(q);
     // pi(p) = p^-1/2 and kappa(q) = q^-1/2 with derivatives 0..3
@@ -81,19 +93,27 @@ __device__ inline void h_partials(double p, double q, double s, double c0, doubl
         const double Cxyz = Cd(i, j, k);
         return 2.0 * (C2(x, y) * C1(z) + C2(x, z) * C1(y) + C2(y, z) * C1(x) + dc * Cxyz);
     };
-    double ep[4], eq[4];
-    exp_radial_sq(p, gs, rc, ep);
-    exp_radial_sq(q, gs, rc, eq);
+    double ep[4], eq[4], dep[4], deq[4];
+    exp_radial_sq(p, gs, rc, ep, dHg ? dep : nullptr, gamma);
+    exp_radial_sq(q, gs, rc, eq, dHg ? deq : nullptr, gamma);
     const double binom[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j)
             for (int k = 0; k < 4; ++k) {
-                if (i + j + k > 3) { Hd[i][j][k] = 0.0; continue; }
-                double acc = 0.0;
+                if (i + j + k > 3) {
+                    Hd[i][j][k] = 0.0;
+                    if (dHg) dHg[i][j][k] = 0.0;
+                    continue;
+                }
+                double acc = 0.0, dacc = 0.0;
                 for (int i1 = 0; i1 <= i; ++i1)
-                    for (int j1 = 0; j1 <= j; ++j1)
-                        acc += binom[i][i1] * binom[j][j1] * Fd(i - i1, j - j1, k) * ep[i1] * eq[j1];
+                    for (int j1 = 0; j1 <= j; ++j1) {
+                        const double w = binom[i][i1] * binom[j][j1] * Fd(i - i1, j - j1, k);
+                        acc += w * ep[i1] * eq[j1];
+                        if (dHg) dacc += w * (dep[i1] * eq[j1] + ep[i1] * deq[j1]);
+                    }
                 Hd[i][j][k] = le * acc;
+                if (dHg) dHg[i][j][k] = le * dacc;
             }
 }
 

@@ -81,6 +81,8 @@ SIGNATURES = {
                                     ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_float)]),
     "fc3_autotune_save": (_c_int, [_handle, ctypes.c_char_p]),
     "fc3_autotune_load": (_c_int, [_handle, ctypes.c_char_p]),
+    "fc3_third_order_sw_param_backward": (_c_int, [_handle, _dp, _c_int, _dp, _dp, _c_double,
+                                                   _c_double, _dp, _vp, _vp, _dp]),
     "fc3_mcc3_backward": (_c_int, [_handle, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp]),
     "fc3_third_order_lj_param_backward": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double,
                                                    _c_double, _c_double, _c_double, _dp, _vp, _vp,

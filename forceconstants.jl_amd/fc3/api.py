@@ -315,10 +315,8 @@ def third_order_param_backward(sys: SuperCellSystem, pot, calc, dpsi: DeviceArra
 
     dpsi FP32: cotangent of the FP32 tensor third_order_device(..., psi32=...) writes
     (mass weighted when `masses` is given); dpsi FP64: of the unweighted FP64 tensor.
-    Returns {"sigma": ..., "epsilon": ...} for LJ."""
+    Returns {"sigma", "epsilon"} for LJ and {"A", "B", "lambda", "gamma"} for SW."""
     ctx = ctx or dpsi.ctx
-    if not isinstance(pot, LJ):
-        raise NotImplementedError("parameter adjoint implemented for LJ")
     pos, L = _sys_args(sys)
     n = sys.n_dof
     if dpsi.shape != (n, n, n):
@@ -332,11 +330,20 @@ def third_order_param_backward(sys: SuperCellSystem, pot, calc, dpsi: DeviceArra
         d32, d64 = None, dpsi.ptr
     else:
         raise TypeError("dpsi must be float32 or float64")
-    g = np.zeros(2, dtype=np.float64)
-    check(ctx.lib.fc3_third_order_lj_param_backward(
-        ctx.h, _ptr(pos), sys.n_atoms, _ptr(L), pot.sigma, pot.epsilon, pot.r_cut, calc.r_cut,
-        calc.tol, _ptr(m) if m is not None else None, d32, d64, _ptr(g)))
-    return {"sigma": float(g[0]), "epsilon": float(g[1])}
+    if isinstance(pot, LJ):
+        g = np.zeros(2, dtype=np.float64)
+        check(ctx.lib.fc3_third_order_lj_param_backward(
+            ctx.h, _ptr(pos), sys.n_atoms, _ptr(L), pot.sigma, pot.epsilon, pot.r_cut, calc.r_cut,
+            calc.tol, _ptr(m) if m is not None else None, d32, d64, _ptr(g)))
+        return {"sigma": float(g[0]), "epsilon": float(g[1])}
+    if isinstance(pot, StillingerWeberSilicon):
+        prm = np.ascontiguousarray(pot.params.as_abi_array(), dtype=np.float64)
+        g = np.zeros(4, dtype=np.float64)
+        check(ctx.lib.fc3_third_order_sw_param_backward(
+            ctx.h, _ptr(pos), sys.n_atoms, _ptr(L), _ptr(prm), calc.r_cut, calc.tol,
+            _ptr(m) if m is not None else None, d32, d64, _ptr(g)))
+        return {"A": float(g[0]), "B": float(g[1]), "lambda": float(g[2]), "gamma": float(g[3])}
+    raise NotImplementedError(f"parameter adjoint for {type(pot).__name__}")
 
 
 # ---- batched snapshots ------------------------------------------------------------------

@@ -214,6 +214,20 @@ int fc3_third_order_lj_param_backward(fc3_handle h, const double* positions, int
                                       const double* masses, const float* dpsi32,
                                       const double* dpsi64, double grad[2]);
 
+/* Reverse mode of third_order(sys, StillingerWeberSilicon(), AutoDiffCalculator)
+ * (src/ifc.jl:20-35 -> src/autodiff/third_order_AD.jl:71-224, asr.jl:19-40,
+ * helper_funcs.jl:20-28) with respect to the SW parameters (SW.jl:3-14):
+ *   grad = {dL/dA, dL/dB, dL/dlambda, dL/dgamma}   (host)
+ * for the cotangent dpsi32 (of the FP32 psi32 of fc3_third_order_sw, mass weighted
+ * when masses != NULL) or dpsi64 (of the FP64 psi64, masses NULL).  sw_params as in
+ * fc3_third_order_sw.  Record blocks are recomputed; when tol > 0 the forward FP64
+ * tensor is assembled into the workspace to apply the tolerance mask (8 n^3 bytes).  */
+int fc3_third_order_sw_param_backward(fc3_handle h, const double* positions, int n_atoms,
+                                      const double box[3], const double* sw_params,
+                                      double r_cut, double tol, const double* masses,
+                                      const float* dpsi32, const double* dpsi64,
+                                      double grad[4]);
+
 #ifdef __cplusplus
 }
 #endif

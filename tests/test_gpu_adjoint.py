@@ -159,3 +159,43 @@ def test_lj_chain_sum_w_k3_squared(ctx):
         print(f"chain dL/d{name}: adjoint {g[name]:.9e} FD {fd:.9e} rel {abs(fd - g[name]) / abs(fd):.2e}",
               flush=True)
         assert abs(fd - g[name]) <= 2e-4 * abs(fd)
+
+
+SW_POT = S.StillingerWeberSilicon()
+
+
+def _sw_with(**kw):
+    from dataclasses import replace
+    return S.StillingerWeberSilicon(params=replace(SW_POT.params, **kw))
+
+
+@pytest.mark.parametrize("tol,seed,weighted", [(0.0, 51, False), (1e-12, 52, False), (1e-12, 53, True)])
+def test_sw_param_backward_central_difference(ctx, tol, seed, weighted):
+    """dF/d(A, B, lambda, gamma) of F = <dPsi, Psi_SW> against FP64 central differences
+    (relative step 1e-6) of the oracle's SW third_order (hyper-dual restatement of the
+    reference's AutoDiffCalculator)."""
+    sysd = S.diamond(S.SI_A, (2, 2, 2), S.SI_MASS, 0.03, seed=seed)
+    calc = S.AutoDiffCalculator(tol, SW_POT.r_cut)
+    n = sysd.n_dof
+    masses = np.random.default_rng(seed).uniform(20.0, 40.0, size=sysd.n_atoms) if weighted else None
+    if weighted:
+        d = _rand((n, n, n), seed + 100, np.float32)
+    else:
+        d = _rand((n, n, n), seed + 100)
+    g = fc3.third_order_param_backward(sysd, SW_POT, calc, fc3.DeviceArray.from_host(d, ctx=ctx),
+                                       masses=masses)
+
+    def F(pot):
+        psi = O.sw_third_order(sysd, pot, calc)
+        if weighted:
+            O.mass_weight_third_order(psi, masses)
+        return _dot(d, psi)
+
+    prm = SW_POT.params
+    for name, field in (("A", "A"), ("B", "B"), ("lambda", "lam"), ("gamma", "gamma")):
+        x = getattr(prm, field)
+        h = 1e-6 * x
+        fd = (F(_sw_with(**{field: x + h})) - F(_sw_with(**{field: x - h}))) / (2 * h)
+        print(f"SW tol={tol} weighted={weighted} dF/d{name}: adjoint {g[name]:.12e} FD {fd:.12e} "
+              f"rel {abs(fd - g[name]) / abs(fd):.2e}", flush=True)
+        assert abs(fd - g[name]) <= 1e-6 * abs(fd)

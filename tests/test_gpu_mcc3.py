@@ -149,3 +149,53 @@ def test_mcc3_bf16x3_rank3_large(ctx, inplace):
     err = np.sqrt(num / den)
     print(f"rank3 n={n} inplace={inplace}: rel fro err over {len(slabs)} slabs {err:.3e}", flush=True)
     assert err <= 1e-5
+
+
+def _sparse(psi):
+    i, j, k = np.nonzero(psi)
+    return i, j, k, psi[i, j, k].astype(np.float64)
+
+
+def test_config_fcc444_fp32_block_size_256(ctx):
+    """The plain small configuration: LJ argon FCC 4x4x4 (768 DoF), FP64 IFC2/IFC3 with
+    tol 1e-12, mass weighting, phi from the host FP64 eigendecomposition, plain FP32
+    mcc3 with block_size = 256 (host result, README.md:76), against the FP64 contraction
+    of the same FP32 Psi on sampled slabs."""
+    sysd = S.fcc(S.ARGON_A, (4, 4, 4), S.ARGON_MASS)
+    calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
+    n = sysd.n_dof
+    _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, S.ARGON_LJ, calc, psi32=psi32, masses=sysd.masses, ctx=ctx)
+    k3 = fc3.mcc3(psi32, fc3.DeviceArray.from_host(phi32, ctx=ctx), 256, precision="fp32")
+    sp = _sparse(psi32.to_host())
+    num = den = 0.0
+    for l in (0, 1, 2, 3, n // 2, n - 1):
+        ref = O.mcc3_fp64_slab(sp, phi32.astype(np.float64), l)
+        num += float(np.sum((k3[:, :, l].astype(np.float64) - ref) ** 2))
+        den += float(np.sum(ref * ref))
+    err = np.sqrt(num / den)
+    print(f"FCC 4x4x4 fp32 block 256: rel fro err {err:.3e}", flush=True)
+    assert err <= TOL["fp32"]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_config_sw_diamond333(ctx, prec):
+    """The three-body configuration: SW silicon on diamond 3x3x3 (648 DoF), sigma_u =
+    0.03 A; IFC2/IFC3 on the GPU (their parity with the oracle is in test_gpu_sw.py),
+    mass-weighted FP32 Psi, phi from the host eigendecomposition; mcc3 in both forms
+    against the FP64 contraction of the same FP32 Psi."""
+    pot = S.StillingerWeberSilicon()
+    calc = S.AutoDiffCalculator(1e-12, pot.r_cut)
+    sysd = S.diamond(S.SI_A, (3, 3, 3), S.SI_MASS, 0.03, seed=648)
+    n = sysd.n_dof
+    _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, pot, calc, ctx=ctx))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, pot, calc, psi32=psi32, masses=sysd.masses, ctx=ctx)
+    k3 = fc3.mcc3(psi32, fc3.DeviceArray.from_host(phi32, ctx=ctx), precision=prec).to_host()
+    ref = O.mcc3_fp64(psi32.to_host(), phi32)
+    err = rel_fro(k3, ref)
+    print(f"SW diamond 3x3x3 {prec}: rel fro err {err:.3e}", flush=True)
+    assert err <= TOL[prec]

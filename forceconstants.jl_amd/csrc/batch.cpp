@@ -117,10 +117,13 @@ static int tune_size(fc3_context* h, BatchState* B, char* base, const SnapPlan& 
     SnapPlan Q = P;
     Q.cfg = Mcc3Config{};
     FC3_TRY(snapshot_work(h->stream, base, Q, sn, prm, psi32, mws, mws_bytes));
+    // candidates of the requested precision: the pre-split kernels at three launch sizes
+    // for BF16X3, else (FP32, or BF16X3 with n % 4 != 0) the FP32 kernels
     std::vector<Mcc3Config> cands;
     if (mcc3_variant_applies((int)P.n, prm.precision, 3))
         for (long t : {8192L, 16384L, 32768L}) cands.push_back(Mcc3Config{3, t});
-    cands.push_back(Mcc3Config{2, 0});
+    else
+        cands.push_back(Mcc3Config{2, 0});
     BatchState::Tuned best;
     best.ms = -1.0f;
     for (const Mcc3Config& c : cands) {

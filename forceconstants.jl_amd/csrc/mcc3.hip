@@ -2,18 +2,19 @@
 //
 // Evaluated as three mode products on MFMA, ordered so that no n^3 temporary is
 // needed beyond the output:
-//   mode 3:     U[(i,j), l] = sum_k Psi[(i,j), k] phi[k, l]      one GEMM, M = n^2, written into K3
+//   mode 3:     U[(i,j), l] = sum_k Psi[(i,j), k] phi[k, l]      one GEMM over M = n^2 rows
 //   modes 1+2:  K3[:,:,l]  = phi^T U_l phi                       two n x n x n GEMMs per slab l,
-//               S_l = U_l^T phi (scratch), K3_l = S_l^T phi      in place on slab l of K3
-// Every GEMM has phi as its B operand, so phi is prepared once (zero-padded, and
-// split into bf16 hi/lo for BF16X3); the A operand is FP32 and is split in the
-// kernel while it is staged into LDS.
+//               S_l = U_l^T phi (scratch), K3_l = S_l^T phi      stacked over batches of slabs
+// Every GEMM has phi as its B operand, so phi is prepared once.
 //
-// Precisions
-//   FP32:    v_mfma_f32_32x32x2_f32, FP32 operands and accumulation.
-//   BF16X3:  a*b ~= ah*bh + ah*bl + al*bh with ah = bf16_rne(a), al = bf16_rne(a - ah),
-//            on v_mfma_f32_32x32x16_bf16, FP32 accumulation.  Intermediates (U, S)
-//            are FP32 and are split again when they are read.
+// Kernel families
+//   v2 (FP32):   v_mfma_f32_32x32x2_f32, FP32 operands read straight from Psi / K3 and
+//                staged through padded LDS; U lives in the K3 buffer.  Also runs BF16X3
+//                requests when n % 4 != 0 (FP32 is the more accurate of the two).
+//   v3 (BF16X3): a*b ~= ah*bh + ah*bl + al*bh with ah = bf16_rne(a), al = bf16_rne(a - ah),
+//                on v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  Operands are split
+//                once into K-blocked hi/lo planes (each GEMM writes its output already
+//                split for the next one) and copied global -> LDS without registers.
 #include <algorithm>
 
 #include "fc3_internal.hpp"
@@ -52,8 +53,7 @@ constexpr int PF = BK + 4;   // fp32 row pitch (80 B): conflict-free fragment re
 struct G2 {
     const float* A;      // A(m, k) = A[m * sam + k * sak]
     long sam, sak;
-    const void* Bhi;     // padded B(k, n) = B[n * ldb + k]: bf16 hi (or fp32)
-    const void* Blo;     // bf16 lo
+    const float* B;      // zero-padded B(k, n) = B[n * ldb + k]
     int ldb;
     float* C;
     long R, SS, CS;      // C(m, c) = C[(m / R) * SS + m % R + c * CS]
@@ -61,19 +61,6 @@ struct G2 {
     int tiles_m, tiles_n;
     long per_xcd;
 };
-
-// BF16 operand tiles in LDS: each row (an A row or a B column) holds BK = 16 bf16,
-// two 16-byte chunks, unpadded.  Blocks of 4 rows fill the 32 banks once; inside a
-// block, row r sits in slot (r & 3) ^ (block & 3) and its chunks are swapped on odd
-// blocks.  This makes the MFMA fragment reads (8 consecutive rows, one chunk each),
-// the row-wise stores (8 rows x one chunk) and the transposing stores of an
-// M-contiguous A (4 rows that are 4 apart x 4 k-quads) all bank-conflict-free.
-// Returns the element offset of chunk c of row r.
-__device__ __forceinline__ int swz(int r, int c)
-{
-    const int b = r >> 2;
-    return 64 * b + 16 * ((r & 3) ^ (b & 3)) + 8 * (c ^ (b & 1));
-}
 
 // ---- global -> registers -------------------------------------------------------
 // A tile (BM x BK fp32), 16 values per thread:
@@ -165,173 +152,19 @@ struct BRegs {
     u32x4 r0, r1, r2, r3;
 };
 
-template <int PREC>
 __device__ __forceinline__ void load_b2(const G2& g, int n0, int k0, BRegs& b)
 {
     const int t = threadIdx.x;
-    if constexpr (PREC == FC3_PREC_BF16X3) {
-        const int col = t >> 1, c = t & 1;
-        const __bf16* ph = static_cast<const __bf16*>(g.Bhi) + (long)(n0 + col) * g.ldb + k0 + 8 * c;
-        const __bf16* pl = static_cast<const __bf16*>(g.Blo) + (long)(n0 + col) * g.ldb + k0 + 8 * c;
-        const long o = 128L * g.ldb;
-        b.r0 = *reinterpret_cast<const u32x4*>(ph);
-        b.r1 = *reinterpret_cast<const u32x4*>(ph + o);
-        b.r2 = *reinterpret_cast<const u32x4*>(pl);
-        b.r3 = *reinterpret_cast<const u32x4*>(pl + o);
-    } else {
-        const int col = t >> 2, q = t & 3;
-        const float* p = static_cast<const float*>(g.Bhi) + (long)(n0 + col) * g.ldb + k0 + 4 * q;
-        const long o = 64L * g.ldb;
-        b.r0 = *reinterpret_cast<const u32x4*>(p);
-        b.r1 = *reinterpret_cast<const u32x4*>(p + o);
-        b.r2 = *reinterpret_cast<const u32x4*>(p + 2 * o);
-        b.r3 = *reinterpret_cast<const u32x4*>(p + 3 * o);
-    }
-}
-
-// ---- registers -> LDS (BF16X3) -------------------------------------------------
-template <bool AMC>
-__device__ __forceinline__ void store_a_split(__bf16* sAh, __bf16* sAl, const float (&v)[16])
-{
-    const int t = threadIdx.x;
-    if constexpr (AMC) {
-        const int mq = t >> 2, kq = t & 3;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            bf16x4 h, l;
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                __bf16 hh, ll;
-                split_bf16(v[4 * r + kk], hh, ll);
-                h[kk] = hh;
-                l[kk] = ll;
-            }
-            const int off = swz(4 * mq + r, kq >> 1) + 4 * (kq & 1);
-            *reinterpret_cast<bf16x4*>(sAh + off) = h;
-            *reinterpret_cast<bf16x4*>(sAl + off) = l;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            bf16x8 h, l;
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-                __bf16 hh, ll;
-                split_bf16(v[8 * c + kk], hh, ll);
-                h[kk] = hh;
-                l[kk] = ll;
-            }
-            const int off = swz(t, c);
-            *reinterpret_cast<bf16x8*>(sAh + off) = h;
-            *reinterpret_cast<bf16x8*>(sAl + off) = l;
-        }
-    }
-}
-
-__device__ __forceinline__ void store_b_bf16(__bf16* sBh, __bf16* sBl, const BRegs& b)
-{
-    const int t = threadIdx.x;
-    const int col = t >> 1, c = t & 1;
-    *reinterpret_cast<u32x4*>(sBh + swz(col, c)) = b.r0;
-    *reinterpret_cast<u32x4*>(sBh + swz(col + 128, c)) = b.r1;
-    *reinterpret_cast<u32x4*>(sBl + swz(col, c)) = b.r2;
-    *reinterpret_cast<u32x4*>(sBl + swz(col + 128, c)) = b.r3;
+    const int col = t >> 2, q = t & 3;
+    const float* p = g.B + (long)(n0 + col) * g.ldb + k0 + 4 * q;
+    const long o = 64L * g.ldb;
+    b.r0 = *reinterpret_cast<const u32x4*>(p);
+    b.r1 = *reinterpret_cast<const u32x4*>(p + o);
+    b.r2 = *reinterpret_cast<const u32x4*>(p + 2 * o);
+    b.r3 = *reinterpret_cast<const u32x4*>(p + 3 * o);
 }
 
 constexpr int TILE_E = BM * BK;   // bf16 elements of one operand tile (BM == BN)
-
-// The K loop, BF16X3.  Two LDS stages and a ring of three register stages for
-// the global prefetch, so a global load has three K steps to arrive.  In step k:
-//   read fragments of step k from stage k % 2
-//   MFMA (al bh), (ah bl)
-//   store registers (data of step k+1, loaded in step k-2) -> stage (k+1) % 2
-//   global load of step k+3 into the registers just freed
-//   MFMA (ah bh); barrier
-// The loop is unrolled by 6 so that stage and register indices are static; K is
-// padded to a multiple of 6 BK (zero-padded phi, A zeroed by its bounds check),
-// and prefetches past the end are clamped to the last step and never consumed.
-constexpr int KSTEP_MULT = 6;
-
-template <bool AMC, bool VEC, bool FAST>
-__device__ __forceinline__ void kloop_bf16(const G2& g, int m0, int n0, __bf16 (*s)[4][TILE_E],
-                                           f32x16 (&acc)[4][4])
-{
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm0 = (wave & 1) * 128, wn0 = (wave >> 1) * 128;
-    const int nk = KSTEP_MULT * ((g.K + KSTEP_MULT * BK - 1) / (KSTEP_MULT * BK));
-    const int klast = (nk - 1) * BK;
-    float av0[16], av1[16], av2[16];
-    BRegs bv0, bv1, bv2;
-    int offa[4], offb[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        offa[i] = swz(wm0 + i * 32 + (lane & 31), lane >> 5);
-        offb[i] = swz(wn0 + i * 32 + (lane & 31), lane >> 5);
-    }
-#define FC3_KN(x) ((x) < klast ? (x) : klast)
-    // Issue order inside one step (one wave per SIMD issues at most one instruction
-    // per slot, so MFMAs must be spread between the other instructions): the first
-    // fragments, then each remaining fragment read, split VALU, LDS store and global
-    // load placed between MFMAs.
-#define FC3_SGB(mask, n) __builtin_amdgcn_sched_group_barrier(mask, n, 0);
-#define FC3_INTERLEAVE()                                                                      \
-    FC3_SGB(0x100, 6)                                                                         \
-    _Pragma("unroll") for (int q = 0; q < 10; ++q) { FC3_SGB(0x008, 1) FC3_SGB(0x100, 1) }     \
-    _Pragma("unroll") for (int q = 0; q < 12; ++q) {                                          \
-        FC3_SGB(0x008, 1) FC3_SGB(0x002, 4) FC3_SGB(0x200, 1) }                               \
-    _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                           \
-        FC3_SGB(0x008, 1) FC3_SGB(0x020, 1) FC3_SGB(0x002, 3) }                               \
-    _Pragma("unroll") for (int q = 0; q < 18; ++q) { FC3_SGB(0x008, 1) FC3_SGB(0x002, 2) }
-#define FC3_MMA(X, Y)                                                                         \
-    _Pragma("unroll") for (int i = 0; i < 4; ++i)                                             \
-        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                         \
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[i], Y[j], acc[i][j], 0, 0, 0);
-#define FC3_STEP(cur, avs, bvs, knext)                                                        \
-    {                                                                                         \
-        bf16x8 ah[4], al[4], bh[4], bl[4];                                                    \
-        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
-            ah[i] = *reinterpret_cast<const bf16x8*>(&s[cur][0][offa[i]]);                    \
-            al[i] = *reinterpret_cast<const bf16x8*>(&s[cur][1][offa[i]]);                    \
-            bh[i] = *reinterpret_cast<const bf16x8*>(&s[cur][2][offb[i]]);                    \
-            bl[i] = *reinterpret_cast<const bf16x8*>(&s[cur][3][offb[i]]);                    \
-        }                                                                                     \
-        FC3_MMA(al, bh)                                                                       \
-        FC3_MMA(ah, bl)                                                                       \
-        store_a_split<AMC>(s[(cur) ^ 1][0], s[(cur) ^ 1][1], avs);                           \
-        store_b_bf16(s[(cur) ^ 1][2], s[(cur) ^ 1][3], bvs);                                 \
-        load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(knext), avs);                                  \
-        load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(knext), bvs);                                  \
-        FC3_MMA(ah, bh)                                                                       \
-        FC3_INTERLEAVE()                                                                      \
-        __syncthreads();                                                                      \
-    }
-    // prologue: stage 0 <- step 0; registers: ring slot r holds step 1 + ((k - 1 - r) mod 3)
-    load_a2<AMC, VEC, FAST>(g, m0, 0, av0);
-    load_b2<FC3_PREC_BF16X3>(g, n0, 0, bv0);
-    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(BK), av1);
-    load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(BK), bv1);
-    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(2 * BK), av2);
-    load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(2 * BK), bv2);
-    store_a_split<AMC>(s[0][0], s[0][1], av0);
-    store_b_bf16(s[0][2], s[0][3], bv0);
-    load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(3 * BK), av0);
-    load_b2<FC3_PREC_BF16X3>(g, n0, FC3_KN(3 * BK), bv0);
-    __syncthreads();
-    // now av1 = step 1, av2 = step 2, av0 = step 3
-    for (int kt = 0; kt < nk; kt += KSTEP_MULT) {
-        FC3_STEP(0, av1, bv1, (kt + 4) * BK)   // k = kt    : stage 1 <- k+1, fetch k+4
-        FC3_STEP(1, av2, bv2, (kt + 5) * BK)   // k = kt + 1: stage 0 <- k+1 (= kt+2)
-        FC3_STEP(0, av0, bv0, (kt + 6) * BK)   // k = kt + 2
-        FC3_STEP(1, av1, bv1, (kt + 7) * BK)   // k = kt + 3
-        FC3_STEP(0, av2, bv2, (kt + 8) * BK)   // k = kt + 4
-        FC3_STEP(1, av0, bv0, (kt + 9) * BK)   // k = kt + 5
-    }
-#undef FC3_STEP
-#undef FC3_MMA
-#undef FC3_INTERLEAVE
-#undef FC3_SGB
-#undef FC3_KN
-}
 
 // The K loop, FP32 (same pipeline; padded row-major LDS tiles).
 template <bool AMC, bool VEC, bool FAST>
@@ -380,7 +213,7 @@ __device__ __forceinline__ void kloop_fp32(const G2& g, int m0, int n0, float (*
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s2], fb[j][s2], acc[i][j], 0, 0, 0); \
         FC3_STORE((cur) ^ 1, avs, bvs)                                                        \
         load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(knext), avs);                                  \
-        load_b2<FC3_PREC_FP32>(g, n0, FC3_KN(knext), bvs);                                    \
+        load_b2(g, n0, FC3_KN(knext), bvs);                                    \
         _Pragma("unroll") for (int s2 = 4; s2 < 8; ++s2)                                      \
             _Pragma("unroll") for (int i = 0; i < 4; ++i)                                     \
                 _Pragma("unroll") for (int j = 0; j < 4; ++j)                                 \
@@ -388,12 +221,12 @@ __device__ __forceinline__ void kloop_fp32(const G2& g, int m0, int n0, float (*
         __syncthreads();                                                                      \
     }
     load_a2<AMC, VEC, FAST>(g, m0, 0, av0);
-    load_b2<FC3_PREC_FP32>(g, n0, 0, bv0);
+    load_b2(g, n0, 0, bv0);
     FC3_STORE(0, av0, bv0)
     load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(BK), av1);
-    load_b2<FC3_PREC_FP32>(g, n0, FC3_KN(BK), bv1);
+    load_b2(g, n0, FC3_KN(BK), bv1);
     load_a2<AMC, VEC, FAST>(g, m0, FC3_KN(2 * BK), av0);
-    load_b2<FC3_PREC_FP32>(g, n0, FC3_KN(2 * BK), bv0);
+    load_b2(g, n0, FC3_KN(2 * BK), bv0);
     __syncthreads();
     for (int kt = 0; kt < nk; kt += 2) {
         FC3_STEPF(0, av1, bv1, (kt + 3) * BK)
@@ -407,7 +240,7 @@ __device__ __forceinline__ void kloop_fp32(const G2& g, int m0, int n0, float (*
 // 256 x 256 macro tile, 4 waves (2 x 2), each a 128 x 128 tile of 4 x 4 MFMA 32x32
 // blocks; XCD-aware tile order.  Interior tiles (inside M, K a multiple of 2 BK)
 // take the unchecked load path.
-template <int PREC, bool AMC, bool VEC>
+template <bool AMC, bool VEC>
 __global__ void __launch_bounds__(NT, 1) k_gemm2(G2 g)
 {
     const long wg = blockIdx.x;
@@ -426,18 +259,11 @@ __global__ void __launch_bounds__(NT, 1) k_gemm2(G2 g)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    if constexpr (PREC == FC3_PREC_BF16X3) {
-        const bool fast = VEC && (m0 + BM <= g.M) && (g.K % (KSTEP_MULT * BK) == 0);
-        __shared__ __attribute__((aligned(16))) __bf16 s[2][4][TILE_E];
-        if (fast) kloop_bf16<AMC, VEC, VEC>(g, m0, n0, s, acc);
-        else kloop_bf16<AMC, VEC, false>(g, m0, n0, s, acc);
-    } else {
-        const bool fast = VEC && (m0 + BM <= g.M) && (g.K % (2 * BK) == 0);
-        __shared__ __attribute__((aligned(16))) float sA[2][BM][PF];
-        __shared__ __attribute__((aligned(16))) float sB[2][BN][PF];
-        if (fast) kloop_fp32<AMC, VEC, VEC>(g, m0, n0, sA, sB, acc);
-        else kloop_fp32<AMC, VEC, false>(g, m0, n0, sA, sB, acc);
-    }
+    const bool fast = VEC && (m0 + BM <= g.M) && (g.K % (2 * BK) == 0);
+    __shared__ __attribute__((aligned(16))) float sA[2][BM][PF];
+    __shared__ __attribute__((aligned(16))) float sB[2][BN][PF];
+    if (fast) kloop_fp32<AMC, VEC, VEC>(g, m0, n0, sA, sB, acc);
+    else kloop_fp32<AMC, VEC, false>(g, m0, n0, sA, sB, acc);
 
     // epilogue: register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
 #pragma unroll
@@ -766,70 +592,47 @@ __global__ void __launch_bounds__(256) k_zero_kblock(__bf16* __restrict__ base, 
 
 }  // namespace v3
 
-// phi (n x n, col-major, ld n) -> zero-padded (Kpad x Npad, ld Kpad): fp32, or bf16 hi/lo.
-__global__ void k_prep_phi(const float* __restrict__ phi, int n, int Kpad, int Npad, int prec,
-                           float* __restrict__ pf, __bf16* __restrict__ ph, __bf16* __restrict__ pl)
+// phi (n x n, col-major, ld n) -> zero-padded FP32 (Kpad x Npad, ld Kpad).
+__global__ void k_prep_phi(const float* __restrict__ phi, int n, int Kpad, int Npad,
+                           float* __restrict__ pf)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)Kpad * Npad) return;
     const int c = (int)(t / Kpad), k = (int)(t - (long)c * Kpad);
-    const float x = (k < n && c < n) ? phi[k + (long)c * n] : 0.0f;
-    if (prec == FC3_PREC_FP32) {
-        pf[t] = x;
-    } else {
-        __bf16 h, l;
-        split_bf16(x, h, l);
-        ph[t] = h;
-        pl[t] = l;
-    }
+    pf[t] = (k < n && c < n) ? phi[k + (long)c * n] : 0.0f;
 }
 
 struct PhiPrep {
-    const void* hi;
-    const void* lo;
+    const float* b;
     int ldb;
 };
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-size_t phi_bytes(int n, int prec)
-{
-    const size_t e = (size_t)round_up(n, BK) * round_up(n, v2::BN);
-    return prec == FC3_PREC_FP32 ? Carve::need(e * 4) : 2 * Carve::need(e * 2);
-}
+size_t phi_bytes(int n) { return Carve::need((size_t)round_up(n, BK) * round_up(n, v2::BN) * 4); }
 
-int prep_phi(hipStream_t s, const float* phi, int n, int prec, Carve& cv, PhiPrep& out)
+int prep_phi(hipStream_t s, const float* phi, int n, Carve& cv, PhiPrep& out)
 {
     const int Kpad = round_up(n, BK), Npad = round_up(n, v2::BN);
     const long e = (long)Kpad * Npad;
-    float* pf = nullptr;
-    __bf16 *ph = nullptr, *pl = nullptr;
-    if (prec == FC3_PREC_FP32) {
-        pf = cv.take<float>(e);
-        out.hi = pf;
-        out.lo = nullptr;
-    } else {
-        ph = cv.take<__bf16>(e);
-        pl = cv.take<__bf16>(e);
-        out.hi = ph;
-        out.lo = pl;
-    }
+    float* pf = cv.take<float>(e);
+    out.b = pf;
     out.ldb = Kpad;
     hipLaunchKernelGGL(k_prep_phi, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, phi, n,
-                       Kpad, Npad, prec, pf, ph, pl);
+                       Kpad, Npad, pf);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
 
 // C(m, c) = sum_k A(m, k) phi(k, c) with A(m, k) = A[m sam + k sak] and
-// C(m, c) = C[(m / R) SS + m % R + c CS]  (v2 kernel).
-int gemm2(hipStream_t s, int kind, int prec, const float* A, long sam, long sak,
-          const PhiPrep& B, float* C, long R, long SS, long CS, long M, int N, int K)
+// C(m, c) = C[(m / R) SS + m % R + c CS]  (v2 kernel, FP32).
+int gemm2(hipStream_t s, int kind, const float* A, long sam, long sak, const PhiPrep& B,
+          float* C, long R, long SS, long CS, long M, int N, int K)
 {
     if (M >= (1L << 31)) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
     v2::G2 g;
     g.A = A; g.sam = sam; g.sak = sak;
-    g.Bhi = B.hi; g.Blo = B.lo; g.ldb = B.ldb;
+    g.B = B.b; g.ldb = B.ldb;
     g.C = C; g.R = R; g.SS = SS; g.CS = CS;
     g.M = (int)M; g.N = N; g.K = K;
     g.tiles_m = (int)((M + v2::BM - 1) / v2::BM);
@@ -844,14 +647,9 @@ int gemm2(hipStream_t s, int kind, int prec, const float* A, long sam, long sak,
                      ((uintptr_t)A % 16 == 0) && ((uintptr_t)C % 16 == 0);
     dim3 gd((unsigned)grid), bd(v2::NT);
     ProfScope prof(kind, s);
-#define FC3_LAUNCH2(P, AM, V) hipLaunchKernelGGL((v2::k_gemm2<P, AM, V>), gd, bd, 0, s, g)
-    if (prec == FC3_PREC_BF16X3) {
-        if (amc) { if (vec) FC3_LAUNCH2(FC3_PREC_BF16X3, true, true); else FC3_LAUNCH2(FC3_PREC_BF16X3, true, false); }
-        else     { if (vec) FC3_LAUNCH2(FC3_PREC_BF16X3, false, true); else FC3_LAUNCH2(FC3_PREC_BF16X3, false, false); }
-    } else {
-        if (amc) { if (vec) FC3_LAUNCH2(FC3_PREC_FP32, true, true); else FC3_LAUNCH2(FC3_PREC_FP32, true, false); }
-        else     { if (vec) FC3_LAUNCH2(FC3_PREC_FP32, false, true); else FC3_LAUNCH2(FC3_PREC_FP32, false, false); }
-    }
+#define FC3_LAUNCH2(AM, V) hipLaunchKernelGGL((v2::k_gemm2<AM, V>), gd, bd, 0, s, g)
+    if (amc) { if (vec) FC3_LAUNCH2(true, true); else FC3_LAUNCH2(true, false); }
+    else     { if (vec) FC3_LAUNCH2(false, true); else FC3_LAUNCH2(false, false); }
 #undef FC3_LAUNCH2
     FC3_HIP(hipGetLastError());
     return FC3_OK;
@@ -873,15 +671,15 @@ int slab_batch(int n)
 
 // Modes 1 and 2 on slabs [l0, l0+cnt) of T (each n x n, stride n^2), in place:
 // S_l = T_l^T phi, then T_l = S_l^T phi, each as one GEMM over the stacked slabs.
-int modes12(hipStream_t s, int prec, float* T, int n, int l0, int cnt, const PhiPrep& B,
+int modes12(hipStream_t s, float* T, int n, int l0, int cnt, const PhiPrep& B,
             float* S)
 {
     const long n2 = (long)n * n;
     // A(m = l n + j, k = i) = T_l[i + n j] = T[m n + k];  C(m, n') -> S_l[j + n n']
-    FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE12, prec, T + l0 * n2, n, 1, B, S, n, n2, n,
+    FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE12, T + l0 * n2, n, 1, B, S, n, n2, n,
                   (long)cnt * n, n, n));
     // A(m = l n + n', k = j) = S_l[j + n n'];  C(m, m') -> T_l[n' + n m']
-    FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE12, prec, S, n, 1, B, T + l0 * n2, n, n2, n,
+    FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE12, S, n, 1, B, T + l0 * n2, n, n2, n,
                   (long)cnt * n, n, n));
     return FC3_OK;
 }
@@ -1140,7 +938,7 @@ size_t mcc3_workspace_bytes(int n, int precision, bool inplace, const Mcc3Config
 {
     const long n2 = (long)n * n;
     const long tiles = cfg && cfg->tiles > 0 ? cfg->tiles : kTilesPerLaunch;
-    size_t b = phi_bytes(n, precision) + Carve::need(sizeof(float) * n2 * slab_batch(n)) + 4096;
+    size_t b = phi_bytes(n) + Carve::need(sizeof(float) * n2 * slab_batch(n)) + 4096;
     if (inplace) {
         // mode-3 row-block scratch: R rows x n
         const long R = 1L << 16;
@@ -1165,15 +963,15 @@ int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int preci
         return mcc3_v3(s, psi, phi, n, k3, ws, ws_bytes, tiles);
     Carve cv(ws, ws_bytes);
     PhiPrep B;
-    FC3_TRY(prep_phi(s, phi, n, precision, cv, B));
+    FC3_TRY(prep_phi(s, phi, n, cv, B));
     const int nb = slab_batch(n);
     const long n2 = (long)n * n;
     float* S = cv.take<float>((size_t)n2 * nb);
     // mode 3: U[(i,j), l] = sum_k Psi[(i,j), k] phi[k, l] -> K3 buffer
-    FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE3, precision, psi, 1, n2, B, k3, n2, 0, n2, n2, n, n));
+    FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE3, psi, 1, n2, B, k3, n2, 0, n2, n2, n, n));
     for (int l0 = 0; l0 < n; l0 += nb) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        FC3_TRY(modes12(s, precision, k3, n, l0, cnt, B, S));
+        FC3_TRY(modes12(s, k3, n, l0, cnt, B, S));
     }
     return FC3_OK;
 }
@@ -1185,13 +983,13 @@ int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int pre
         return mcc3_v3_inplace(s, psi, phi, n, ws, ws_bytes);   // default launch sizing
     Carve cv(ws, ws_bytes);
     PhiPrep B;
-    FC3_TRY(prep_phi(s, phi, n, precision, cv, B));
+    FC3_TRY(prep_phi(s, phi, n, cv, B));
     const int nb = slab_batch(n);
     const long n2 = (long)n * n;
     float* S = cv.take<float>((size_t)n2 * nb);
     for (int l0 = 0; l0 < n; l0 += nb) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        FC3_TRY(modes12(s, precision, psi, n, l0, cnt, B, S));
+        FC3_TRY(modes12(s, psi, n, l0, cnt, B, S));
     }
     // mode 3 in place by row blocks through scratch
     const long R0 = 1L << 16;
@@ -1199,7 +997,7 @@ int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int pre
     float* W = cv.take<float>((size_t)R * n);
     for (long r0 = 0; r0 < n2; r0 += R) {
         const long rows = (n2 - r0) < R ? (n2 - r0) : R;
-        FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE3, precision, psi + r0, 1, n2, B, W, rows, 0, rows,
+        FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE3, psi + r0, 1, n2, B, W, rows, 0, rows,
                       rows, n, n));
         FC3_HIP(hipMemcpy2DAsync(psi + r0, sizeof(float) * n2, W, sizeof(float) * rows,
                                  sizeof(float) * rows, n, hipMemcpyDeviceToDevice, s));

@@ -44,7 +44,8 @@ typedef enum {
 
 typedef enum {
     FC3_PREC_FP32 = 0,    /* plain FP32 on FP32 MFMA (the reference's Float32 path) */
-    FC3_PREC_BF16X3 = 1   /* BF16 MFMA, hi/lo split, 3 products, FP32 accumulate */
+    FC3_PREC_BF16X3 = 1   /* BF16 MFMA, hi/lo split, 3 products, FP32 accumulate;
+                             mcc3 with n % 4 != 0 runs the FP32 form instead */
 } fc3_precision;
 
 typedef struct fc3_context* fc3_handle;
@@ -182,9 +183,9 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
                                   double r_cut, double tol, int precision, int flags);
 /* Nodes of the last captured batch graph. */
 int fc3_batch_graph_nodes(fc3_handle h, size_t* nodes);
-/* Tuning table: the chosen mcc3 configuration of size n (variant 2 = FP32-operand
- * kernels, 3 = pre-split BF16x3 kernels; tiles = workgroups per GEMM launch) and its
- * time; save/load as text lines "n precision variant tiles ms".                   */
+/* Tuning table: the chosen mcc3 configuration of size n (variant 2 = FP32 kernels,
+ * 3 = pre-split BF16x3 kernels; tiles = workgroups per GEMM launch) and its time;
+ * save/load as text lines "n precision variant tiles ms".                         */
 int fc3_autotune_query(fc3_handle h, int n, int precision, int* variant, long* tiles,
                        float* ms);
 int fc3_autotune_save(fc3_handle h, const char* path);

@@ -366,99 +366,127 @@ __device__ __forceinline__ void lds_copy16(const __bf16* g, __bf16* lds)
                                      16, 0, 0);
 }
 
-template <bool SPLIT_OUT>
+// Tile shape: a BM x BN macro tile over 4 waves arranged WM x WN, each wave TM x TN MFMA
+// 32x32 blocks: 256 x 256 as 2 x 2 waves of 4 x 4 blocks.  (A 256 x 288 shape, 4 x 1
+// waves of 2 x 9 blocks, would fit n = 2592 = 9 x 288 without an almost empty last
+// column tile, but ran 20% slower: every wave then reads all B fragments.)
+template <int SHAPE>
+struct Shape;
+template <>
+struct Shape<0> {
+    static constexpr int WM = 2, WN = 2, TM = 4, TN = 4;
+};
+
+template <bool SPLIT_OUT, int SHAPE>
 __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
 {
+    using SH = Shape<SHAPE>;
+    constexpr int TBM = SH::WM * SH::TM * 32, TBN = SH::WN * SH::TN * 32;
+    static_assert(TBM == BM, "A tile is 256 rows");
+    constexpr int AE = TBM * BK, BE = TBN * BK;        // bf16 elements per plane
+    constexpr int CA = TBM / 32, CB = TBN / 32;        // 64-chunk copies per plane
     const long wg = blockIdx.x;
     const long tile = (wg % kXcd) * g.per_xcd + wg / kXcd;
     if (tile >= (long)g.tiles_m * g.tiles_n) return;
     const int tm = (int)(tile / g.tiles_n), tn = (int)(tile - (long)tm * g.tiles_n);
-    const int m0 = tm * BM, n0 = tn * BN;
+    const int m0 = tm * TBM, n0 = tn * TBN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm0 = (wave & 1) * 128, wn0 = (wave >> 1) * 128;
+    const int wm0 = (wave % SH::WM) * (SH::TM * 32), wn0 = (wave / SH::WM) * (SH::TN * 32);
 
     // One LDS object per stage: the compiler then sees that the fragment reads of one
     // stage do not alias the copies in flight into the others, and adds no wait.
-    __shared__ __attribute__((aligned(16))) __bf16 s0[4 * TILE_E];
-    __shared__ __attribute__((aligned(16))) __bf16 s1[4 * TILE_E];
-    __shared__ __attribute__((aligned(16))) __bf16 s2[4 * TILE_E];
-    __shared__ __attribute__((aligned(16))) __bf16 s3[4 * TILE_E];
+    // A stage holds A hi, A lo, B hi, B lo.
+    __shared__ __attribute__((aligned(16))) __bf16 s0[2 * AE + 2 * BE];
+    __shared__ __attribute__((aligned(16))) __bf16 s1[2 * AE + 2 * BE];
+    __shared__ __attribute__((aligned(16))) __bf16 s2[2 * AE + 2 * BE];
+    __shared__ __attribute__((aligned(16))) __bf16 s3[2 * AE + 2 * BE];
 
-    // Copy plan: wave w fills array w (0 A hi, 1 A lo, 2 B hi, 3 B lo) of a stage with
-    // 8 copies of 64 chunks; chunk P = 64 q + lane of the tile (layout: lds_chunk)
-    // holds row r = 16 (P >> 5) + (P & 15), k-half c = (P >> 4) & 1.  Copy q covers
-    // rows 32 q .. 32 q + 31, one contiguous KB of the K block (one slab permitting).
-    const __bf16* src[8];
+    // Copy plan: wave w fills plane w (0 A hi, 1 A lo, 2 B hi, 3 B lo) of a stage with
+    // CA or CB copies of 64 chunks; chunk P = 64 q + lane of the plane (layout:
+    // lds_chunk) holds row r = 16 (P >> 5) + (P & 15), k-half c = (P >> 4) & 1.  Copy q
+    // covers rows 32 q .. 32 q + 31, one contiguous KB of the K block (one slab permitting).
+    constexpr int CMAX = CA > CB ? CA : CB;
+    const bool is_a = wave < 2;
+    const int ncopy = is_a ? CA : CB;
+    const __bf16* src[CMAX];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < CMAX; ++q) {
         const int P = 64 * q + lane;
         const int r = 16 * (P >> 5) + (P & 15);
         const int c = (P >> 4) & 1;
-        if (wave < 2) {
-            int m = m0 + r;
+        if (is_a) {
+            int m = m0 + (q < CA ? r : 0);
             if (m >= g.M) m = g.M - 1;        // rows past M feed only discarded rows of C
             const unsigned sl = (unsigned)m / (unsigned)g.a_R;
             const long arow = (long)sl * g.a_SS + 16L * (m - (int)sl * g.a_R);
             src[q] = g.A + arow + (wave ? g.a_lo : 0) + 8 * c;
         } else {
-            src[q] = g.B + (wave == 3 ? g.b_lo : 0) + 16L * (n0 + r) + 8 * c;
+            src[q] = g.B + (wave == 3 ? g.b_lo : 0) + 16L * (n0 + (q < CB ? r : 0)) + 8 * c;
         }
     }
-    const long kstride = wave < 2 ? g.a_KS : g.b_KS;   // elements per K block
+    const long kstride = is_a ? g.a_KS : g.b_KS;   // elements per K block
     const int nk = g.K / BK;
-    const int wofs = wave * TILE_E;
-    // this wave's 8 copies of step k into stage array `st`; steps past the end copy the
+    const int wofs = wave == 0 ? 0 : wave == 1 ? AE : wave == 2 ? 2 * AE : 2 * AE + BE;
+    // this wave's copies of step k into stage array `st`; steps past the end copy the
     // last step again (into a stage that is not read any more), so that every step has
     // the same number of copies in flight
     auto issue = [&](__bf16* st, int k) {
         const long kk = (k < nk ? k : nk - 1) * kstride;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) lds_copy16(src[q] + kk, st + wofs + 512 * q);
+        for (int q = 0; q < CMAX; ++q)
+            if (q < ncopy) lds_copy16(src[q] + kk, st + wofs + 512 * q);
     };
 
-    f32x16 acc[4][4];
+    f32x16 acc[SH::TM][SH::TN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < SH::TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < SH::TN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    int offa[4], offb[4];
+    int offa[SH::TM], offb[SH::TN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        offa[i] = 8 * lds_chunk(wm0 + i * 32 + (lane & 31), lane >> 5);
-        offb[i] = 8 * lds_chunk(wn0 + i * 32 + (lane & 31), lane >> 5);
-    }
+    for (int i = 0; i < SH::TM; ++i) offa[i] = 8 * lds_chunk(wm0 + i * 32 + (lane & 31), lane >> 5);
+#pragma unroll
+    for (int j = 0; j < SH::TN; ++j) offb[j] = 8 * lds_chunk(wn0 + j * 32 + (lane & 31), lane >> 5);
 
 #define FC3_MMA3(X, Y)                                                                      \
-    _Pragma("unroll") for (int i = 0; i < 4; ++i)                                           \
-        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                       \
+    _Pragma("unroll") for (int i = 0; i < SH::TM; ++i)                                      \
+        _Pragma("unroll") for (int j = 0; j < SH::TN; ++j)                                  \
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[i], Y[j], acc[i][j], 0, 0, 0);
+    // This wave's copies of the two newest steps may be in flight at a barrier.
+#define FC3_RING_BARRIER()                                                                  \
+    {                                                                                       \
+        if (is_a) ring_barrier<2 * CA>();                                                   \
+        else ring_barrier<2 * CB>();                                                        \
+    }
     // Step K reads stage SRD and refills stage SWR (read in step K-1, retired by the
     // barrier that ended it) with step K+3.  The closing barrier leaves the copies of
     // steps K+2 and K+3 in flight and those of K+1 in LDS.
 #define FC3_STEP3(K, SRD, SWR)                                                              \
     {                                                                                       \
-        bf16x8 ah[4], al[4], bh[4], bl[4];                                                  \
-        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                     \
+        bf16x8 ah[SH::TM], al[SH::TM], bh[SH::TN], bl[SH::TN];                              \
+        _Pragma("unroll") for (int i = 0; i < SH::TM; ++i) {                                \
             ah[i] = *reinterpret_cast<const bf16x8*>(SRD + offa[i]);                        \
-            al[i] = *reinterpret_cast<const bf16x8*>(SRD + TILE_E + offa[i]);               \
-            bh[i] = *reinterpret_cast<const bf16x8*>(SRD + 2 * TILE_E + offb[i]);           \
-            bl[i] = *reinterpret_cast<const bf16x8*>(SRD + 3 * TILE_E + offb[i]);           \
+            al[i] = *reinterpret_cast<const bf16x8*>(SRD + AE + offa[i]);                   \
+        }                                                                                   \
+        _Pragma("unroll") for (int j = 0; j < SH::TN; ++j) {                                \
+            bh[j] = *reinterpret_cast<const bf16x8*>(SRD + 2 * AE + offb[j]);               \
+            bl[j] = *reinterpret_cast<const bf16x8*>(SRD + 2 * AE + BE + offb[j]);          \
         }                                                                                   \
         FC3_MMA3(al, bh)                                                                    \
         issue(SWR, (K) + 3);                                                                \
         FC3_MMA3(ah, bl)                                                                    \
         FC3_MMA3(ah, bh)                                                                    \
-        ring_barrier<16>();                                                                 \
+        FC3_RING_BARRIER()                                                                  \
     }
 
     // prologue: steps 0..2 in flight, step 0 landed
     issue(s0, 0);
     issue(s1, 1);
     issue(s2, 2);
-    ring_barrier<16>();
+    FC3_RING_BARRIER()
     int k = 0;
     for (; k + 4 <= nk; k += 4) {
         FC3_STEP3(k, s0, s3)
@@ -470,14 +498,15 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
     if (k + 1 < nk) FC3_STEP3(k + 1, s1, s0)
     if (k + 2 < nk) FC3_STEP3(k + 2, s2, s1)
 #undef FC3_STEP3
+#undef FC3_RING_BARRIER
 #undef FC3_MMA3
     wait_vmcnt0();   // no copy may still be landing when the workgroup's LDS is released
 
     // epilogue: register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < SH::TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < SH::TN; ++j) {
             const int col = n0 + wn0 + j * 32 + (lane & 31);
             if (col >= g.N) continue;
 #pragma unroll
@@ -784,10 +813,9 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
     const long grid = g.per_xcd * kXcd;
     if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
     ProfScope prof(kind, s);
-    if (split_out)
-        hipLaunchKernelGGL(v3::k_gemm3<true>, dim3((unsigned)grid), dim3(v3::NT), 0, s, g);
-    else
-        hipLaunchKernelGGL(v3::k_gemm3<false>, dim3((unsigned)grid), dim3(v3::NT), 0, s, g);
+    const dim3 gd((unsigned)grid), bd(v3::NT);
+    if (split_out) hipLaunchKernelGGL((v3::k_gemm3<true, 0>), gd, bd, 0, s, g);
+    else hipLaunchKernelGGL((v3::k_gemm3<false, 0>), gd, bd, 0, s, g);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }

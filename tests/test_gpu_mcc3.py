@@ -34,7 +34,7 @@ def random_case(n, seed):
     return np.asfortranarray(psi.astype(np.float32)), np.asfortranarray(phi.astype(np.float32))
 
 
-@pytest.mark.parametrize("n", [16, 20, 48, 96, 200, 99, 256, 272, 324])
+@pytest.mark.parametrize("n", [16, 20, 48, 96, 200, 99, 256, 272, 288, 324])
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_mcc3_random(ctx, n, prec):
     psi, phi = random_case(n, 100 + n)

@@ -184,6 +184,75 @@ def batched_snapshots(ctx, per_size=32, replays=3):
     return res
 
 
+def adjoint_timing(ctx, cells=5, reps=2):
+    """The differentiation configuration: LJ argon FCC 5x5x5 (1500 DoF), sigma_u = 0.05 A
+    (seeded).  Forward = assembly (mass-weighted FP32 Psi) + BF16x3 mcc3; backward =
+    mcc3 pullback (dL/dPsi and dL/dphi) + the sigma/epsilon pullback of the assembly.
+    Timed with the cotangent dL/dK3 = K3 (L = 1/2 sum K3^2: the timing does not depend on
+    the weights).  Gradients are checked in tests at sizes where FP64 differences are
+    affordable; here dL/depsilon is also checked against <dL/dPsi, Psi> / epsilon, exact
+    because Psi is linear in epsilon."""
+    import fc3
+    from fc3 import systems as S
+    calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
+    sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, 0.05, seed=1500)
+    n = sysd.n_dof
+    _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
+    dphi = fc3.DeviceArray.from_host(phi, dtype=np.float32, ctx=ctx)
+    psi = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    k3 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+
+    def forward():
+        fc3.third_order_device(sysd, S.ARGON_LJ, calc, psi32=psi, masses=sysd.masses, ctx=ctx)
+        fc3.mcc3(psi, dphi, precision="bf16x3", out=k3)
+
+    def backward():
+        dpsi, dph = fc3.mcc3_backward(psi, dphi, k3, precision="bf16x3")
+        g = fc3.third_order_param_backward(sysd, S.ARGON_LJ, calc, dpsi, masses=sysd.masses)
+        return dpsi, dph, g
+
+    forward()
+    dpsi, dph, g = backward()
+    ctx.synchronize()
+    # exact check: dL/depsilon = <dL/dPsi, Psi> / epsilon
+    import ctypes
+    from fc3._lib import check
+    num = 0.0
+    a = np.empty((n, n), dtype=np.float32, order="F")
+    b = np.empty((n, n), dtype=np.float32, order="F")
+    for l in range(n):
+        off = 4 * n * n * l
+        check(ctx.lib.fc3_memcpy_d2h(ctx.h, a.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.c_void_p(dpsi.ptr.value + off), 4 * n * n))
+        check(ctx.lib.fc3_memcpy_d2h(ctx.h, b.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.c_void_p(psi.ptr.value + off), 4 * n * n))
+        num += float(np.dot(a.ravel().astype(np.float64), b.ravel().astype(np.float64)))
+    eps_check = num / S.ARGON_LJ.epsilon
+    dpsi.free()
+    dph.free()
+    ctx.timer_start()
+    for _ in range(reps):
+        forward()
+    t_fwd = ctx.timer_stop() / reps
+    ctx.timer_start()
+    for _ in range(reps):
+        forward()
+        dp, df, _ = backward()
+        dp.free()
+        df.free()
+    t_fb = ctx.timer_stop() / reps
+    for d in (psi, k3, dphi):
+        d.free()
+    res = {"config": f"LJ argon FCC {cells}x{cells}x{cells} ({n} DoF)", "forward_ms": t_fwd,
+           "forward_backward_ms": t_fb, "ratio": t_fb / t_fwd,
+           "dL_dsigma": g["sigma"], "dL_depsilon": g["epsilon"],
+           "dL_depsilon_vs_exact_rel": abs(g["epsilon"] - eps_check) / abs(eps_check),
+           "note": "backward = mcc3 pullback (dPsi on the BF16x3 kernels is not used: dphi "
+                   "needs the FP32 mode terms, which also give dPsi) + sigma/epsilon pullback"}
+    log(f"adjoint: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +263,8 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-snapshot section (96 snapshots of 324/768/1500 DoF)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward+backward timing at FCC 5x5x5 (1500 DoF)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -320,6 +391,8 @@ def main():
     k3.free()
     if not a.no_batched:
         out["batched"] = batched_snapshots(ctx)
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_timing(ctx)
     if not a.no_cpu_baseline:
         log("cpu baselines")
         out["cpu_baseline"] = cpu_baseline_mcc3()

@@ -370,11 +370,25 @@ __device__ __forceinline__ void lds_copy16(const __bf16* g, __bf16* lds)
 // 32x32 blocks: 256 x 256 as 2 x 2 waves of 4 x 4 blocks.  (A 256 x 288 shape, 4 x 1
 // waves of 2 x 9 blocks, would fit n = 2592 = 9 x 288 without an almost empty last
 // column tile, but ran 20% slower: every wave then reads all B fragments.)
+// Narrow shapes 1-3 (256 x 32, 64, 128 as 4 x 1 waves of 2 x TN blocks) take the last
+// N % 256 columns in a launch of their own, so that no 256-wide tile is spent on them.
 template <int SHAPE>
 struct Shape;
 template <>
 struct Shape<0> {
     static constexpr int WM = 2, WN = 2, TM = 4, TN = 4;
+};
+template <>
+struct Shape<1> {
+    static constexpr int WM = 4, WN = 1, TM = 2, TN = 1;
+};
+template <>
+struct Shape<2> {
+    static constexpr int WM = 4, WN = 1, TM = 2, TN = 2;
+};
+template <>
+struct Shape<3> {
+    static constexpr int WM = 4, WN = 1, TM = 2, TN = 4;
 };
 
 template <bool SPLIT_OUT, int SHAPE>
@@ -797,6 +811,45 @@ struct OutMap {
     long R, SS, KS, CS, lo;
 };
 
+// One k_gemm3 launch of tile shape SHAPE (tbn columns per tile) over g's M x N.
+template <bool SPLIT, int SHAPE>
+int launch3(hipStream_t s, int kind, v3::G3 g, int tbn)
+{
+    g.tiles_m = (int)((g.M + v3::BM - 1) / v3::BM);
+    g.tiles_n = (g.N + tbn - 1) / tbn;
+    const long tiles = (long)g.tiles_m * g.tiles_n;
+    g.per_xcd = (tiles + kXcd - 1) / kXcd;
+    const long grid = g.per_xcd * kXcd;
+    if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
+    ProfScope prof(kind, s);
+    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, SHAPE>), dim3((unsigned)grid), dim3(v3::NT), 0, s, g);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+template <bool SPLIT>
+int launch3_cols(hipStream_t s, int kind, const v3::G3& g)
+{
+    // full 256-wide column tiles, then the remaining N % 256 columns (<= 128) with a
+    // narrow tile; a remainder above 128 keeps a 256-wide tile
+    const int rem = g.N % 256;
+    const int shape = rem == 0 ? 0 : rem <= 32 ? 1 : rem <= 64 ? 2 : rem <= 128 ? 3 : 0;
+    const int nmain = shape == 0 ? g.N : g.N - rem;
+    if (nmain > 0) {
+        v3::G3 m = g;
+        m.N = nmain;
+        FC3_TRY((launch3<SPLIT, 0>(s, kind, m, 256)));
+    }
+    if (shape == 0) return FC3_OK;
+    v3::G3 r = g;
+    r.N = rem;
+    r.B = g.B + 16L * nmain;   // column c of B at 16 c inside a K block
+    r.C = static_cast<char*>(g.C) + (SPLIT ? sizeof(__bf16) : sizeof(float)) * nmain * g.c_CS;
+    if (shape == 1) return launch3<SPLIT, 1>(s, kind, r, 32);
+    if (shape == 2) return launch3<SPLIT, 2>(s, kind, r, 64);
+    return launch3<SPLIT, 3>(s, kind, r, 128);
+}
+
 // C(m, c) = sum_k A(m, k) phi(k, c); A(m, k) = A[(m / a_R) a_SS + (k / 16) a_KS + 16 (m % a_R)
 // + k % 16], lo at + a_lo.
 int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_SS, long a_KS,
@@ -814,18 +867,7 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
     g.B = B.b; g.b_lo = B.lo; g.b_KS = B.ks;
     g.C = o.C; g.c_SS = o.SS; g.c_KS = o.KS; g.c_CS = o.CS; g.c_lo = o.lo; g.c_R = (int)o.R;
     g.M = (int)M; g.N = N; g.K = K;
-    g.tiles_m = (int)((M + v3::BM - 1) / v3::BM);
-    g.tiles_n = (N + v3::BN - 1) / v3::BN;
-    const long tiles = (long)g.tiles_m * g.tiles_n;
-    g.per_xcd = (tiles + kXcd - 1) / kXcd;
-    const long grid = g.per_xcd * kXcd;
-    if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
-    ProfScope prof(kind, s);
-    const dim3 gd((unsigned)grid), bd(v3::NT);
-    if (split_out) hipLaunchKernelGGL((v3::k_gemm3<true, 0>), gd, bd, 0, s, g);
-    else hipLaunchKernelGGL((v3::k_gemm3<false, 0>), gd, bd, 0, s, g);
-    FC3_HIP(hipGetLastError());
-    return FC3_OK;
+    return split_out ? launch3_cols<true>(s, kind, g) : launch3_cols<false>(s, kind, g);
 }
 
 // Launch sizing for the pre-split path.  A launch needs about 16k workgroups (64 per

@@ -303,8 +303,8 @@ __global__ void __launch_bounds__(NT, 1) k_gemm2(G2 g)
 // a 256-row tile is then 8 KB of contiguous memory, so the K loop is global -> LDS
 // copies (global_load_lds, 16 bytes per lane, each instruction one contiguous KB) with
 // no registers, VALU or LDS stores in between, then fragment reads and MFMAs.  Four
-// LDS stages in a ring; the copies for step k+3 are issued in step k into the stage
-// read in step k-1.  The epilogue writes C as FP32, or split again into bf16 hi/lo
+// LDS stages in a ring; step k multiplies fragments read during step k-1, reads
+// stage k+1, and refills its own stage with step k+4.  The epilogue writes C as FP32, or split again into bf16 hi/lo
 // planes in the K-blocked layout the next mode product reads.
 namespace v3 {
 
@@ -342,6 +342,7 @@ template <int N>
 __device__ __forceinline__ void ring_barrier()
 {
     static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_sched_barrier(0);   // nothing of the step is scheduled past the wait
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | (0x7 << 4));
     __builtin_amdgcn_s_barrier();
@@ -428,7 +429,8 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
     // lds_chunk) holds row r = 16 (P >> 5) + (P & 15), k-half c = (P >> 4) & 1.  Copy q
     // covers rows 32 q .. 32 q + 31, one contiguous KB of the K block (one slab permitting).
     constexpr int CMAX = CA > CB ? CA : CB;
-    const bool is_a = wave < 2;
+    // wave-uniform (an SGPR value), so that branches on it are scalar branches
+    const bool is_a = __builtin_amdgcn_readfirstlane(wave) < 2;
     const int ncopy = is_a ? CA : CB;
     const __bf16* src[CMAX];
 #pragma unroll
@@ -450,8 +452,8 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
     const int nk = g.K / BK;
     const int wofs = wave == 0 ? 0 : wave == 1 ? AE : wave == 2 ? 2 * AE : 2 * AE + BE;
     // this wave's copies of step k into stage array `st`; steps past the end copy the
-    // last step again (into a stage that is not read any more), so that every step has
-    // the same number of copies in flight
+    // last step again (into a stage whose fragments are read but not used), so that
+    // every step has the same number of copies in flight
     auto issue = [&](__bf16* st, int k) {
         const long kk = (k < nk ? k : nk - 1) * kstride;
 #pragma unroll
@@ -477,49 +479,89 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
     _Pragma("unroll") for (int i = 0; i < SH::TM; ++i)                                      \
         _Pragma("unroll") for (int j = 0; j < SH::TN; ++j)                                  \
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[i], Y[j], acc[i][j], 0, 0, 0);
-    // This wave's copies of the two newest steps may be in flight at a barrier.
-#define FC3_RING_BARRIER()                                                                  \
+#define FC3_RING_BARRIER(D)                                                                 \
     {                                                                                       \
-        if (is_a) ring_barrier<2 * CA>();                                                   \
-        else ring_barrier<2 * CB>();                                                        \
+        if constexpr (CA == CB) ring_barrier<(D) * CA>();                                   \
+        else if (is_a) ring_barrier<(D) * CA>();                                            \
+        else ring_barrier<(D) * CB>();                                                      \
     }
-    // Step K reads stage SRD and refills stage SWR (read in step K-1, retired by the
-    // barrier that ended it) with step K+3.  The closing barrier leaves the copies of
-    // steps K+2 and K+3 in flight and those of K+1 in LDS.
-#define FC3_STEP3(K, SRD, SWR)                                                              \
+    // Fragments of one stage (hi and lo of this wave's A rows and B columns), in two
+    // register sets: step k multiplies set k % 2 while set (k + 1) % 2 is read from the
+    // next stage, so no MFMA waits for LDS.
+    bf16x8 ah0[SH::TM], al0[SH::TM], bh0[SH::TN], bl0[SH::TN];
+    bf16x8 ah1[SH::TM], al1[SH::TM], bh1[SH::TN], bl1[SH::TN];
+#define FC3_READ(ST, AH, AL, BH, BL)                                                        \
     {                                                                                       \
-        bf16x8 ah[SH::TM], al[SH::TM], bh[SH::TN], bl[SH::TN];                              \
         _Pragma("unroll") for (int i = 0; i < SH::TM; ++i) {                                \
-            ah[i] = *reinterpret_cast<const bf16x8*>(SRD + offa[i]);                        \
-            al[i] = *reinterpret_cast<const bf16x8*>(SRD + AE + offa[i]);                   \
+            AH[i] = *reinterpret_cast<const bf16x8*>(ST + offa[i]);                         \
+            AL[i] = *reinterpret_cast<const bf16x8*>(ST + AE + offa[i]);                    \
         }                                                                                   \
         _Pragma("unroll") for (int j = 0; j < SH::TN; ++j) {                                \
-            bh[j] = *reinterpret_cast<const bf16x8*>(SRD + 2 * AE + offb[j]);               \
-            bl[j] = *reinterpret_cast<const bf16x8*>(SRD + 2 * AE + BE + offb[j]);          \
+            BH[j] = *reinterpret_cast<const bf16x8*>(ST + 2 * AE + offb[j]);                \
+            BL[j] = *reinterpret_cast<const bf16x8*>(ST + 2 * AE + BE + offb[j]);           \
         }                                                                                   \
-        FC3_MMA3(al, bh)                                                                    \
-        issue(SWR, (K) + 3);                                                                \
-        FC3_MMA3(ah, bl)                                                                    \
-        FC3_MMA3(ah, bh)                                                                    \
-        FC3_RING_BARRIER()                                                                  \
     }
+    // Scheduling hint for one step: the fragment reads go out two per MFMA from the top of
+    // the step (left alone, the scheduler sinks them to the barrier, where their latency
+    // is exposed), then the copies one per two MFMAs (when every wave issues the same
+    // number, so that they are not behind a branch), then the remaining MFMAs.
+#define FC3_ORDER()                                                                         \
+    {                                                                                       \
+        constexpr int NREAD = 2 * (SH::TM + SH::TN), NMMA = 3 * SH::TM * SH::TN;            \
+        constexpr int NCP = CA == CB ? CA : 0;                                              \
+        static_assert(NREAD / 2 + 2 * NCP <= NMMA, "step schedule");                        \
+        _Pragma("unroll") for (int r = 0; r < NREAD / 2; ++r) {                             \
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                              \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                              \
+        }                                                                                   \
+        _Pragma("unroll") for (int r = 0; r < NCP; ++r) {                                   \
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                              \
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);                              \
+        }                                                                                   \
+        __builtin_amdgcn_sched_group_barrier(0x008, NMMA - NREAD / 2 - 2 * NCP, 0);         \
+    }
+    // Step K (stage K % 4, fragments already in registers): read stage K + 1 into the
+    // other register set, multiply, and refill stage K % 4 (whose reads every wave
+    // finished before the barrier that ended step K - 1) with step K + 4.  The closing
+    // barrier waits for the fragment reads and leaves the copies of steps K + 3 and
+    // K + 4 in flight, so stage K + 2 has landed for the next step's reads.
+#define FC3_STEP4(K, SNX, SCUR, AH, AL, BH, BL, NAH, NAL, NBH, NBL)                         \
+    {                                                                                       \
+        FC3_READ(SNX, NAH, NAL, NBH, NBL)                                                   \
+        FC3_MMA3(AL, BH)                                                                    \
+        issue(SCUR, (K) + 4);                                                               \
+        FC3_MMA3(AH, BL)                                                                    \
+        FC3_MMA3(AH, BH)                                                                    \
+        FC3_ORDER()                                                                         \
+        FC3_RING_BARRIER(2)                                                                 \
+    }
+#define FC3_EVEN(K, SNX, SCUR) FC3_STEP4(K, SNX, SCUR, ah0, al0, bh0, bl0, ah1, al1, bh1, bl1)
+#define FC3_ODD(K, SNX, SCUR) FC3_STEP4(K, SNX, SCUR, ah1, al1, bh1, bl1, ah0, al0, bh0, bl0)
 
-    // prologue: steps 0..2 in flight, step 0 landed
+    // prologue: steps 0..3 in flight; once step 0 has landed its fragments are read, and
+    // the second barrier retires those reads (stage 0 may be refilled) with step 1 landed
     issue(s0, 0);
     issue(s1, 1);
     issue(s2, 2);
-    FC3_RING_BARRIER()
+    issue(s3, 3);
+    FC3_RING_BARRIER(3)
+    FC3_READ(s0, ah0, al0, bh0, bl0)
+    FC3_RING_BARRIER(2)
     int k = 0;
     for (; k + 4 <= nk; k += 4) {
-        FC3_STEP3(k, s0, s3)
-        FC3_STEP3(k + 1, s1, s0)
-        FC3_STEP3(k + 2, s2, s1)
-        FC3_STEP3(k + 3, s3, s2)
+        FC3_EVEN(k, s1, s0)
+        FC3_ODD(k + 1, s2, s1)
+        FC3_EVEN(k + 2, s3, s2)
+        FC3_ODD(k + 3, s0, s3)
     }
-    if (k < nk) FC3_STEP3(k, s0, s3)
-    if (k + 1 < nk) FC3_STEP3(k + 1, s1, s0)
-    if (k + 2 < nk) FC3_STEP3(k + 2, s2, s1)
-#undef FC3_STEP3
+    if (k < nk) FC3_EVEN(k, s1, s0)
+    if (k + 1 < nk) FC3_ODD(k + 1, s2, s1)
+    if (k + 2 < nk) FC3_EVEN(k + 2, s3, s2)
+#undef FC3_EVEN
+#undef FC3_ODD
+#undef FC3_STEP4
+#undef FC3_ORDER
+#undef FC3_READ
 #undef FC3_RING_BARRIER
 #undef FC3_MMA3
     wait_vmcnt0();   // no copy may still be landing when the workgroup's LDS is released

@@ -303,9 +303,9 @@ __global__ void __launch_bounds__(NT, 1) k_gemm2(G2 g)
 // a 256-row tile is then 8 KB of contiguous memory, so the K loop is global -> LDS
 // copies (global_load_lds, 16 bytes per lane, each instruction one contiguous KB) with
 // no registers, VALU or LDS stores in between, then fragment reads and MFMAs.  Four
-// LDS stages in a ring; step k multiplies fragments read during step k-1, reads
-// stage k+1, and refills its own stage with step k+4.  The epilogue writes C as FP32, or split again into bf16 hi/lo
-// planes in the K-blocked layout the next mode product reads.
+// LDS stages in a ring; step k multiplies fragments read during step k-1, reads stage
+// k+1, and refills its own stage with step k+4.  The epilogue writes C as FP32, or
+// split again into bf16 hi/lo planes in the K-blocked layout the next mode product reads.
 namespace v3 {
 
 using v2::BM;

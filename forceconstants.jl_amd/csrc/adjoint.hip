@@ -227,8 +227,23 @@ int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const floa
                                            T1, n, n2, n));
     FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, T_, n, n, n, &one, T1, n, n2, phi, n, 0, &zero,
                                            T2, n, n2, n));
-    // 7. dphi += Psi_(3)^T H12_(3)   (contraction over n^2)
-    FC3_BLAS(rocblas_sgemm(b, T_, N_, n, n, (int)n2, &one, psi, (int)n2, T2, (int)n2, &one, dphi, n));
+    // 7. dphi += Psi_(3)^T H12_(3)   (contraction over n^2).  As one GEMM this is only
+    //    (n/256)^2 output tiles over a K of n^2, which rocBLAS runs at a small fraction of
+    //    the MFMA rate; instead K is cut into S chunks whose n x n products go to slabs of
+    //    T1 (free after step 6), summed in chunk order.
+    {
+        // S + 1 slabs must fit T1's n slabs
+        const int S = n <= 2 ? 1 : (n - 1 < 64 ? n - 1 : 64);
+        const long L = n2 / S, rem = n2 - (long)S * L;
+        FC3_BLAS(rocblas_sgemm_strided_batched(b, T_, N_, n, n, (int)L, &one, psi, (int)n2, L, T2,
+                                               (int)n2, L, &zero, T1, n, n2, S));
+        if (rem)
+            FC3_BLAS(rocblas_sgemm(b, T_, N_, n, n, (int)rem, &one, psi + (long)S * L, (int)n2,
+                                   T2 + (long)S * L, (int)n2, &zero, T1 + (long)S * n2, n));
+        hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, T1, n2,
+                           S + (rem ? 1 : 0), dphi);
+        FC3_HIP(hipGetLastError());
+    }
     // 8. dpsi = H12 x3 phi
     if (dpsi)
         FC3_BLAS(rocblas_sgemm(b, N_, T_, (int)n2, n, n, &one, T2, (int)n2, phi, n, &zero, dpsi,

@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdint.h>
+#include <omp.h>
 
 #define D3 3
 
@@ -62,6 +63,9 @@ static double ipow(double x, int n)
 }
 
 double fco_ipow(double x, int n) { return ipow(x, n); }
+
+/* threads the threaded loops use (OpenMP's default team size) */
+int fco_num_threads(void) { return omp_get_max_threads(); }
 
 /* nearest_mirror! (helper_funcs.jl:7-18): one shift per component, strict bounds. */
 static void mirror(double r[3], const double L[3])
@@ -114,9 +118,10 @@ static double lj_phi3(double d1, double d2, double d3, double dist, const double
     return t1 + t2;
 }
 
-/* apply_tols!, helper_funcs.jl:20-28 */
+/* apply_tols!, helper_funcs.jl:20-28 (elementwise, so threads change nothing) */
 static void apply_tols(double* a, size_t len, double tol)
 {
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < len; ++i)
         if (fabs(a[i]) < tol) a[i] = 0.0;
 }
@@ -137,10 +142,13 @@ static void asr2(double* f, int N)
 }
 
 /* ASR!(::Array{T,3}), asr.jl:19-40: every (i,j,i) entry is zeroed and replaced by
- * -(sum over k of Psi[i a, j b, k c]) in ascending k.                             */
+ * -(sum over k of Psi[i a, j b, k c]) in ascending k.  Threaded over i like the
+ * reference's @tasks loop: atom i's iteration reads and writes only rows 3i..3i+2,
+ * and each sum keeps its ascending order, so the result does not depend on threads. */
 static void asr3(double* p, int N)
 {
     const long n = 3L * N, n2 = n * n;
+#pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < N; ++j)
             for (int a = 0; a < 3; ++a) {
@@ -193,16 +201,22 @@ int fco_lj_ifc2(const double* pos, int N, const double* L, double sigma, double 
 
 /* third_order! with the analytical LJ calculator (third_order_analytical.jl:4-69)
  * followed by ASR! and apply_tols! (ifc.jl:20-35).  The six permuted stores of
- * each (a,b,c) iteration are replayed in the reference's order.                */
+ * each (a,b,c) iteration are replayed in the reference's order.  The pair loop is
+ * threaded over i with dynamic scheduling, as the reference's @tasks loop
+ * (:14-15): pair (i<j) stores only into the permutations of the atom multisets
+ * {i,i,j} and {i,j,j}, which no other pair shares, so no two iterations store to
+ * one entry and the result does not depend on the thread count.                 */
 int fco_lj_ifc3(const double* pos, int N, const double* L, double sigma, double eps,
                 double pot_rcut, double rcut, double tol, double* out)
 {
     for (int c = 0; c < 3; ++c)
         if (!(pot_rcut < L[c])) return 2;
     const long n = 3L * N, n2 = n * n;
-    memset(out, 0, sizeof(double) * n2 * n);
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < n; ++k) memset(out + n2 * k, 0, sizeof(double) * n2);
     const double rc2 = rcut * rcut;
 #define P3(x, y, z) out[(x) + n * (y) + n2 * (z)]
+#pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < N; ++i)
         for (int j = i + 1; j < N; ++j) {
             for (int fam = 0; fam < 2; ++fam) {

@@ -75,6 +75,8 @@ def lib():
         L.fco_lj_ifc3_coo.restype = ctypes.c_long
         L.fco_ipow.argtypes = [ctypes.c_double, ctypes.c_int]
         L.fco_ipow.restype = ctypes.c_double
+        L.fco_num_threads.argtypes = []
+        L.fco_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 

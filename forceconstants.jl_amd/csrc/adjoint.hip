@@ -10,8 +10,9 @@
 //     T3[a,b] = sum_ij Psi[i,j,a] H12[i,j,b],  H12 = G x1 phi x2 phi
 //   (x_q phi maps index q of G to index i with weight phi[i, .]).  G x3 phi is shared by
 //   H23 and H13; H12 also gives dL/dPsi = H12 x3 phi.  18 n^4 flops, two n^3 buffers.
-//   The mode products and the n x n^2 x n contractions are plain FP32 GEMMs (rocBLAS,
-//   column-major like the tensors); dL/dPsi alone runs on the library's mcc3 kernels.
+//   The n x n^2 x n contractions are FP32 GEMMs (rocBLAS, column-major like the
+//   tensors).  The mode products run on the BF16x3 mcc3 kernels for BF16X3 (n % 4 == 0),
+//   else as FP32 GEMMs; dL/dPsi alone runs on the library's mcc3 kernels.
 //
 // LJ parameters (third_order_analytical.jl:4-69, asr.jl:19-40): every entry of Psi is
 // +-E_JK[t] of one pair block or a self block S_J = -sum_K E_JK, so dL/dtheta =
@@ -51,6 +52,23 @@ __global__ void k_transpose(const float* __restrict__ in, int n, float* __restri
     for (int y = threadIdx.y; y < 32; y += 8) {
         const int j = j0 + threadIdx.x, i = i0 + y;
         if (i < n && j < n) out[j + (long)n * i] = t[threadIdx.x][y];
+    }
+}
+
+// Slab-wise transpose of gridDim.z slabs: out[j + n i + n^2 z] = in[i + n j + n^2 z]
+__global__ void k_transpose_slabs(const float* __restrict__ in, int n, float* __restrict__ out)
+{
+    __shared__ float t[32][33];
+    const long z = (long)blockIdx.z * n * n;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    for (int y = threadIdx.y; y < 32; y += 8) {
+        const int i = i0 + threadIdx.x, j = j0 + y;
+        if (i < n && j < n) t[y][threadIdx.x] = in[z + i + (long)n * j];
+    }
+    __syncthreads();
+    for (int y = threadIdx.y; y < 32; y += 8) {
+        const int j = j0 + threadIdx.x, i = i0 + y;
+        if (i < n && j < n) out[z + j + (long)n * i] = t[threadIdx.x][y];
     }
 }
 
@@ -178,7 +196,12 @@ static int get_blas(fc3_context* h, rocblas_handle* out)
 size_t mcc3_backward_workspace_bytes(int n, int precision, bool need_dphi)
 {
     const size_t n3 = (size_t)n * n * n;
-    if (need_dphi) return 2 * Carve::need(sizeof(float) * n3) + 4096;
+    if (need_dphi) {
+        size_t b = 2 * Carve::need(sizeof(float) * n3) + 4096;
+        if (precision == FC3_PREC_BF16X3 && mcc3_adjoint_v3_applies(n, nullptr, nullptr, nullptr))
+            b += Carve::need(sizeof(float) * (size_t)n * n) + mcc3_adjoint_v3_bytes(n);
+        return b;
+    }
     return Carve::need(sizeof(float) * (size_t)n * n) + mcc3_workspace_bytes(n, precision, false) +
            4096;
 }
@@ -206,27 +229,61 @@ int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const floa
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 backward workspace");
     const float one = 1.0f, zero = 0.0f;
     const rocblas_operation N_ = rocblas_operation_none, T_ = rocblas_operation_transpose;
-    // 1. T1 = G x3 phi:  (n^2 x n) = G (n^2 x n) * phi^T
-    FC3_BLAS(rocblas_sgemm(b, N_, T_, (int)n2, n, n, &one, G, (int)n2, phi, n, &zero, T1, (int)n2));
-    // 2. T2 = H23, slab k: T1_k * phi^T
-    FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, T_, n, n, n, &one, T1, n, n2, phi, n, 0, &zero,
-                                           T2, n, n2, n));
-    // 3. dphi = Psi_(1) * H23_(1)^T   (contraction over n^2)
-    FC3_BLAS(rocblas_sgemm(b, N_, T_, n, n, (int)n2, &one, psi, n, T2, n, &zero, dphi, n));
-    // 4. T2 = H13, slab k: phi * T1_k
-    FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, N_, n, n, n, &one, phi, n, 0, T1, n, n2, &zero,
-                                           T2, n, n2, n));
-    // 5. dphi += sum_k Psi_k^T H13_k  (per-slab products into T1, then a slab sum)
-    FC3_BLAS(rocblas_sgemm_strided_batched(b, T_, N_, n, n, n, &one, psi, n, n2, T2, n, n2, &zero,
-                                           T1, n, n2, n));
-    hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, T1, n2, n,
-                       dphi);
-    FC3_HIP(hipGetLastError());
-    // 6. T2 = H12, slab l: (phi * G_l) * phi^T
-    FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, N_, n, n, n, &one, phi, n, 0, G, n, n2, &zero,
-                                           T1, n, n2, n));
-    FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, T_, n, n, n, &one, T1, n, n2, phi, n, 0, &zero,
-                                           T2, n, n2, n));
+    const bool v3 = precision == FC3_PREC_BF16X3 && mcc3_adjoint_v3_applies(n, psi, G, dpsi ? dpsi : G);
+    float* phiT = nullptr;
+    char* rest = nullptr;
+    size_t rest_bytes = 0;
+    if (v3) {
+        // The mode products on the BF16x3 kernels (M = phi^T maps an index back); the
+        // three n x n^2 x n contractions stay FP32 GEMMs.
+        phiT = cv.take<float>((size_t)n2);
+        rest = cv.take<char>(0);
+        if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 backward workspace");
+        rest_bytes = cv.cap - cv.off;
+        hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, s, phi,
+                           n, phiT);
+        FC3_HIP(hipGetLastError());
+        // H23 = G x2 phi x3 phi: modes31_t of the slab-transposed G (G's first two
+        // indices swapped) writes H23 in its natural layout
+        hipLaunchKernelGGL(k_transpose_slabs, dim3((n + 31) / 32, (n + 31) / 32, n), dim3(32, 8), 0,
+                           s, G, n, T1);
+        FC3_HIP(hipGetLastError());
+        FC3_TRY(mcc3_modes31_t(s, T1, phiT, n, T2, rest, rest_bytes));
+        // dphi = Psi_(1) * H23_(1)^T
+        FC3_BLAS(rocblas_sgemm(b, N_, T_, n, n, (int)n2, &one, psi, n, T2, n, &zero, dphi, n));
+        // Y = H13 with each slab transposed (Y_k = H13_k^T); dphi += sum_k Psi_k^T Y_k^T
+        FC3_TRY(mcc3_modes31_t(s, G, phiT, n, T2, rest, rest_bytes));
+        FC3_BLAS(rocblas_sgemm_strided_batched(b, T_, T_, n, n, n, &one, psi, n, n2, T2, n, n2,
+                                               &zero, T1, n, n2, n));
+        hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, T1,
+                           n2, n, dphi);
+        FC3_HIP(hipGetLastError());
+        // T2 = H12 = G x1 phi x2 phi
+        FC3_TRY(mcc3_modes12(s, G, phiT, n, T2, rest, rest_bytes));
+    } else {
+        // 1. T1 = G x3 phi:  (n^2 x n) = G (n^2 x n) * phi^T
+        FC3_BLAS(rocblas_sgemm(b, N_, T_, (int)n2, n, n, &one, G, (int)n2, phi, n, &zero, T1,
+                               (int)n2));
+        // 2. T2 = H23, slab k: T1_k * phi^T
+        FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, T_, n, n, n, &one, T1, n, n2, phi, n, 0,
+                                               &zero, T2, n, n2, n));
+        // 3. dphi = Psi_(1) * H23_(1)^T   (contraction over n^2)
+        FC3_BLAS(rocblas_sgemm(b, N_, T_, n, n, (int)n2, &one, psi, n, T2, n, &zero, dphi, n));
+        // 4. T2 = H13, slab k: phi * T1_k
+        FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, N_, n, n, n, &one, phi, n, 0, T1, n, n2,
+                                               &zero, T2, n, n2, n));
+        // 5. dphi += sum_k Psi_k^T H13_k  (per-slab products into T1, then a slab sum)
+        FC3_BLAS(rocblas_sgemm_strided_batched(b, T_, N_, n, n, n, &one, psi, n, n2, T2, n, n2,
+                                               &zero, T1, n, n2, n));
+        hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, T1,
+                           n2, n, dphi);
+        FC3_HIP(hipGetLastError());
+        // 6. T2 = H12, slab l: (phi * G_l) * phi^T
+        FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, N_, n, n, n, &one, phi, n, 0, G, n, n2,
+                                               &zero, T1, n, n2, n));
+        FC3_BLAS(rocblas_sgemm_strided_batched(b, N_, T_, n, n, n, &one, T1, n, n2, phi, n, 0,
+                                               &zero, T2, n, n2, n));
+    }
     // 7. dphi += Psi_(3)^T H12_(3)   (contraction over n^2).  As one GEMM this is only
     //    (n/256)^2 output tiles over a K of n^2, which rocBLAS runs at a small fraction of
     //    the MFMA rate; instead K is cut into S chunks whose n x n products go to slabs of
@@ -245,6 +302,7 @@ int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const floa
         FC3_HIP(hipGetLastError());
     }
     // 8. dpsi = H12 x3 phi
+    if (dpsi && v3) return mcc3_mode3(s, T2, phiT, n, dpsi, rest, rest_bytes);
     if (dpsi)
         FC3_BLAS(rocblas_sgemm(b, N_, T_, (int)n2, n, n, &one, T2, (int)n2, phi, n, &zero, dpsi,
                                (int)n2));

@@ -155,6 +155,21 @@ int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int preci
              float* k3, void* ws, size_t ws_bytes, const Mcc3Config* cfg = nullptr);
 int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int precision,
                      void* ws, size_t ws_bytes);
+// Mode products of the mcc3 adjoint on the BF16X3 kernels, out of place, FP32 in and out.
+// M is n x n column-major, M(k, c) = M[k + n c]; X, Y are n^3 (Y must not alias X).
+//   mode3:     Y[i + n j + n^2 c] = sum_k X[i + n j + n^2 k] M(k, c)
+//   modes31_t: Y[j + n b + n^2 c] = sum_{i,k} X[i + n j + n^2 k] M(i, b) M(k, c)
+//              (the (b, j) slab is stored transposed)
+//   modes12:   Y[b + n c + n^2 l] = sum_{i,j} X[i + n j + n^2 l] M(i, b) M(j, c)
+// They apply when mcc3_adjoint_v3_applies (n % 4 == 0, n >= 16, 16-byte aligned tensors).
+bool mcc3_adjoint_v3_applies(int n, const void* a, const void* b, const void* c);
+size_t mcc3_adjoint_v3_bytes(int n);
+int mcc3_mode3(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
+               size_t ws_bytes);
+int mcc3_modes31_t(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
+                   size_t ws_bytes);
+int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
+                 size_t ws_bytes);
 
 // ---- adjoints (adjoint.hip) ---------------------------------------------------------
 void release_blas(fc3_context* h);

@@ -1052,7 +1052,107 @@ int mcc3_v3_inplace(hipStream_t s, float* psi, const float* phi, int n, void* ws
     return FC3_OK;
 }
 
+// Scratch of the adjoint's mode products: the split M, the mode-1 operand U of
+// modes31_t (n slabs), mode-3 row blocks, and two slab batches for modes12.
+struct AdjScratch {
+    long P, R;
+    int nb;
+};
+AdjScratch adj_scratch(int n)
+{
+    AdjScratch a;
+    a.P = (long)n * padk(n);
+    a.nb = slab_batch3(n, kScratchSlabsInplace, kTilesPerLaunch);
+    a.R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
+    return a;
+}
+
 }  // namespace
+
+bool mcc3_adjoint_v3_applies(int n, const void* a, const void* b, const void* c)
+{
+    return v3_applies(n, FC3_PREC_BF16X3) && aligned16(a) && aligned16(b) && aligned16(c);
+}
+
+size_t mcc3_adjoint_v3_bytes(int n)
+{
+    const AdjScratch a = adj_scratch(n);
+    return phi3_bytes(n) + Carve::need(2 * sizeof(__bf16) * (size_t)a.P * n) +
+           Carve::need(2 * sizeof(__bf16) * (size_t)a.R * padk(n)) +
+           2 * Carve::need(2 * sizeof(__bf16) * (size_t)a.P * a.nb) + 4096;
+}
+
+int mcc3_mode3(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
+               size_t ws_bytes)
+{
+    Carve cv(ws, ws_bytes);
+    Split3 B;
+    FC3_TRY(prep_phi3(s, M, n, cv, B));
+    const AdjScratch a = adj_scratch(n);
+    const long n2 = (long)n * n;
+    __bf16* Xs = cv.take<__bf16>((size_t)2 * a.R * padk(n));
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 adjoint workspace too small");
+    for (long r0 = 0; r0 < n2; r0 += a.R) {
+        const long rows = (n2 - r0) < a.R ? (n2 - r0) : a.R;
+        const OutMap oY{Y + r0, rows, 0, 16, n2, 0};
+        FC3_TRY(mode3_block(s, X, n, r0, rows, B, Xs, oY, false));
+    }
+    return FC3_OK;
+}
+
+int mcc3_modes31_t(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
+                   size_t ws_bytes)
+{
+    Carve cv(ws, ws_bytes);
+    Split3 B;
+    FC3_TRY(prep_phi3(s, M, n, cv, B));
+    const AdjScratch a = adj_scratch(n);
+    const long n2 = (long)n * n, P = a.P;
+    __bf16* U = cv.take<__bf16>((size_t)2 * P * n);
+    __bf16* Xs = cv.take<__bf16>((size_t)2 * a.R * padk(n));
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 adjoint workspace too small");
+    FC3_TRY(zero_kpad(s, U, n, n));
+    // mode 3 into U, the mode-1 operand: A(c n + j, i) = U_c(i, j) (as in the forward)
+    for (long r0 = 0; r0 < n2; r0 += a.R) {
+        const long rows = (n2 - r0) < a.R ? (n2 - r0) : a.R;
+        const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * P, P};
+        FC3_TRY(mode3_block(s, X, n, r0, rows, B, Xs, oU, true));
+    }
+    // mode 1: C(c n + j, b) = Y[j + n b + n^2 c]
+    const int nb = a.nb;
+    for (int l0 = 0; l0 < n; l0 += nb) {
+        const int cnt = (n - l0) < nb ? (n - l0) : nb;
+        const OutMap oY{Y + n2 * l0, n, n2, 16, n, 0};
+        FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U + 2 * P * l0, P, n, 2 * P, 16L * n, B, oY,
+                      false, (long)cnt * n, n, padk(n)));
+    }
+    return FC3_OK;
+}
+
+int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
+                 size_t ws_bytes)
+{
+    Carve cv(ws, ws_bytes);
+    Split3 B;
+    FC3_TRY(prep_phi3(s, M, n, cv, B));
+    const AdjScratch a = adj_scratch(n);
+    const long n2 = (long)n * n, P = a.P;
+    const int nb = a.nb;
+    __bf16* Ys = cv.take<__bf16>((size_t)2 * P * nb);
+    __bf16* S = cv.take<__bf16>((size_t)2 * P * nb);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 adjoint workspace too small");
+    FC3_TRY(zero_kpad(s, Ys, n, nb));
+    FC3_TRY(zero_kpad(s, S, n, nb));
+    for (int l0 = 0; l0 < n; l0 += nb) {
+        const int cnt = (n - l0) < nb ? (n - l0) : nb;
+        const long q = n2 / 4 * cnt;
+        hipLaunchKernelGGL(v3::k_split_slabs, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
+                           X + n2 * l0, n, padk(n), (long)cnt, Ys);
+        FC3_HIP(hipGetLastError());
+        FC3_TRY(modes12_v3(s, Ys, n, cnt, B, S, Y + n2 * l0));
+    }
+    return FC3_OK;
+}
 
 size_t mcc3_workspace_bytes(int n, int precision, bool inplace, const Mcc3Config* cfg)
 {

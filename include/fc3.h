@@ -195,8 +195,12 @@ int fc3_autotune_load(fc3_handle h, const char* path);
 /* Reverse mode of mcc3 (src/mcc.jl:15-17) for a cotangent dk3 = dL/dK3 (device n^3):
  *   dpsi = dL/dPsi = dk3 x1 phi x2 phi x3 phi            (device n^3, or NULL)
  *   dphi = dL/dphi, the three mode terms                  (device n x n, or NULL; needs psi)
- * psi, phi, dk3 device FP32.  With dphi the work is FP32 (18 n^4 flops, two n^3
- * buffers of workspace); dpsi alone runs the mcc3 kernels at `precision`.          */
+ * psi, phi, dk3 device FP32.  With dphi the work is 20 n^4 flops (18 without dpsi) in
+ * two n^3 FP32 buffers of workspace: for FC3_PREC_BF16X3 with n % 4 == 0, n >= 16 and
+ * 16-byte aligned tensors the seven mode products run on the BF16x3 kernels (plus
+ * about one n^3 of split scratch) and the three n x n^2 x n contractions as FP32
+ * GEMMs; otherwise all nine are FP32 GEMMs.  dpsi alone runs the mcc3 kernels at
+ * `precision`.                                                                       */
 int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const float* dk3,
                       int n, int precision, float* dpsi, float* dphi);
 

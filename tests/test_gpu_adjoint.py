@@ -59,14 +59,16 @@ def test_mcc3_backward_dpsi_adjoint_identity(ctx, n):
         assert rel <= (2e-6 if prec == "fp32" else 1e-5)
 
 
-@pytest.mark.parametrize("n", [48, 96])
-def test_mcc3_backward_dphi_central_difference(ctx, n):
+@pytest.mark.parametrize("n,prec", [(48, "fp32"), (96, "fp32"), (48, "bf16x3"), (96, "bf16x3"),
+                                    (100, "bf16x3")])
+def test_mcc3_backward_dphi_central_difference(ctx, n, prec):
     psi = _rand((n, n, n), 11, np.float32)
     phi = _rand((n, n), 12, np.float32) / np.sqrt(n)
     phi = np.asfortranarray(phi.astype(np.float32))
     G = _rand((n, n, n), 13, np.float32)
     d = fc3.DeviceArray.from_host
-    _, dphi = fc3.mcc3_backward(d(psi, ctx=ctx), d(phi, ctx=ctx), d(G, ctx=ctx), dpsi=False)
+    _, dphi = fc3.mcc3_backward(d(psi, ctx=ctx), d(phi, ctx=ctx), d(G, ctx=ctx), dpsi=False,
+                                precision=prec)
     gf = dphi.to_host().astype(np.float64)
     p64, f64, g64 = (a.astype(np.float64) for a in (psi, phi, G))
     h = 1e-4
@@ -74,8 +76,26 @@ def test_mcc3_backward_dphi_central_difference(ctx, n):
         D = _rand((n, n), seed)
         fd = (_dot(g64, O.mcc3_fp64(p64, f64 + h * D)) - _dot(g64, O.mcc3_fp64(p64, f64 - h * D))) / (2 * h)
         ad = _dot(gf, D)
-        print(f"n={n} dir {seed}: FD {fd:.9e} adjoint {ad:.9e} rel {abs(fd - ad) / abs(fd):.2e}", flush=True)
+        print(f"n={n} {prec} dir {seed}: FD {fd:.9e} adjoint {ad:.9e} rel {abs(fd - ad) / abs(fd):.2e}", flush=True)
         assert abs(fd - ad) <= 2e-5 * np.linalg.norm(gf) * np.linalg.norm(D)
+
+
+@pytest.mark.parametrize("n", [48, 100])
+def test_mcc3_backward_bf16x3_matches_fp32(ctx, n):
+    """The BF16x3 pullback (mode products on the BF16x3 kernels) against the FP32 one,
+    both outputs, relative Frobenius error <= 1e-5 (the BF16x3 forward's bar)."""
+    psi = _rand((n, n, n), 31, np.float32)
+    phi, _ = np.linalg.qr(np.random.default_rng(32).standard_normal((n, n)))
+    phi = np.asfortranarray(phi.astype(np.float32))
+    G = _rand((n, n, n), 33, np.float32)
+    d = fc3.DeviceArray.from_host
+    args = (d(psi, ctx=ctx), d(phi, ctx=ctx), d(G, ctx=ctx))
+    p32, f32 = (a.to_host().astype(np.float64) for a in fc3.mcc3_backward(*args, precision="fp32"))
+    pbf, fbf = (a.to_host().astype(np.float64) for a in fc3.mcc3_backward(*args, precision="bf16x3"))
+    rp = np.linalg.norm((pbf - p32).ravel()) / np.linalg.norm(p32.ravel())
+    rf = np.linalg.norm((fbf - f32).ravel()) / np.linalg.norm(f32.ravel())
+    print(f"n={n} bf16x3 vs fp32: dPsi rel {rp:.2e}, dphi rel {rf:.2e}", flush=True)
+    assert rp <= 1e-5 and rf <= 1e-5
 
 
 def _lj_sys(cells, su, seed):

@@ -256,8 +256,8 @@ def adjoint_timing(ctx, cells=5, reps=2):
            "forward_backward_ms": t_fb, "ratio": t_fb / t_fwd,
            "dL_dsigma": g["sigma"], "dL_depsilon": g["epsilon"],
            "dL_depsilon_vs_exact_rel": abs(g["epsilon"] - eps_check) / abs(eps_check),
-           "note": "backward = mcc3 pullback in BF16X3 (the seven mode products on the BF16x3 "
-                   "kernels, the three n x n^2 x n contractions as FP32 GEMMs) + the "
+           "note": "backward = mcc3 pullback in BF16X3 (the seven mode products and the "
+                   "three n x n^2 x n contractions, split K, on the BF16x3 kernels) + the "
                    "sigma/epsilon pullback"}
     log(f"adjoint: {res}")
     return res

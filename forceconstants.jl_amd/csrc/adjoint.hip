@@ -10,9 +10,10 @@
 //     T3[a,b] = sum_ij Psi[i,j,a] H12[i,j,b],  H12 = G x1 phi x2 phi
 //   (x_q phi maps index q of G to index i with weight phi[i, .]).  G x3 phi is shared by
 //   H23 and H13; H12 also gives dL/dPsi = H12 x3 phi.  18 n^4 flops, two n^3 buffers.
-//   The n x n^2 x n contractions are FP32 GEMMs (rocBLAS, column-major like the
-//   tensors).  The mode products run on the BF16x3 mcc3 kernels for BF16X3 (n % 4 == 0),
-//   else as FP32 GEMMs; dL/dPsi alone runs on the library's mcc3 kernels.
+//   For BF16X3 (n % 4 == 0) the mode products and the n x n^2 x n contractions run on
+//   the BF16x3 mcc3 kernel (the contractions with split K); otherwise they are FP32
+//   GEMMs (rocBLAS, column-major like the tensors).  dL/dPsi alone runs on the
+//   library's mcc3 kernels.
 //
 // LJ parameters (third_order_analytical.jl:4-69, asr.jl:19-40): every entry of Psi is
 // +-E_JK[t] of one pair block or a self block S_J = -sum_K E_JK, so dL/dtheta =
@@ -72,14 +73,15 @@ __global__ void k_transpose_slabs(const float* __restrict__ in, int n, float* __
     }
 }
 
-// out[e] += sum_k T[e + n2 k], k ascending, for e < n2
-__global__ void k_sum_slabs(const float* __restrict__ T, long n2, int cnt, float* __restrict__ out)
+// out[e] (+)= sum_k T[e + n2 k], k ascending, for e < n2 (accumulate or overwrite)
+__global__ void k_sum_slabs(const float* __restrict__ T, long n2, int cnt, float* __restrict__ out,
+                            int accumulate = 1)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n2) return;
     float acc = 0.0f;
     for (int k = 0; k < cnt; ++k) acc += T[e + n2 * k];
-    out[e] += acc;
+    out[e] = accumulate ? out[e] + acc : acc;
 }
 
 // One thread per (atom J, neighbour slot s): accumulates sum_t dE_JK[t]/dsigma * W[t]
@@ -199,7 +201,9 @@ size_t mcc3_backward_workspace_bytes(int n, int precision, bool need_dphi)
     if (need_dphi) {
         size_t b = 2 * Carve::need(sizeof(float) * n3) + 4096;
         if (precision == FC3_PREC_BF16X3 && mcc3_adjoint_v3_applies(n, nullptr, nullptr, nullptr))
-            b += Carve::need(sizeof(float) * (size_t)n * n) + mcc3_adjoint_v3_bytes(n);
+            b += Carve::need(sizeof(float) * (size_t)n * n) +
+                 std::max(mcc3_adjoint_v3_bytes(n),
+                          mcc3_contract_applies(n) ? mcc3_contract_bytes(n) : size_t(0));
         return b;
     }
     return Carve::need(sizeof(float) * (size_t)n * n) + mcc3_workspace_bytes(n, precision, false) +
@@ -230,36 +234,58 @@ int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const floa
     const float one = 1.0f, zero = 0.0f;
     const rocblas_operation N_ = rocblas_operation_none, T_ = rocblas_operation_transpose;
     const bool v3 = precision == FC3_PREC_BF16X3 && mcc3_adjoint_v3_applies(n, psi, G, dpsi ? dpsi : G);
+    const bool cv3 = v3 && mcc3_contract_applies(n);
     float* phiT = nullptr;
     char* rest = nullptr;
     size_t rest_bytes = 0;
+    const unsigned sum_grid = (unsigned)((n2 + 255) / 256);
     if (v3) {
         // The mode products on the BF16x3 kernels (M = phi^T maps an index back); the
-        // three n x n^2 x n contractions stay FP32 GEMMs.
+        // three n x n^2 x n contractions with Psi on the BF16x3 kernel with split K, or
+        // as FP32 GEMMs where that does not apply.
         phiT = cv.take<float>((size_t)n2);
         rest = cv.take<char>(0);
         if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 backward workspace");
         rest_bytes = cv.cap - cv.off;
+        const dim3 tgrid((n + 31) / 32, (n + 31) / 32, n);
         hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, s, phi,
                            n, phiT);
         FC3_HIP(hipGetLastError());
+        int zc = 0;
         // H23 = G x2 phi x3 phi: modes31_t of the slab-transposed G (G's first two
         // indices swapped) writes H23 in its natural layout
-        hipLaunchKernelGGL(k_transpose_slabs, dim3((n + 31) / 32, (n + 31) / 32, n), dim3(32, 8), 0,
-                           s, G, n, T1);
+        hipLaunchKernelGGL(k_transpose_slabs, tgrid, dim3(32, 8), 0, s, G, n, T1);
         FC3_HIP(hipGetLastError());
         FC3_TRY(mcc3_modes31_t(s, T1, phiT, n, T2, rest, rest_bytes));
         // dphi = Psi_(1) * H23_(1)^T
-        FC3_BLAS(rocblas_sgemm(b, N_, T_, n, n, (int)n2, &one, psi, n, T2, n, &zero, dphi, n));
-        // Y = H13 with each slab transposed (Y_k = H13_k^T); dphi += sum_k Psi_k^T Y_k^T
+        if (cv3) {
+            FC3_TRY(mcc3_contract(s, 0, psi, T2, n, T1, &zc, rest, rest_bytes));
+            hipLaunchKernelGGL(k_sum_slabs, dim3(sum_grid), dim3(256), 0, s, T1, n2, zc, dphi, 0);
+            FC3_HIP(hipGetLastError());
+        } else {
+            FC3_BLAS(rocblas_sgemm(b, N_, T_, n, n, (int)n2, &one, psi, n, T2, n, &zero, dphi, n));
+        }
+        // Y = H13 with its first two indices swapped (Y_k = H13_k^T):
+        // dphi += sum_k Psi_k^T Y_k^T, i.e. the first form with both slab-transposed
         FC3_TRY(mcc3_modes31_t(s, G, phiT, n, T2, rest, rest_bytes));
-        FC3_BLAS(rocblas_sgemm_strided_batched(b, T_, T_, n, n, n, &one, psi, n, n2, T2, n, n2,
-                                               &zero, T1, n, n2, n));
-        hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, T1,
-                           n2, n, dphi);
+        if (cv3) {
+            hipLaunchKernelGGL(k_transpose_slabs, tgrid, dim3(32, 8), 0, s, psi, n, T1);
+            FC3_HIP(hipGetLastError());
+            FC3_TRY(mcc3_contract(s, 0, T1, T2, n, T1, &zc, rest, rest_bytes));
+        } else {
+            FC3_BLAS(rocblas_sgemm_strided_batched(b, T_, T_, n, n, n, &one, psi, n, n2, T2, n,
+                                                   n2, &zero, T1, n, n2, n));
+            zc = n;
+        }
+        hipLaunchKernelGGL(k_sum_slabs, dim3(sum_grid), dim3(256), 0, s, T1, n2, zc, dphi, 1);
         FC3_HIP(hipGetLastError());
-        // T2 = H12 = G x1 phi x2 phi
+        // T2 = H12 = G x1 phi x2 phi; dphi += Psi_(3)^T H12_(3)
         FC3_TRY(mcc3_modes12(s, G, phiT, n, T2, rest, rest_bytes));
+        if (cv3) {
+            FC3_TRY(mcc3_contract(s, 1, psi, T2, n, T1, &zc, rest, rest_bytes));
+            hipLaunchKernelGGL(k_sum_slabs, dim3(sum_grid), dim3(256), 0, s, T1, n2, zc, dphi, 1);
+            FC3_HIP(hipGetLastError());
+        }
     } else {
         // 1. T1 = G x3 phi:  (n^2 x n) = G (n^2 x n) * phi^T
         FC3_BLAS(rocblas_sgemm(b, N_, T_, (int)n2, n, n, &one, G, (int)n2, phi, n, &zero, T1,
@@ -288,7 +314,7 @@ int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const floa
     //    (n/256)^2 output tiles over a K of n^2, which rocBLAS runs at a small fraction of
     //    the MFMA rate; instead K is cut into S chunks whose n x n products go to slabs of
     //    T1 (free after step 6), summed in chunk order.
-    {
+    if (!cv3) {
         // S + 1 slabs must fit T1's n slabs
         const int S = n <= 2 ? 1 : (n - 1 < 64 ? n - 1 : 64);
         const long L = n2 / S, rem = n2 - (long)S * L;

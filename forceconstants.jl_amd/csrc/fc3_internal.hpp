@@ -170,6 +170,16 @@ int mcc3_modes31_t(hipStream_t s, const float* X, const float* M, int n, float* 
                    size_t ws_bytes);
 int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
                  size_t ws_bytes);
+// The adjoint's n x n^2 x n contractions on the BF16X3 kernel with split K: partials
+// P_z[a + n b] = sum over chunk z of e < n^2 of A(a, e) B(e, b) at P + z n^2,
+// z < *zc <= min(64, n)
+// (the caller sums them).  form 0: A(a, e) = X[a + n e], B(e, b) = H[b + n e];
+// form 1: A(a, e) = X[e + n^2 a], B(e, b) = H[e + n^2 b].  Applies for n % 4 == 0,
+// 16 <= n and n^2 / 64 < 65535 (aligned tensors).
+bool mcc3_contract_applies(int n);
+size_t mcc3_contract_bytes(int n);
+int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n, float* P,
+                  int* zc, void* ws, size_t ws_bytes);
 
 // ---- adjoints (adjoint.hip) ---------------------------------------------------------
 void release_blas(fc3_context* h);

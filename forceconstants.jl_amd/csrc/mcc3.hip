@@ -331,6 +331,10 @@ struct G3 {
     int M, N, K;         // K % BK == 0, M % 4 == 0, c_R % 4 == 0
     int tiles_m, tiles_n;
     long per_xcd;
+    // split K: workgroups with blockIdx.y = z take K blocks [z kc, min((z + 1) kc, K / BK))
+    // and write C + z c_ZS (kc = K / BK and gridDim.y = 1 for an unsplit product)
+    int kc;
+    long c_ZS;
 };
 
 // Workgroup barrier for the copy ring: s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding,
@@ -449,7 +453,10 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
         }
     }
     const long kstride = is_a ? g.a_KS : g.b_KS;   // elements per K block
-    const int nk = g.K / BK;
+    const int kz = blockIdx.y, kbeg = kz * g.kc;
+    const int nk = min(g.kc, g.K / BK - kbeg);    // >= 1: the host launches no empty chunk
+#pragma unroll
+    for (int q = 0; q < CMAX; ++q) src[q] += kbeg * kstride;
     const int wofs = wave == 0 ? 0 : wave == 1 ? AE : wave == 2 ? 2 * AE : 2 * AE + BE;
     // this wave's copies of step k into stage array `st`; steps past the end copy the
     // last step again (into a stage whose fragments are read but not used), so that
@@ -584,7 +591,7 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
                 const float x0 = acc[i][j][4 * q], x1 = acc[i][j][4 * q + 1];
                 const float x2 = acc[i][j][4 * q + 2], x3 = acc[i][j][4 * q + 3];
                 if constexpr (SPLIT_OUT) {
-                    __bf16* C = static_cast<__bf16*>(g.C);
+                    __bf16* C = static_cast<__bf16*>(g.C) + kz * g.c_ZS;
                     bf16x4 h, l;
                     __bf16 hh, ll;
                     split_bf16(x0, hh, ll); h[0] = hh; l[0] = ll;
@@ -594,7 +601,7 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
                     *reinterpret_cast<bf16x4*>(C + off) = h;
                     *reinterpret_cast<bf16x4*>(C + off + g.c_lo) = l;
                 } else {
-                    float* C = static_cast<float*>(g.C);
+                    float* C = static_cast<float*>(g.C) + kz * g.c_ZS;
                     *reinterpret_cast<float4*>(C + off) = make_float4(x0, x1, x2, x3);
                 }
             }
@@ -666,6 +673,27 @@ __global__ void __launch_bounds__(256) k_split_slabs(const float* __restrict__ s
     const long off = l * 2 * P + (long)(i >> 4) * 16 * n + 16L * j + (i & 15);
     *reinterpret_cast<bf16x4*>(Y + off) = h;
     *reinterpret_cast<bf16x4*>(Y + off + P) = lo;
+}
+
+// rows x K matrix with K contiguous (src[e + K r]) -> the K-blocked operand
+// Y[(e / 16) 16 rows + 16 r + e % 16] (hi), lo at + rows K.  K % 16 == 0.
+__global__ void __launch_bounds__(256) k_split_rows(const float* __restrict__ src, long K,
+                                                     long rows, __bf16* __restrict__ Y)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // group of 4 values
+    const long per = K / 4;
+    if (q >= per * rows) return;
+    const long r = q / per, e = 4 * (q - r * per);
+    const float4 x = *reinterpret_cast<const float4*>(src + e + K * r);
+    bf16x4 h, lo;
+    __bf16 a, b;
+    split_bf16(x.x, a, b); h[0] = a; lo[0] = b;
+    split_bf16(x.y, a, b); h[1] = a; lo[1] = b;
+    split_bf16(x.z, a, b); h[2] = a; lo[2] = b;
+    split_bf16(x.w, a, b); h[3] = a; lo[3] = b;
+    const long off = (e >> 4) * 16 * rows + 16 * r + (e & 15);
+    *reinterpret_cast<bf16x4*>(Y + off) = h;
+    *reinterpret_cast<bf16x4*>(Y + off + rows * K) = lo;
 }
 
 // Zeroes the last K block (block_elems bf16 at block_off) of both planes of `slabs` slabs
@@ -863,8 +891,10 @@ int launch3(hipStream_t s, int kind, v3::G3 g, int tbn)
     g.per_xcd = (tiles + kXcd - 1) / kXcd;
     const long grid = g.per_xcd * kXcd;
     if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
+    const int nkb = g.K / v3::BK, zc = (nkb + g.kc - 1) / g.kc;
     ProfScope prof(kind, s);
-    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, SHAPE>), dim3((unsigned)grid), dim3(v3::NT), 0, s, g);
+    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, SHAPE>), dim3((unsigned)grid, (unsigned)zc), dim3(v3::NT),
+                       0, s, g);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
@@ -895,7 +925,8 @@ int launch3_cols(hipStream_t s, int kind, const v3::G3& g)
 // C(m, c) = sum_k A(m, k) phi(k, c); A(m, k) = A[(m / a_R) a_SS + (k / 16) a_KS + 16 (m % a_R)
 // + k % 16], lo at + a_lo.
 int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_SS, long a_KS,
-          const Split3& B, const OutMap& o, bool split_out, long M, int N, int K)
+          const Split3& B, const OutMap& o, bool split_out, long M, int N, int K, int kc = 0,
+          long c_ZS = 0)
 {
     if (M >= (1L << 31) || a_R >= (1L << 31) || o.R >= (1L << 31))
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
@@ -909,6 +940,8 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
     g.B = B.b; g.b_lo = B.lo; g.b_KS = B.ks;
     g.C = o.C; g.c_SS = o.SS; g.c_KS = o.KS; g.c_CS = o.CS; g.c_lo = o.lo; g.c_R = (int)o.R;
     g.M = (int)M; g.N = N; g.K = K;
+    g.kc = kc > 0 ? kc : K / v3::BK;
+    g.c_ZS = c_ZS;
     return split_out ? launch3_cols<true>(s, kind, g) : launch3_cols<false>(s, kind, g);
 }
 
@@ -1152,6 +1185,54 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
         FC3_TRY(modes12_v3(s, Ys, n, cnt, B, S, Y + n2 * l0));
     }
     return FC3_OK;
+}
+
+bool mcc3_contract_applies(int n)
+{
+    // the row-contiguous split runs n^2 / 64 blocks in grid y
+    return v3_applies(n, FC3_PREC_BF16X3) && (long)n * n / 64 + 1 <= 65535;
+}
+
+// one split operand: 2 n^3 bf16, plus 16 Npad of slack that the column tiles past n read
+static size_t contract_operand_bytes(int n)
+{
+    return Carve::need(sizeof(__bf16) * (2 * (size_t)n * n * n + 16 * (size_t)round_up(n, v3::BN)));
+}
+
+size_t mcc3_contract_bytes(int n) { return 2 * contract_operand_bytes(n) + 4096; }
+
+int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n, float* P,
+                  int* zc, void* ws, size_t ws_bytes)
+{
+    Carve cv(ws, ws_bytes);
+    const long n2 = (long)n * n, n3 = n2 * n;
+    __bf16* SA = cv.take<__bf16>(contract_operand_bytes(n) / sizeof(__bf16));
+    __bf16* SB = cv.take<__bf16>(contract_operand_bytes(n) / sizeof(__bf16));
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 adjoint workspace too small");
+    for (int t = 0; t < 2; ++t) {
+        const float* src = t == 0 ? X : H;
+        __bf16* dst = t == 0 ? SA : SB;
+        if (form == 0) {
+            // rows r < n, K index e < n^2 at stride n: src[r + n e]
+            dim3 gt((unsigned)((n + 63) / 64), (unsigned)((n2 + 63) / 64));
+            hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, src, (long)n, (int)n2,
+                               (int)n2, 0L, (long)n, dst);
+        } else {
+            const long q = n3 / 4;
+            hipLaunchKernelGGL(v3::k_split_rows, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
+                               src, n2, (long)n, dst);
+        }
+        FC3_HIP(hipGetLastError());
+    }
+    // K split into at most min(64, n) chunks; partial z at P + z n^2 as P_z[a + n b]
+    const int nkb = (int)(n2 / v3::BK);
+    const int maxz = n < 64 ? n : 64;
+    const int kc = (nkb + maxz - 1) / maxz;
+    *zc = (nkb + kc - 1) / kc;
+    const Split3 B{SB, n3, 16L * n};
+    const OutMap oP{P, n, 0, 16, n, 0};
+    return gemm3(s, FC3_KERNEL_GEMM_MODE12, SA, n3, n, 0, 16L * n, B, oP, false, n, n, (int)n2,
+                 kc, n2);
 }
 
 size_t mcc3_workspace_bytes(int n, int precision, bool inplace, const Mcc3Config* cfg)

@@ -59,8 +59,8 @@ def test_mcc3_backward_dpsi_adjoint_identity(ctx, n):
         assert rel <= (2e-6 if prec == "fp32" else 1e-5)
 
 
-@pytest.mark.parametrize("n,prec", [(48, "fp32"), (96, "fp32"), (48, "bf16x3"), (96, "bf16x3"),
-                                    (100, "bf16x3")])
+@pytest.mark.parametrize("n,prec", [(48, "fp32"), (96, "fp32"), (20, "bf16x3"), (48, "bf16x3"),
+                                    (96, "bf16x3"), (100, "bf16x3")])
 def test_mcc3_backward_dphi_central_difference(ctx, n, prec):
     psi = _rand((n, n, n), 11, np.float32)
     phi = _rand((n, n), 12, np.float32) / np.sqrt(n)
@@ -80,10 +80,11 @@ def test_mcc3_backward_dphi_central_difference(ctx, n, prec):
         assert abs(fd - ad) <= 2e-5 * np.linalg.norm(gf) * np.linalg.norm(D)
 
 
-@pytest.mark.parametrize("n", [48, 100])
+@pytest.mark.parametrize("n", [20, 48, 100])
 def test_mcc3_backward_bf16x3_matches_fp32(ctx, n):
-    """The BF16x3 pullback (mode products on the BF16x3 kernels) against the FP32 one,
-    both outputs, relative Frobenius error <= 1e-5 (the BF16x3 forward's bar)."""
+    """The BF16x3 pullback (mode products and split-K contractions on the BF16x3 kernels)
+    against the FP32 one, both outputs, relative Frobenius error <= 1e-5 (the BF16x3
+    forward's bar)."""
     psi = _rand((n, n, n), 31, np.float32)
     phi, _ = np.linalg.qr(np.random.default_rng(32).standard_normal((n, n)))
     phi = np.asfortranarray(phi.astype(np.float32))

@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-snapshot section (96 snapshots of 324/768/1500 DoF)")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="skip the fused third_order + mcc3 (no dense Psi) timing")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+backward timing at FCC 5x5x5 (1500 DoF)")
     a = ap.parse_args()
@@ -332,6 +334,21 @@ def main():
     ref = {l: O.mcc3_fp64_slab(coo32, phi64, l) for l in slabs}
     log("oracle slabs ready")
     flop = 6.0 * n ** 4
+
+    def slab_error():
+        """Relative Frobenius error of k3's sampled slabs against the FP64 reference."""
+        import ctypes
+        from fc3._lib import check
+        buf = np.empty((n, n), dtype=np.float32, order="F")
+        num = den = 0.0
+        for l in slabs:
+            check(ctx.lib.fc3_memcpy_d2h(ctx.h, buf.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_void_p(k3.ptr.value + 4 * n * n * l), 4 * n * n))
+            d = buf.astype(np.float64) - ref[l]
+            num += float((d * d).sum())
+            den += float((ref[l] * ref[l]).sum())
+        return (num / den) ** 0.5
+
     results = {}
     runs = [("bf16x3", a.steps)] + ([("fp32", a.fp32_steps)] if a.fp32_steps > 0 else [])
     for prec, steps in runs:
@@ -346,17 +363,7 @@ def main():
         m3_ms, m3_c = profile(ctx, 1)
         m12_ms, m12_c = profile(ctx, 2)
         profile_enable(ctx, False)
-        import ctypes
-        from fc3._lib import check
-        buf = np.empty((n, n), dtype=np.float32, order="F")
-        num = den = 0.0
-        for l in slabs:
-            check(ctx.lib.fc3_memcpy_d2h(ctx.h, buf.ctypes.data_as(ctypes.c_void_p),
-                                         ctypes.c_void_p(k3.ptr.value + 4 * n * n * l), 4 * n * n))
-            d = buf.astype(np.float64) - ref[l]
-            num += float((d * d).sum())
-            den += float((ref[l] * ref[l]).sum())
-        err = (num / den) ** 0.5
+        err = slab_error()
         mul = 3.0 if prec == "bf16x3" else 1.0
         results[prec] = {
             "ms_per_step": t, "TFLOP_per_s": flop / (t * 1e-3) / 1e12, "rel_fro_err": err,
@@ -398,6 +405,27 @@ def main():
         "assembly": asm,
     }
     d32.free()
+    if not a.no_fused:
+        # third_order + mcc3 without the dense Psi: sparse mode 1 from the pair blocks,
+        # modes 2 and 3 in BF16x3 (same positions, masses, phi and K3 output)
+        for _ in range(a.warmup):
+            fc3.third_order_mcc3(sysd, S.ARGON_LJ, calc, dphi, masses=sysd.masses, out=k3,
+                                 ctx=ctx)
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(a.steps):
+            fc3.third_order_mcc3(sysd, S.ARGON_LJ, calc, dphi, masses=sysd.masses, out=k3,
+                                 ctx=ctx)
+        t_f = ctx.timer_stop() / a.steps
+        out["fused"] = {
+            "ms_per_step": t_f, "rel_fro_err": slab_error(),
+            "dense_ms_per_step": asm["ms_per_step"] + bf["ms_per_step"],
+            "speedup_vs_dense": (asm["ms_per_step"] + bf["ms_per_step"]) / t_f,
+            "note": "third_order + mass weighting + mcc3 with no n^3 Psi: mode 1 over Psi's "
+                    "nonzeros from the pair blocks (FP32), modes 2 and 3 BF16x3; dense = "
+                    "assembly (FP64 + FP32 Psi) + BF16x3 mcc3 above",
+        }
+        log(f"fused: {out['fused']}")
     k3.free()
     if not a.no_batched:
         out["batched"] = batched_snapshots(ctx)

@@ -173,6 +173,14 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int cnt, doub
 
 }  // namespace
 
+int transpose_square(hipStream_t s, const float* in, int n, float* out)
+{
+    hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, s, in, n,
+                       out);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
 void release_blas(fc3_context* h)
 {
     if (h && h->blas) {

@@ -124,6 +124,15 @@ int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks&
 
 int mass_weight3(hipStream_t s, double* psi, int N, const double* d_masses);
 
+// Sparse mode-1 product fused to the pair assembly (no dense Psi): U1 = Psi32 x1 phi^T
+// from the blocks, with Psi32 exactly the FP32 writer's values, stored split (bf16
+// hi/lo) in the K-blocked mode-2 operand layout of mcc3_from_mode1 (P = n Kp per
+// half-slab, 2 P per slab, n slabs).  phiT = phi transposed (n x n).  Dynamic LDS of
+// lj_mode1_sparse_lds(cap) bytes (at most 64 KB).
+size_t lj_mode1_sparse_lds(int cap);
+int lj_mode1_sparse(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
+                    const double* d_masses, const float* phiT, int Kp, __bf16* U);
+
 // ---- Stillinger-Weber (sw.hip) ------------------------------------------------------
 // Workspace for one SW assembly with neighbour capacity `cap` (upper bound).
 size_t sw_workspace_bytes(int N, int cap, bool third);
@@ -176,6 +185,15 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
 // (the caller sums them).  form 0: A(a, e) = X[a + n e], B(e, b) = H[b + n e];
 // form 1: A(a, e) = X[e + n^2 a], B(e, b) = H[e + n^2 b].  Applies for n % 4 == 0,
 // 16 <= n and n^2 / 64 < 65535 (aligned tensors).
+// K padding of the split operands (n rounded up to 16), and modes 2 and 3 of mcc3 on
+// the BF16X3 kernels from a split mode-1 result U (as lj_mode1_sparse writes it):
+// K3 = U x2 phi x3 phi, FP32 in k3 (n % 4 == 0, n >= 16).
+int mcc3_kpad(int n);
+size_t mcc3_from_mode1_bytes(int n);
+int mcc3_from_mode1(hipStream_t s, const __bf16* U, const float* phi, int n, float* k3, void* ws,
+                    size_t ws_bytes);
+// out[j + n i] = in[i + n j] (device, n x n)
+int transpose_square(hipStream_t s, const float* in, int n, float* out);
 bool mcc3_contract_applies(int n);
 size_t mcc3_contract_bytes(int n);
 int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n, float* P,

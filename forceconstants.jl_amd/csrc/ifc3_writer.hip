@@ -16,6 +16,7 @@
 // ncols*n range of Psi), first resolves each column's nonzero rows into LDS, then
 // streams the whole range with 16-byte stores (8-byte for FP32), zeros included:
 // HBM sees exactly 8 n^3 (+ 4 n^3) bytes of writes and nothing else.
+#include "bf16_split.hpp"
 #include "fc3_internal.hpp"
 
 namespace fc3 {
@@ -214,6 +215,131 @@ k_mass_weight3(double* __restrict__ psi, int N, const double* __restrict__ m)
     }
 }
 
+// Sparse mode-1 product fused to the assembly (no dense Psi): for slab kk = 3K + g and
+// the 16 columns jj = 16 blockIdx.x + c,
+//   U1[p, jj, kk] = sum_i phiT[p + n i] Psi32[i, jj, kk]
+// over the nonzero rows i of column (jj, kk), with Psi32 the value the FP32 writer
+// stores there (tolerance, mass weighting and rounding as above).  U1 goes out split
+// into bf16 hi/lo in the K-blocked mode-2 operand layout: hi at
+// U[kk 2P + blockIdx.x 16 n + 16 p + c], lo at + P (P = n Kp); columns jj >= n (the K
+// padding) are zero.  Pair columns have 6 entries and the at most 3 self columns of a
+// block 3 + 3 count[K] each, so 16 * 6 + 3 * (3 + 3 cap) entries bound a block (dynamic
+// LDS: rows, then values).
+template <bool MW>
+__global__ void __launch_bounds__(256)
+k_lj_mode1_sparse(const int* __restrict__ count, const int* __restrict__ nbr,
+                  const int* __restrict__ rev, int N, int cap, const double* __restrict__ E,
+                  const double* __restrict__ S, double tol, const double* __restrict__ masses,
+                  const float* __restrict__ phiT, int Kp, __bf16* __restrict__ U)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
+    const int n = 3 * N;
+    const int kk = blockIdx.y;
+    const int K = kk / 3, g = kk - 3 * (kk / 3);
+    const int jb = blockIdx.x;
+    __shared__ int s_cnt[16], s_off[17];
+    int* e_row = reinterpret_cast<int*>(dyn);
+    float* e_val = reinterpret_cast<float*>(dyn) + (16 * 6 + 3 * (3 + 3 * cap));
+    const int cntK = count[K];
+    const int* nbrK = nbr + (size_t)K * cap;
+    const double mK = MW ? masses[K] : 0.0;
+    // 1. entries per column (thread c < 16 owns column c)
+    const int t = threadIdx.x;
+    int J = -1, b = 0, type = 0, rs = -1;
+    if (t < 16) {
+        const int jj = 16 * jb + t;
+        if (jj < n) {
+            J = jj / 3;
+            b = jj - 3 * J;
+            if (J == K) {
+                type = 2;
+            } else {
+                const int sl = find_slot(nbrK, cntK, J);
+                rs = sl >= 0 ? rev[(size_t)K * cap + sl] : -1;
+                if (rs >= 0) type = 1;
+            }
+        }
+        s_cnt[t] = type == 1 ? 6 : type == 2 ? 3 + 3 * cntK : 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int o = 0;
+        for (int c = 0; c < 16; ++c) { s_off[c] = o; o += s_cnt[c]; }
+        s_off[16] = o;
+    }
+    __syncthreads();
+    if (t < 16 && type != 0) {
+        int* row = e_row + s_off[t];
+        float* val = e_val + s_off[t];
+        if (type == 1) {
+            const double* e = E + ((size_t)J * cap + rs) * 27;
+            const double mJ = MW ? masses[J] : 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double vJ = tol_cut(e[a + 3 * b + 9 * g], tol);
+                const double vK = tol_cut(-e[b + 3 * g + 9 * a], tol);
+                row[a] = 3 * J + a;
+                row[3 + a] = 3 * K + a;
+                if (MW) {
+                    val[a] = (float)(vJ != 0.0 ? vJ / sqrt((mJ * mJ) * mK) : vJ);
+                    val[3 + a] = (float)(vK != 0.0 ? vK / sqrt((mK * mJ) * mK) : vK);
+                } else {
+                    val[a] = (float)vJ;
+                    val[3 + a] = (float)vK;
+                }
+            }
+        } else {
+            // self column: rows of K (ASR! self block) and of every neighbour I of K
+            for (int a = 0; a < 3; ++a) {
+                const double v = tol_cut(S[(size_t)K * 27 + a + 3 * b + 9 * g], tol);
+                row[a] = 3 * K + a;
+                val[a] = MW ? (float)(v != 0.0 ? v / sqrt((mK * mK) * mK) : v) : (float)v;
+            }
+            for (int sl = 0; sl < cntK; ++sl) {
+                const int I = nbrK[sl];
+                const double mI = MW ? masses[I] : 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const double v =
+                        tol_cut(E[((size_t)K * cap + sl) * 27 + b + 3 * g + 9 * a], tol);
+                    row[3 + 3 * sl + a] = 3 * I + a;
+                    val[3 + 3 * sl + a] =
+                        MW ? (float)(v != 0.0 ? v / sqrt((mI * mK) * mK) : v) : (float)v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // 2. thread p: the 16 columns' sums, split and stored as two 32-byte runs
+    const long P = (long)n * Kp;
+    __bf16* Uh = U + (long)kk * 2 * P + (long)jb * 16 * n;
+    for (int p = t; p < n; p += blockDim.x) {
+        float acc[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            float a = 0.0f;
+            for (int e = s_off[c]; e < s_off[c + 1]; ++e)
+                a = fmaf(phiT[p + (long)n * e_row[e]], e_val[e], a);
+            acc[c] = a;
+        }
+        bf16x8 h0, h1, l0, l1;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            __bf16 hh, ll;
+            split_bf16(acc[c], hh, ll);
+            h0[c] = hh;
+            l0[c] = ll;
+            split_bf16(acc[8 + c], hh, ll);
+            h1[c] = hh;
+            l1[c] = ll;
+        }
+        bf16x8* dh = reinterpret_cast<bf16x8*>(Uh + 16L * p);
+        bf16x8* dl = reinterpret_cast<bf16x8*>(Uh + P + 16L * p);
+        dh[0] = h0;
+        dh[1] = h1;
+        dl[0] = l0;
+        dl[1] = l1;
+    }
+}
+
 }  // namespace
 
 int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
@@ -229,6 +355,30 @@ int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks&
         dispatch_writer<2>(s, nl, b, tol, d_masses, psi64, psi32, (int)cpw);
     else
         dispatch_writer<1>(s, nl, b, tol, d_masses, psi64, psi32, (int)cpw);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+size_t lj_mode1_sparse_lds(int cap)
+{
+    return (size_t)(16 * 6 + 3 * (3 + 3 * cap)) * (sizeof(int) + sizeof(float));
+}
+
+int lj_mode1_sparse(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
+                    const double* d_masses, const float* phiT, int Kp, __bf16* U)
+{
+    const int n = 3 * nl.n_atoms;
+    const size_t lds = lj_mode1_sparse_lds(nl.cap);
+    if (lds > 64 * 1024)
+        return set_error(FC3_ERR_UNSUPPORTED, "fused mode 1: too many neighbours per atom");
+    dim3 grid((unsigned)(Kp / 16), (unsigned)n);
+    ProfScope prof(FC3_KERNEL_IFC3_WRITE, s);
+    if (d_masses)
+        hipLaunchKernelGGL(k_lj_mode1_sparse<true>, grid, dim3(256), lds, s, nl.count, nl.nbr,
+                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kp, U);
+    else
+        hipLaunchKernelGGL(k_lj_mode1_sparse<false>, grid, dim3(256), lds, s, nl.count, nl.nbr,
+                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kp, U);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }

@@ -17,24 +17,16 @@
 //                split for the next one) and copied global -> LDS without registers.
 #include <algorithm>
 
+#include "bf16_split.hpp"
 #include "fc3_internal.hpp"
 
 namespace fc3 {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int BK = 96;                // K padding granularity of the prepared phi (6 x 16)
 constexpr int kXcd = 8;               // XCDs on MI355X
-
-__device__ __forceinline__ void split_bf16(float x, __bf16& h, __bf16& l)
-{
-    h = static_cast<__bf16>(x);
-    const float hf = static_cast<float>(h);
-    l = static_cast<__bf16>(x - hf);
-}
 
 // ---------------------------------------------------------------------------------
 // GEMM v2: 256 x 256 macro tile, 4 waves (2 x 2), each wave a 128 x 128 tile of 4 x 4
@@ -1183,6 +1175,45 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
                            X + n2 * l0, n, padk(n), (long)cnt, Ys);
         FC3_HIP(hipGetLastError());
         FC3_TRY(modes12_v3(s, Ys, n, cnt, B, S, Y + n2 * l0));
+    }
+    return FC3_OK;
+}
+
+int mcc3_kpad(int n) { return padk(n); }
+
+size_t mcc3_from_mode1_bytes(int n)
+{
+    const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
+    return phi3_bytes(n) + Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n)) + 4096;
+}
+
+int mcc3_from_mode1(hipStream_t s, const __bf16* U, const float* phi, int n, float* k3, void* ws,
+                    size_t ws_bytes)
+{
+    if (!v3_applies(n, FC3_PREC_BF16X3) || !aligned16(U) || !aligned16(k3))
+        return set_error(FC3_ERR_UNSUPPORTED, "fused mcc3: needs n % 4 == 0, n >= 16");
+    Carve cv(ws, ws_bytes);
+    Split3 B;
+    FC3_TRY(prep_phi3(s, phi, n, cv, B));
+    const long n2 = (long)n * n;
+    const int Kp = padk(n);
+    const long P = (long)n * Kp;
+    const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
+    __bf16* X = cv.take<__bf16>((size_t)2 * R * Kp);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "fused mcc3 workspace too small");
+    // mode 2 by slab batches: C(l n + p, q) = K3[p + n q + n^2 l]
+    const int nb = slab_batch3(n, size_t(1) << 62, kTilesPerLaunch);
+    for (int l0 = 0; l0 < n; l0 += nb) {
+        const int cnt = (n - l0) < nb ? (n - l0) : nb;
+        const OutMap oT{k3 + n2 * l0, n, n2, 16, n, 0};
+        FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U + 2 * P * l0, P, n, 2 * P, 16L * n, B, oT,
+                      false, (long)cnt * n, n, Kp));
+    }
+    // mode 3 in place by row blocks (each block split into X before the GEMM overwrites it)
+    for (long r0 = 0; r0 < n2; r0 += R) {
+        const long rows = (n2 - r0) < R ? (n2 - r0) : R;
+        const OutMap oK{k3 + r0, rows, 0, 16, n2, 0};
+        FC3_TRY(mode3_block(s, k3, n, r0, rows, B, X, oK, false));
     }
     return FC3_OK;
 }

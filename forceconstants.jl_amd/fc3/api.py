@@ -191,6 +191,28 @@ def third_order_device(sys: SuperCellSystem, pot, calc, psi64: Optional[DeviceAr
     check(st)
 
 
+def third_order_mcc3(sys: SuperCellSystem, pot, calc, phi: DeviceArray, masses=None,
+                     out: Optional[DeviceArray] = None, ctx: Optional[Context] = None
+                     ) -> DeviceArray:
+    """K3 = mcc3(mass-weighted FP32 third_order(sys, LJ, calc), phi) without forming Psi:
+    the mode-1 product over Psi's nonzeros straight from the pair blocks, modes 2 and 3
+    in BF16x3 (fc3_third_order_mcc3_lj).  LJ with the analytical calculator only;
+    3N % 4 == 0."""
+    ctx = ctx or Context.default()
+    if not (isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)):
+        raise NotImplementedError(f"third_order_mcc3 for {type(pot).__name__} with {type(calc).__name__}")
+    n = sys.n_dof
+    if phi.shape != (n, n):
+        raise AssertionError("third_order_mcc3: phi must be n x n")
+    k3 = out if out is not None else DeviceArray((n, n, n), np.float32, ctx)
+    pos, box = _sys_args(sys)
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    check(ctx.lib.fc3_third_order_mcc3_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
+                                          pot.epsilon, pot.r_cut, calc.r_cut, calc.tol,
+                                          _ptr(m) if m is not None else None, phi.ptr, k3.ptr))
+    return k3
+
+
 def third_order(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = None
                 ) -> DenseForceConstants:
     """third_order(sys, pot, calc) -> DenseForceConstants{3} (host values)."""

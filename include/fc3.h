@@ -124,6 +124,21 @@ int fc3_third_order_lj(fc3_handle h, const double* positions, int n_atoms,
                        double pot_r_cut, double r_cut, double tol,
                        const double* masses, double* psi64, float* psi32);
 
+/* Fused third_order + mass weighting + mcc3 for LJ without the dense tensor: the
+ * K3 = mcc3(Psi32, phi) that fc3_third_order_lj (psi32, masses) followed by fc3_mcc3
+ * (FC3_PREC_BF16X3) gives, with the mode-1 product taken over Psi's nonzeros straight
+ * from the pair blocks (FP32; the values are exactly those psi32 would hold) and
+ * modes 2 and 3 on the BF16x3 kernels.  No counterpart in the reference; it follows
+ * the sparse mode-1 intent of src/todo/old_mcc.jl:17-65.  Neither the FP64 nor the
+ * FP32 n^3 Psi is formed; the workspace holds about one n^3 FP32 of split mode-1
+ * result.
+ *   phi: device n x n FP32;  k3: device n^3 FP32, 16-byte aligned;  masses: host or NULL.
+ * Needs 3N % 4 == 0 and 3N >= 16 (FC3_ERR_UNSUPPORTED otherwise).                  */
+int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
+                            const double box[3], double sigma, double epsilon,
+                            double pot_r_cut, double r_cut, double tol, const double* masses,
+                            const float* phi, float* k3);
+
 /* third_order(sys, StillingerWeberSilicon(), AutoDiffCalculator) replaces
  * src/ifc.jl:20-35 -> src/autodiff/third_order_AD.jl:71-224 (+ ASR!, apply_tols!),
  * with the same fused outputs as fc3_third_order_lj.                            */

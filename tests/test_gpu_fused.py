@@ -1,0 +1,77 @@
+"""Fused LJ third_order + mcc3 without the dense tensor (fc3_third_order_mcc3_lj) on the GPU.
+
+Pins:
+  * against the FP64 oracle: K3 = mcc3 of the oracle's mass-weighted Psi (the reference
+    loop restated, then mass_weight_third_order!), contracted in FP64 from the same FP32
+    Psi and phi, relative Frobenius error <= 1e-5 (the BF16x3 bar);
+  * against the library's dense path (fc3_third_order_lj into psi32, then fc3_mcc3 in
+    BF16x3) on the same inputs, <= 1e-5: the two differ only in how the mode-1 product
+    is evaluated (over the nonzeros in FP32, or densely in BF16x3).
+Sizes cover K padding (n = 324 -> 336) and none (n = 96, 768), with and without
+masses, and a tolerance that zeroes entries (1e-8, as the reference's "LJ Values" test).
+"""
+import numpy as np
+import pytest
+
+import fc3
+from fc3 import systems as S
+from fc3._lib import FC3Error
+from oracle import fc3_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LJ = S.ARGON_LJ
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a.astype(np.float64) - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def _case(ctx, cells, seed, su, tol):
+    sysd = S.fcc(S.ARGON_A, cells, S.ARGON_MASS, su, seed=seed)
+    calc = S.AnalyticalCalculator(tol, LJ.r_cut)
+    f2 = O.lj_second_order(sysd, LJ, calc)
+    _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    return sysd, calc, phi32, fc3.DeviceArray.from_host(phi32, ctx=ctx)
+
+
+@pytest.mark.parametrize("cells,seed,su,tol,weighted", [
+    ((3, 3, 3), 41, 0.05, 1e-12, True),
+    ((2, 2, 2), 42, 0.08, 1e-8, False),
+    ((2, 2, 2), 43, 0.03, 1e-12, True),
+])
+def test_fused_matches_fp64_oracle(ctx, cells, seed, su, tol, weighted):
+    sysd, calc, phi32, dphi = _case(ctx, cells, seed, su, tol)
+    masses = sysd.masses if weighted else None
+    k3 = fc3.third_order_mcc3(sysd, LJ, calc, dphi, masses=masses, ctx=ctx).to_host()
+    psi = O.lj_third_order(sysd, LJ, calc)
+    if weighted:
+        O.mass_weight_third_order(psi, sysd.masses)
+    ref = O.mcc3_fp64(psi.astype(np.float32), phi32)
+    err = _rel(k3, ref)
+    print(f"{cells} seed {seed} tol {tol} weighted {weighted}: rel fro err vs FP64 {err:.3e}",
+          flush=True)
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("cells,seed", [((3, 3, 3), 44), ((4, 4, 4), 45)])
+def test_fused_matches_dense_path(ctx, cells, seed):
+    sysd, calc, phi32, dphi = _case(ctx, cells, seed, 0.05, 1e-12)
+    n = sysd.n_dof
+    k3f = fc3.third_order_mcc3(sysd, LJ, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, LJ, calc, psi32=psi32, masses=sysd.masses, ctx=ctx)
+    k3d = fc3.mcc3(psi32, dphi, precision="bf16x3").to_host()
+    err = _rel(k3f, k3d.astype(np.float64))
+    print(f"{cells} (n={n}): fused vs dense BF16x3 rel {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_fused_rejects_unsupported_size(ctx):
+    full = S.fcc(S.ARGON_A, (2, 2, 2), S.ARGON_MASS, 0.0, seed=1)
+    sysd = S.SuperCellSystem(full.positions[:31], full.masses[:31], full.box_sizes)   # 3N = 93
+    calc = S.AnalyticalCalculator(1e-12, LJ.r_cut)
+    dphi = fc3.DeviceArray.from_host(np.eye(93, dtype=np.float32, order="F"), ctx=ctx)
+    with pytest.raises(FC3Error):
+        fc3.third_order_mcc3(sysd, LJ, calc, dphi, ctx=ctx)

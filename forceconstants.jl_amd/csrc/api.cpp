@@ -366,23 +366,26 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
     if (n % 4 != 0 || n < 16 || n * n >= (1L << 31) || (uintptr_t)k3 % 16 != 0)
         return set_error(FC3_ERR_UNSUPPORTED,
                          "fused third_order + mcc3: needs 3N % 4 == 0, 3N >= 16, aligned k3");
-    const size_t Kp = (size_t)mcc3_kpad((int)n);
-    const size_t u_bytes = Carve::need(sizeof(uint16_t) * 2 * (size_t)n * Kp * n);   // U, bf16
+    // U: the gathered mode-1 result, 2 (3n) Kc bf16 per atom
+    auto u_bytes = [&](int cap) {
+        return Carve::need(sizeof(uint16_t) * 2 * 3 * (size_t)n * mcc3_gather_k(cap) * N);
+    };
     Prepared P;
     FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
         return Carve::need(sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * (27 + 9))) +
-               Carve::need(sizeof(float) * (size_t)n * n) + u_bytes + mcc3_from_mode1_bytes((int)n) +
-               4096;
+               Carve::need(sizeof(float) * (size_t)n * n) + u_bytes(cap) +
+               mcc3_from_gathered_bytes((int)n, cap) + 4096;
     }, P));
     if (lj_mode1_sparse_lds(P.nl.cap) > 64 * 1024)
         return set_error(FC3_ERR_UNSUPPORTED, "fused mode 1: too many neighbours per atom");
+    const int Kc = mcc3_gather_k(P.nl.cap);
     Carve cv(P.extra, P.extra_bytes);
     const size_t slots = (size_t)N * P.nl.cap;
     double* E3 = cv.take<double>(slots * 27 + slots * 9 + (size_t)N * (27 + 9));
     double* E2 = E3 + slots * 27;
     double* S3 = E2 + slots * 9;
     float* phiT = cv.take<float>((size_t)n * n);
-    __bf16* U = cv.take<__bf16>(2 * (size_t)n * Kp * n);
+    __bf16* U = cv.take<__bf16>(2 * 3 * (size_t)n * Kc * N);
     char* rest = cv.take<char>(0);
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "fused third_order + mcc3 workspace");
     FC3_TRY(lj_pair_blocks(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, S3, E2, nullptr));
@@ -390,9 +393,9 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
     b.E = E3;
     b.S = S3;
     FC3_TRY(transpose_square(h->stream, phi, (int)n, phiT));
-    FC3_TRY(lj_mode1_sparse(h->stream, P.nl, b, tol, masses ? P.masses : nullptr, phiT, (int)Kp,
-                            U));
-    return mcc3_from_mode1(h->stream, U, phi, (int)n, k3, rest, cv.cap - cv.off);
+    FC3_TRY(lj_mode1_gathered(h->stream, P.nl, b, tol, masses ? P.masses : nullptr, phiT, Kc, U));
+    return mcc3_from_gathered(h->stream, U, phi, (int)n, P.nl.count, P.nl.nbr, P.nl.cap, k3, rest,
+                              cv.cap - cv.off);
 }
 
 int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int dim,

@@ -124,14 +124,16 @@ int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks&
 
 int mass_weight3(hipStream_t s, double* psi, int N, const double* d_masses);
 
-// Sparse mode-1 product fused to the pair assembly (no dense Psi): U1 = Psi32 x1 phi^T
-// from the blocks, with Psi32 exactly the FP32 writer's values, stored split (bf16
-// hi/lo) in the K-blocked mode-2 operand layout of mcc3_from_mode1 (P = n Kp per
-// half-slab, 2 P per slab, n slabs).  phiT = phi transposed (n x n).  Dynamic LDS of
-// lj_mode1_sparse_lds(cap) bytes (at most 64 KB).
+// Sparse mode-1 product fused to the pair assembly (no dense Psi): for every atom K
+// and its three slabs kk = 3K + g, U_K(g n + p, t) = sum_i phiT[p + n i] Psi32[i, jj_t, kk]
+// over the columns jj_t = 3 J_t + b (t = 3 s + b; J_0 = K, J_s = nbr[K][s - 1]), the
+// only columns of those slabs that hold nonzeros; Psi32 exactly the FP32 writer's
+// values.  Split (bf16 hi/lo), K-blocked over t with Kc = mcc3_gather_k(cap) (zero
+// past the count): hi at U[K 2 (3n) Kc + (t / 16) 16 (3n) + 16 (g n + p) + t % 16], lo at
+// + 3n Kc.  Dynamic LDS of lj_mode1_sparse_lds(cap) bytes (at most 64 KB).
 size_t lj_mode1_sparse_lds(int cap);
-int lj_mode1_sparse(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
-                    const double* d_masses, const float* phiT, int Kp, __bf16* U);
+int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
+                      const double* d_masses, const float* phiT, int Kc, __bf16* U);
 
 // ---- Stillinger-Weber (sw.hip) ------------------------------------------------------
 // Workspace for one SW assembly with neighbour capacity `cap` (upper bound).
@@ -185,13 +187,13 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
 // (the caller sums them).  form 0: A(a, e) = X[a + n e], B(e, b) = H[b + n e];
 // form 1: A(a, e) = X[e + n^2 a], B(e, b) = H[e + n^2 b].  Applies for n % 4 == 0,
 // 16 <= n and n^2 / 64 < 65535 (aligned tensors).
-// K padding of the split operands (n rounded up to 16), and modes 2 and 3 of mcc3 on
-// the BF16X3 kernels from a split mode-1 result U (as lj_mode1_sparse writes it):
-// K3 = U x2 phi x3 phi, FP32 in k3 (n % 4 == 0, n >= 16).
-int mcc3_kpad(int n);
-size_t mcc3_from_mode1_bytes(int n);
-int mcc3_from_mode1(hipStream_t s, const __bf16* U, const float* phi, int n, float* k3, void* ws,
-                    size_t ws_bytes);
+// Modes 2 and 3 of mcc3 on the BF16X3 kernels from the gathered mode-1 result U of
+// lj_mode1_gathered: mode 2 as one batched product per atom over its neighbourhood's
+// columns (gathered phi rows), mode 3 in place; K3 FP32 in k3 (n % 4 == 0, n >= 16).
+int mcc3_gather_k(int cap);
+size_t mcc3_from_gathered_bytes(int n, int cap);
+int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, int n, const int* count,
+                       const int* nbr, int cap, float* k3, void* ws, size_t ws_bytes);
 // out[j + n i] = in[i + n j] (device, n x n)
 int transpose_square(hipStream_t s, const float* in, int n, float* out);
 bool mcc3_contract_applies(int n);

@@ -215,28 +215,29 @@ k_mass_weight3(double* __restrict__ psi, int N, const double* __restrict__ m)
     }
 }
 
-// Sparse mode-1 product fused to the assembly (no dense Psi): for slab kk = 3K + g and
-// the 16 columns jj = 16 blockIdx.x + c,
-//   U1[p, jj, kk] = sum_i phiT[p + n i] Psi32[i, jj, kk]
-// over the nonzero rows i of column (jj, kk), with Psi32 the value the FP32 writer
-// stores there (tolerance, mass weighting and rounding as above).  U1 goes out split
-// into bf16 hi/lo in the K-blocked mode-2 operand layout: hi at
-// U[kk 2P + blockIdx.x 16 n + 16 p + c], lo at + P (P = n Kp); columns jj >= n (the K
-// padding) are zero.  Pair columns have 6 entries and the at most 3 self columns of a
-// block 3 + 3 count[K] each, so 16 * 6 + 3 * (3 + 3 cap) entries bound a block (dynamic
-// LDS: rows, then values).
+// Sparse mode-1 product fused to the assembly (no dense Psi).  Slab kk = 3K + g holds
+// nonzeros only in the columns jj_t = 3 J_t + b of K's neighbourhood (t = 3 s + b; J_0 =
+// K, the self column; J_s = nbr[K][s - 1], pair columns).  For the 16 such columns
+// t = 16 blockIdx.x + c,
+//   U_K(g n + p, t) = sum_i phiT[p + n i] Psi32[i, jj_t, kk]
+// over the nonzero rows i of column (jj_t, kk), with Psi32 the value the FP32 writer
+// stores there (tolerance, mass weighting and rounding as above).  U goes out split
+// into bf16 hi/lo, K-blocked over t: hi at U[K 2 M Kc + blockIdx.x 16 M + 16 (g n + p) +
+// c] with M = 3n, lo at + M Kc; columns t past the count are zero.  Pair columns have 6
+// entries and the 3 self columns (all in the first block) 3 + 3 count[K] each, so
+// 16 * 6 + 3 * (3 + 3 cap) entries bound a block (dynamic LDS: rows, then values).
 template <bool MW>
 __global__ void __launch_bounds__(256)
-k_lj_mode1_sparse(const int* __restrict__ count, const int* __restrict__ nbr,
-                  const int* __restrict__ rev, int N, int cap, const double* __restrict__ E,
-                  const double* __restrict__ S, double tol, const double* __restrict__ masses,
-                  const float* __restrict__ phiT, int Kp, __bf16* __restrict__ U)
+k_lj_mode1_gathered(const int* __restrict__ count, const int* __restrict__ nbr,
+                    const int* __restrict__ rev, int N, int cap, const double* __restrict__ E,
+                    const double* __restrict__ S, double tol, const double* __restrict__ masses,
+                    const float* __restrict__ phiT, int Kc, __bf16* __restrict__ U)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
     const int n = 3 * N;
     const int kk = blockIdx.y;
     const int K = kk / 3, g = kk - 3 * (kk / 3);
-    const int jb = blockIdx.x;
+    const int tb = blockIdx.x;
     __shared__ int s_cnt[16], s_off[17];
     int* e_row = reinterpret_cast<int*>(dyn);
     float* e_val = reinterpret_cast<float*>(dyn) + (16 * 6 + 3 * (3 + 3 * cap));
@@ -247,17 +248,15 @@ k_lj_mode1_sparse(const int* __restrict__ count, const int* __restrict__ nbr,
     const int t = threadIdx.x;
     int J = -1, b = 0, type = 0, rs = -1;
     if (t < 16) {
-        const int jj = 16 * jb + t;
-        if (jj < n) {
-            J = jj / 3;
-            b = jj - 3 * J;
-            if (J == K) {
-                type = 2;
-            } else {
-                const int sl = find_slot(nbrK, cntK, J);
-                rs = sl >= 0 ? rev[(size_t)K * cap + sl] : -1;
-                if (rs >= 0) type = 1;
-            }
+        const int col = 16 * tb + t, sl = col / 3;
+        b = col - 3 * sl;
+        if (sl == 0) {
+            J = K;
+            type = 2;
+        } else if (sl - 1 < cntK) {
+            J = nbrK[sl - 1];
+            rs = rev[(size_t)K * cap + sl - 1];   // slot of K in J's list
+            if (rs >= 0) type = 1;
         }
         s_cnt[t] = type == 1 ? 6 : type == 2 ? 3 + 3 * cntK : 0;
     }
@@ -309,8 +308,8 @@ k_lj_mode1_sparse(const int* __restrict__ count, const int* __restrict__ nbr,
     }
     __syncthreads();
     // 2. thread p: the 16 columns' sums, split and stored as two 32-byte runs
-    const long P = (long)n * Kp;
-    __bf16* Uh = U + (long)kk * 2 * P + (long)jb * 16 * n;
+    const long M = 3L * n;
+    __bf16* Uh = U + (long)K * 2 * M * Kc + (long)tb * 16 * M + 16L * g * n;
     for (int p = t; p < n; p += blockDim.x) {
         float acc[16];
 #pragma unroll
@@ -332,7 +331,7 @@ k_lj_mode1_sparse(const int* __restrict__ count, const int* __restrict__ nbr,
             l1[c] = ll;
         }
         bf16x8* dh = reinterpret_cast<bf16x8*>(Uh + 16L * p);
-        bf16x8* dl = reinterpret_cast<bf16x8*>(Uh + P + 16L * p);
+        bf16x8* dl = reinterpret_cast<bf16x8*>(Uh + M * Kc + 16L * p);
         dh[0] = h0;
         dh[1] = h1;
         dl[0] = l0;
@@ -364,21 +363,23 @@ size_t lj_mode1_sparse_lds(int cap)
     return (size_t)(16 * 6 + 3 * (3 + 3 * cap)) * (sizeof(int) + sizeof(float));
 }
 
-int lj_mode1_sparse(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
-                    const double* d_masses, const float* phiT, int Kp, __bf16* U)
+int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
+                      const double* d_masses, const float* phiT, int Kc, __bf16* U)
 {
     const int n = 3 * nl.n_atoms;
     const size_t lds = lj_mode1_sparse_lds(nl.cap);
     if (lds > 64 * 1024)
         return set_error(FC3_ERR_UNSUPPORTED, "fused mode 1: too many neighbours per atom");
-    dim3 grid((unsigned)(Kp / 16), (unsigned)n);
+    if (Kc % 16 != 0 || Kc < 3 * (nl.cap + 1))
+        return set_error(FC3_ERR_ARGUMENT, "fused mode 1: gathered K too small");
+    dim3 grid((unsigned)(Kc / 16), (unsigned)n);
     ProfScope prof(FC3_KERNEL_IFC3_WRITE, s);
     if (d_masses)
-        hipLaunchKernelGGL(k_lj_mode1_sparse<true>, grid, dim3(256), lds, s, nl.count, nl.nbr,
-                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kp, U);
+        hipLaunchKernelGGL(k_lj_mode1_gathered<true>, grid, dim3(256), lds, s, nl.count, nl.nbr,
+                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kc, U);
     else
-        hipLaunchKernelGGL(k_lj_mode1_sparse<false>, grid, dim3(256), lds, s, nl.count, nl.nbr,
-                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kp, U);
+        hipLaunchKernelGGL(k_lj_mode1_gathered<false>, grid, dim3(256), lds, s, nl.count, nl.nbr,
+                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kc, U);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }

@@ -323,10 +323,12 @@ struct G3 {
     int M, N, K;         // K % BK == 0, M % 4 == 0, c_R % 4 == 0
     int tiles_m, tiles_n;
     long per_xcd;
-    // split K: workgroups with blockIdx.y = z take K blocks [z kc, min((z + 1) kc, K / BK))
-    // and write C + z c_ZS (kc = K / BK and gridDim.y = 1 for an unsplit product)
-    int kc;
-    long c_ZS;
+    // grid y (nz workgroup rows, z = blockIdx.y), either
+    //   split K (zk = 1): z takes K blocks [z kc, min((z + 1) kc, K / BK)), or
+    //   a batch (zk = 0): z multiplies A + z a_ZS by B + z b_ZS over all of K;
+    // either way it writes C + z c_ZS.  An unsplit product: zk = 1, kc = K / BK, nz = 1.
+    int kc, zk, nz;
+    long a_ZS, b_ZS, c_ZS;
 };
 
 // Workgroup barrier for the copy ring: s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding,
@@ -445,10 +447,12 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
         }
     }
     const long kstride = is_a ? g.a_KS : g.b_KS;   // elements per K block
-    const int kz = blockIdx.y, kbeg = kz * g.kc;
-    const int nk = min(g.kc, g.K / BK - kbeg);    // >= 1: the host launches no empty chunk
+    const int kz = blockIdx.y, kbeg = g.zk ? kz * g.kc : 0;
+    // >= 1: the host launches no empty chunk
+    const int nk = g.zk ? min(g.kc, g.K / BK - kbeg) : g.K / BK;
+    const long zoff = kbeg * kstride + (long)kz * (is_a ? g.a_ZS : g.b_ZS);
 #pragma unroll
-    for (int q = 0; q < CMAX; ++q) src[q] += kbeg * kstride;
+    for (int q = 0; q < CMAX; ++q) src[q] += zoff;
     const int wofs = wave == 0 ? 0 : wave == 1 ? AE : wave == 2 ? 2 * AE : 2 * AE + BE;
     // this wave's copies of step k into stage array `st`; steps past the end copy the
     // last step again (into a stage whose fragments are read but not used), so that
@@ -883,10 +887,10 @@ int launch3(hipStream_t s, int kind, v3::G3 g, int tbn)
     g.per_xcd = (tiles + kXcd - 1) / kXcd;
     const long grid = g.per_xcd * kXcd;
     if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
-    const int nkb = g.K / v3::BK, zc = (nkb + g.kc - 1) / g.kc;
+    if (g.nz < 1 || g.nz > 65535) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid y out of range");
     ProfScope prof(kind, s);
-    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, SHAPE>), dim3((unsigned)grid, (unsigned)zc), dim3(v3::NT),
-                       0, s, g);
+    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, SHAPE>), dim3((unsigned)grid, (unsigned)g.nz),
+                       dim3(v3::NT), 0, s, g);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
@@ -918,22 +922,28 @@ int launch3_cols(hipStream_t s, int kind, const v3::G3& g)
 // + k % 16], lo at + a_lo.
 int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_SS, long a_KS,
           const Split3& B, const OutMap& o, bool split_out, long M, int N, int K, int kc = 0,
-          long c_ZS = 0)
+          long c_ZS = 0, int batch = 0, long a_ZS = 0, long b_ZS = 0)
 {
     if (M >= (1L << 31) || a_R >= (1L << 31) || o.R >= (1L << 31))
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
     const bool ok = K % v3::BK == 0 && M % 4 == 0 && M > 0 && a_lo % 8 == 0 && a_SS % 8 == 0 &&
                     a_KS % 8 == 0 && B.lo % 8 == 0 && B.ks % 8 == 0 && o.R % 4 == 0 &&
                     o.SS % 4 == 0 && o.KS % 4 == 0 && o.CS % 4 == 0 && o.lo % 4 == 0 &&
-                    aligned16(A) && aligned16(B.b) && aligned16(o.C);
+                    aligned16(A) && aligned16(B.b) && aligned16(o.C) && a_ZS % 8 == 0 &&
+                    b_ZS % 8 == 0 && c_ZS % 4 == 0;
     if (!ok) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: v3 GEMM operand layout");
     v3::G3 g;
     g.A = A; g.a_lo = a_lo; g.a_SS = a_SS; g.a_KS = a_KS; g.a_R = (int)a_R;
     g.B = B.b; g.b_lo = B.lo; g.b_KS = B.ks;
     g.C = o.C; g.c_SS = o.SS; g.c_KS = o.KS; g.c_CS = o.CS; g.c_lo = o.lo; g.c_R = (int)o.R;
     g.M = (int)M; g.N = N; g.K = K;
-    g.kc = kc > 0 ? kc : K / v3::BK;
-    g.c_ZS = c_ZS;
+    if (batch > 0) {   // batch of products over z < batch
+        g.zk = 0; g.kc = K / v3::BK; g.nz = batch;
+    } else {           // K split into chunks of kc blocks (kc = 0: unsplit)
+        g.zk = 1; g.kc = kc > 0 ? kc : K / v3::BK;
+        g.nz = (K / v3::BK + g.kc - 1) / g.kc;
+    }
+    g.a_ZS = a_ZS; g.b_ZS = b_ZS; g.c_ZS = c_ZS;
     return split_out ? launch3_cols<true>(s, kind, g) : launch3_cols<false>(s, kind, g);
 }
 
@@ -1179,36 +1189,74 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
     return FC3_OK;
 }
 
-int mcc3_kpad(int n) { return padk(n); }
-
-size_t mcc3_from_mode1_bytes(int n)
+// Gathered phi rows for the fused path: for atom K, B_K(t, q) = phi[row_K(t), q] with
+// row_K(3 s + b) = 3 K + b (s = 0) or 3 nbr[K][s - 1] + b (s <= count[K]), zero past
+// the count or past n; split, K-blocked, hi at Bg[K 2 Kc Npad + (t / 16) 16 Npad + 16 q +
+// t % 16], lo at + Kc Npad.  Thread (q, t block, K) writes 16 t as two 32-byte runs.
+__global__ void __launch_bounds__(256) k_gather_phi_nbr(const float* __restrict__ phi, int n,
+                                                         const int* __restrict__ count,
+                                                         const int* __restrict__ nbr, int cap,
+                                                         int Kc, int Npad, __bf16* __restrict__ Bg)
 {
-    const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
-    return phi3_bytes(n) + Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n)) + 4096;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Npad) return;
+    const int tb = blockIdx.y, K = blockIdx.z;
+    const int cnt = count[K];
+    bf16x8 h0, h1, l0, l1;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int t = 16 * tb + c, sl = t / 3, b = t - 3 * (t / 3);
+        int row = -1;
+        if (sl == 0) row = 3 * K + b;
+        else if (sl - 1 < cnt) row = 3 * nbr[(size_t)K * cap + sl - 1] + b;
+        const float x = (row >= 0 && q < n) ? phi[row + (long)n * q] : 0.0f;
+        __bf16 hh, ll;
+        split_bf16(x, hh, ll);
+        if (c < 8) { h0[c] = hh; l0[c] = ll; } else { h1[c - 8] = hh; l1[c - 8] = ll; }
+    }
+    const long base = (long)K * 2 * Kc * Npad + (long)tb * 16 * Npad + 16L * q;
+    bf16x8* dh = reinterpret_cast<bf16x8*>(Bg + base);
+    bf16x8* dl = reinterpret_cast<bf16x8*>(Bg + base + (long)Kc * Npad);
+    dh[0] = h0;
+    dh[1] = h1;
+    dl[0] = l0;
+    dl[1] = l1;
 }
 
-int mcc3_from_mode1(hipStream_t s, const __bf16* U, const float* phi, int n, float* k3, void* ws,
-                    size_t ws_bytes)
+int mcc3_gather_k(int cap) { return round_up(3 * (cap + 1), 16); }
+
+size_t mcc3_from_gathered_bytes(int n, int cap)
 {
-    if (!v3_applies(n, FC3_PREC_BF16X3) || !aligned16(U) || !aligned16(k3))
+    const int N = n / 3, Kc = mcc3_gather_k(cap), Npad = round_up(n, v3::BN);
+    const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
+    return phi3_bytes(n) + Carve::need(2 * sizeof(__bf16) * (size_t)Kc * Npad * N) +
+           Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n)) + 4096;
+}
+
+int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, int n, const int* count,
+                       const int* nbr, int cap, float* k3, void* ws, size_t ws_bytes)
+{
+    if (!v3_applies(n, FC3_PREC_BF16X3) || !aligned16(U) || !aligned16(k3) || n % 3 != 0)
         return set_error(FC3_ERR_UNSUPPORTED, "fused mcc3: needs n % 4 == 0, n >= 16");
+    const int N = n / 3, Kc = mcc3_gather_k(cap), Npad = round_up(n, v3::BN);
     Carve cv(ws, ws_bytes);
     Split3 B;
     FC3_TRY(prep_phi3(s, phi, n, cv, B));
     const long n2 = (long)n * n;
-    const int Kp = padk(n);
-    const long P = (long)n * Kp;
     const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
-    __bf16* X = cv.take<__bf16>((size_t)2 * R * Kp);
+    __bf16* Bg = cv.take<__bf16>((size_t)2 * Kc * Npad * N);
+    __bf16* X = cv.take<__bf16>((size_t)2 * R * padk(n));
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "fused mcc3 workspace too small");
-    // mode 2 by slab batches: C(l n + p, q) = K3[p + n q + n^2 l]
-    const int nb = slab_batch3(n, size_t(1) << 62, kTilesPerLaunch);
-    for (int l0 = 0; l0 < n; l0 += nb) {
-        const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        const OutMap oT{k3 + n2 * l0, n, n2, 16, n, 0};
-        FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U + 2 * P * l0, P, n, 2 * P, 16L * n, B, oT,
-                      false, (long)cnt * n, n, Kp));
-    }
+    hipLaunchKernelGGL(k_gather_phi_nbr, dim3((unsigned)((Npad + 255) / 256), (unsigned)(Kc / 16),
+                                              (unsigned)N),
+                       dim3(256), 0, s, phi, n, count, nbr, cap, Kc, Npad, Bg);
+    FC3_HIP(hipGetLastError());
+    // mode 2, one product per atom K over its 3 slabs: C(g n + p, q) = K3[p + n q + n^2 (3K + g)]
+    // = sum_t U_K(g n + p, t) B_K(t, q)
+    const Split3 BK{Bg, (long)Kc * Npad, 16L * Npad};
+    const OutMap oT{k3, n, n2, 16, n, 0};
+    FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, 3L * n * Kc, 3L * n, 0, 16L * 3 * n, BK, oT, false,
+                  3L * n, n, Kc, 0, 3 * n2, N, 2L * 3 * n * Kc, 2L * Kc * Npad));
     // mode 3 in place by row blocks (each block split into X before the GEMM overwrites it)
     for (long r0 = 0; r0 < n2; r0 += R) {
         const long rows = (n2 - r0) < R ? (n2 - r0) : R;

@@ -127,11 +127,12 @@ int fc3_third_order_lj(fc3_handle h, const double* positions, int n_atoms,
 /* Fused third_order + mass weighting + mcc3 for LJ without the dense tensor: the
  * K3 = mcc3(Psi32, phi) that fc3_third_order_lj (psi32, masses) followed by fc3_mcc3
  * (FC3_PREC_BF16X3) gives, with the mode-1 product taken over Psi's nonzeros straight
- * from the pair blocks (FP32; the values are exactly those psi32 would hold) and
- * modes 2 and 3 on the BF16x3 kernels.  No counterpart in the reference; it follows
- * the sparse mode-1 intent of src/todo/old_mcc.jl:17-65.  Neither the FP64 nor the
- * FP32 n^3 Psi is formed; the workspace holds about one n^3 FP32 of split mode-1
- * result.
+ * from the pair blocks (FP32; the values are exactly those psi32 would hold), mode 2
+ * over each atom's neighbourhood columns (the only nonzero ones) and mode 3 on the
+ * BF16x3 kernels.  No counterpart in the reference; it follows the sparse mode-1
+ * intent of src/todo/old_mcc.jl:17-65.  Neither the FP64 nor the FP32 n^3 Psi is
+ * formed; the workspace holds 12 n^2 Kc bytes of mode-1 result, Kc = 3 (max
+ * neighbours + 1) rounded up to 16.
  *   phi: device n x n FP32;  k3: device n^3 FP32, 16-byte aligned;  masses: host or NULL.
  * Needs 3N % 4 == 0 and 3N >= 16 (FC3_ERR_UNSUPPORTED otherwise).                  */
 int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,

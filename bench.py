@@ -157,7 +157,7 @@ def batched_snapshots(ctx, per_size=32, replays=3):
     tune = {}
     for n in sorted(out_bufs):
         v, tiles, ms = fc3.autotune_query(n, "bf16x3", ctx=ctx)
-        tune[str(n)] = {"variant": v, "tiles": tiles, "mcc3_ms": ms}
+        tune[str(n)] = {"variant": v, "tiles": tiles, "snapshot_ms": ms}
     # error of the surviving outputs (last two per size) on sampled slabs
     import ctypes
     from fc3._lib import check

@@ -83,6 +83,24 @@ struct Params {
     int precision;
 };
 
+// Variant 4: the fused path (no n^3 Psi: sparse mode 1 from the pair blocks, mode 2
+// over each atom's neighbourhood columns, mode 3), for BF16X3 where it applies.
+constexpr int kVariantFused = 4;
+
+bool fused_applies(long n, int cap, int precision, const void* k3)
+{
+    return precision == FC3_PREC_BF16X3 && n % 4 == 0 && n >= 16 &&
+           lj_mode1_sparse_lds(cap) <= 64 * 1024 && (uintptr_t)k3 % 16 == 0;
+}
+
+// Scratch of the fused path: phi^T, the gathered mode-1 result, then modes 2-3.
+size_t fused_bytes(long n, int cap)
+{
+    return align256(sizeof(float) * (size_t)n * n) +
+           align256(sizeof(uint16_t) * 2 * 3 * (size_t)n * mcc3_gather_k(cap) * (n / 3)) +
+           mcc3_from_gathered_bytes((int)n, cap);
+}
+
 // Kernels of one snapshot on stream s (no synchronisation, no allocation).
 int snapshot_work(hipStream_t s, char* base, const SnapPlan& P, const fc3_snapshot& sn,
                   const Params& prm, float* psi32, void* mws, size_t mws_bytes)
@@ -103,6 +121,21 @@ int snapshot_work(hipStream_t s, char* base, const SnapPlan& P, const fc3_snapsh
     Ifc3Blocks b;
     b.E = E3;
     b.S = S3;
+    if (P.cfg.variant == kVariantFused) {
+        const int n = (int)P.n;
+        const int Kc = mcc3_gather_k(P.cap);
+        char* w = static_cast<char*>(mws);
+        float* phiT = reinterpret_cast<float*>(w);
+        w += align256(sizeof(float) * (size_t)n * n);
+        __bf16* U = reinterpret_cast<__bf16*>(w);
+        w += align256(sizeof(uint16_t) * 2 * 3 * (size_t)n * Kc * P.N);
+        const size_t used = (size_t)(w - static_cast<char*>(mws));
+        if (used > mws_bytes) return set_error(FC3_ERR_OUT_OF_MEMORY, "batch fused scratch");
+        FC3_TRY(transpose_square(s, sn.phi, n, phiT));
+        FC3_TRY(lj_mode1_gathered(s, nl, b, prm.tol, m, phiT, Kc, U));
+        return mcc3_from_gathered(s, U, sn.phi, n, nl.count, nl.nbr, P.cap, sn.k3, w,
+                                  mws_bytes - used);
+    }
     FC3_TRY(write_ifc3_pairwise(s, nl, b, prm.tol, m, nullptr, psi32));
     return mcc3_run(s, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &P.cfg);
 }
@@ -113,13 +146,12 @@ static int tune_size(fc3_context* h, BatchState* B, char* base, const SnapPlan& 
                      const fc3_snapshot& sn, const Params& prm, float* psi32, void* mws,
                      size_t mws_bytes)
 {
-    // assemble Psi once, then time each configuration's mcc3 (after one warm-up run)
-    SnapPlan Q = P;
-    Q.cfg = Mcc3Config{};
-    FC3_TRY(snapshot_work(h->stream, base, Q, sn, prm, psi32, mws, mws_bytes));
-    // candidates of the requested precision: the pre-split kernels at three launch sizes
-    // for BF16X3, else (FP32, or BF16X3 with n % 4 != 0) the FP32 kernels
+    // candidates of the requested precision: the fused path and the pre-split kernels at
+    // three launch sizes for BF16X3, else (FP32, or BF16X3 with n % 4 != 0) the FP32
+    // kernels; each is timed over the whole snapshot (assembly included) after a warm-up
     std::vector<Mcc3Config> cands;
+    if (fused_applies(P.n, P.cap, prm.precision, sn.k3))
+        cands.push_back(Mcc3Config{kVariantFused, 0});
     if (mcc3_variant_applies((int)P.n, prm.precision, 3))
         for (long t : {8192L, 16384L, 32768L}) cands.push_back(Mcc3Config{3, t});
     else
@@ -127,10 +159,14 @@ static int tune_size(fc3_context* h, BatchState* B, char* base, const SnapPlan& 
     BatchState::Tuned best;
     best.ms = -1.0f;
     for (const Mcc3Config& c : cands) {
-        if (mcc3_workspace_bytes((int)P.n, prm.precision, false, &c) > mws_bytes) continue;
-        FC3_TRY(mcc3_run(h->stream, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &c));
+        if (c.variant != kVariantFused &&
+            mcc3_workspace_bytes((int)P.n, prm.precision, false, &c) > mws_bytes)
+            continue;
+        SnapPlan Q = P;
+        Q.cfg = c;
+        FC3_TRY(snapshot_work(h->stream, base, Q, sn, prm, psi32, mws, mws_bytes));
         FC3_HIP(hipEventRecord(h->ev_start, h->stream));
-        FC3_TRY(mcc3_run(h->stream, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &c));
+        FC3_TRY(snapshot_work(h->stream, base, Q, sn, prm, psi32, mws, mws_bytes));
         FC3_HIP(hipEventRecord(h->ev_stop, h->stream));
         FC3_HIP(hipEventSynchronize(h->ev_stop));
         float ms = 0.0f;
@@ -236,6 +272,10 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
                 if (!mcc3_variant_applies((int)P[i].n, precision, v)) continue;
                 max_mws = std::max(max_mws, mcc3_workspace_bytes((int)P[i].n, precision, false, &c));
             }
+        if (fused_applies(P[i].n, P[i].cap, precision, snaps[i].k3))
+            max_mws = std::max(max_mws, fused_bytes(P[i].n, P[i].cap));
+        else if (P[i].cfg.variant == kVariantFused)
+            P[i].cfg = Mcc3Config{};   // a loaded table may name it where it does not apply
     }
     const size_t off_psi = off;
     off = align256(off + sizeof(float) * (size_t)max_n * max_n * max_n);
@@ -282,7 +322,12 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
             FC3_TRY(tune_size(h, B, base, P[i], snaps[i], prm, psi32, mws, max_mws));
         }
         h->prof.enabled = prof_was;
-        for (int i = 0; i < count; ++i) P[i].cfg = B->tuned[{(int)P[i].n, precision}].cfg;
+        for (int i = 0; i < count; ++i) {
+            P[i].cfg = B->tuned[{(int)P[i].n, precision}].cfg;
+            if (P[i].cfg.variant == kVariantFused &&
+                !fused_applies(P[i].n, P[i].cap, precision, snaps[i].k3))
+                P[i].cfg = Mcc3Config{};
+        }
     }
 
     if (flags & FC3_BATCH_NO_GRAPH) {
@@ -379,7 +424,9 @@ int fc3_autotune_load(fc3_handle h, const char* path)
         long tiles;
         float ms;
         if (std::sscanf(line, "%d %d %d %ld %f", &n, &prec, &var, &tiles, &ms) != 5) continue;
-        if (!mcc3_variant_applies(n, prec, var)) continue;
+        const bool fused_ok = var == kVariantFused && prec == FC3_PREC_BF16X3 && n % 4 == 0 &&
+                              n >= 16;
+        if (!fused_ok && !mcc3_variant_applies(n, prec, var)) continue;
         BatchState::Tuned t;
         t.cfg = Mcc3Config{var, tiles};
         t.ms = ms;

@@ -185,22 +185,24 @@ typedef struct {
 } fc3_snapshot;
 
 enum {
-    FC3_BATCH_AUTOTUNE = 1,    /* time the mcc3 configurations of sizes not seen before */
+    FC3_BATCH_AUTOTUNE = 1,    /* time the configurations of sizes not seen before */
     FC3_BATCH_NO_GRAPH = 2     /* issue the kernels directly instead of one graph launch */
 };
 
 /* For every snapshot: Psi = third_order(LJ, AnalyticalCalculator) mass weighted to FP32
- * (as fc3_third_order_lj's psi32), then k3 = mcc3(Psi, phi) at `precision`.  All
- * snapshots run from one captured graph (cached for an unchanged plan: same sizes,
- * pointers and configurations), with one shared Psi buffer of the largest size.
- * Returns once the work is enqueued.                                              */
+ * (as fc3_third_order_lj's psi32), then k3 = mcc3(Psi, phi) at `precision`: either
+ * through the dense Psi (one shared buffer of the largest size) or, for BF16X3 where
+ * it applies and the tuner picks it, by the fused path of fc3_third_order_mcc3_lj
+ * (no Psi).  All snapshots run from one captured graph (cached for an unchanged plan:
+ * same sizes, pointers and configurations).  Returns once the work is enqueued.    */
 int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int count,
                                   double sigma, double epsilon, double pot_r_cut,
                                   double r_cut, double tol, int precision, int flags);
 /* Nodes of the last captured batch graph. */
 int fc3_batch_graph_nodes(fc3_handle h, size_t* nodes);
-/* Tuning table: the chosen mcc3 configuration of size n (variant 2 = FP32 kernels,
- * 3 = pre-split BF16x3 kernels; tiles = workgroups per GEMM launch) and its time;
+/* Tuning table: the chosen configuration of size n (variant 2 = dense Psi + FP32
+ * kernels, 3 = dense Psi + pre-split BF16x3 kernels, 4 = the fused path; tiles =
+ * workgroups per GEMM launch) and the time of one snapshot (assembly included);
  * save/load as text lines "n precision variant tiles ms".                         */
 int fc3_autotune_query(fc3_handle h, int n, int precision, int* variant, long* tiles,
                        float* ms);

@@ -85,7 +85,7 @@ def test_autotune_table_roundtrip(ctx, tmp_path):
     ctx.synchronize()
     v, t, ms = fc3.autotune_query(768, "bf16x3", ctx=ctx)
     print(f"n=768 bf16x3: variant {v} tiles {t} {ms:.3f} ms", flush=True)
-    assert v in (2, 3) and ms > 0
+    assert v in (2, 3, 4) and ms > 0
     path = str(tmp_path / "tune.txt")
     fc3.autotune_save(path, ctx=ctx)
     other = fc3.Context(ctx.device)

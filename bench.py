@@ -131,7 +131,21 @@ def batched_snapshots(ctx, per_size=32, replays=3):
     rng = np.random.default_rng(96)
     out_bufs, snaps, meta = {}, [], []
     t0 = time.perf_counter()
+    modes = {}
     for cells in (3, 4, 5):
+        # phi on the host (IFC2 on the GPU, D and eigh on the host, as the reference does)
+        # against dynamical matrix + rocSOLVER eigensolver on the GPU, for one snapshot
+        sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, 0.05, seed=cells)
+        fc3.modes_device(sysd, S.ARGON_LJ, calc, ctx=ctx)      # warm-up (library load)
+        th = time.perf_counter()
+        fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
+        th = time.perf_counter() - th
+        td = time.perf_counter()
+        _, dphi_dev = fc3.modes_device(sysd, S.ARGON_LJ, calc, ctx=ctx)
+        ctx.synchronize()
+        td = time.perf_counter() - td
+        dphi_dev.free()
+        modes[str(sysd.n_dof)] = {"host_ms": th * 1e3, "device_ms": td * 1e3}
         for j in range(per_size):
             seed = int(rng.integers(1 << 30))
             su = float(np.random.default_rng(seed).uniform(0.02, 0.10))
@@ -186,7 +200,7 @@ def batched_snapshots(ctx, per_size=32, replays=3):
     res = {"snapshots": nsnap, "sizes_dof": sorted(out_bufs), "ms_per_batch": t,
            "snapshots_per_s": nsnap / (t * 1e-3), "K3_TFLOP_per_s": flop / (t * 1e-3) / 1e12,
            "graph_nodes": nodes, "replays_timed": replays, "autotuned": tune,
-           "errors_sampled": errs,
+           "errors_sampled": errs, "modes_setup_per_snapshot": modes,
            "note": "assembly (FP64 blocks, mass-weighted FP32 Psi) + BF16x3 mcc3 per snapshot, "
                    "one graph launch per batch"}
     log(f"batched: {res}")

@@ -203,6 +203,14 @@ static int get_blas(fc3_context* h, rocblas_handle* out)
     return FC3_OK;
 }
 
+int blas_handle(fc3_context* h, void** out)
+{
+    rocblas_handle b = nullptr;
+    FC3_TRY(get_blas(h, &b));
+    *out = b;
+    return FC3_OK;
+}
+
 size_t mcc3_backward_workspace_bytes(int n, int precision, bool need_dphi)
 {
     const size_t n3 = (size_t)n * n * n;

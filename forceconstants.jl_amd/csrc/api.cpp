@@ -431,6 +431,53 @@ int fc3_second_order_sw(fc3_handle h, const double* positions, int n_atoms, cons
                        P.extra, P.extra_bytes, tol, nullptr, nullptr, nullptr, ifc2);
 }
 
+int fc3_modes_lj(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                 double sigma, double epsilon, double pot_r_cut, double r_cut, double tol,
+                 const double* masses, int num_rigid_translation, double* freqs_sq,
+                 double* phi64, float* phi32)
+{
+    (void)pot_r_cut;
+    FC3_TRY(check_handle(h));
+    FC3_TRY(check_system(positions, n_atoms, box));
+    if (!masses || !freqs_sq) return set_error(FC3_ERR_ARGUMENT, "masses, freqs_sq must be non-null");
+    if (num_rigid_translation < 0 || num_rigid_translation > 3 * n_atoms)
+        return set_error(FC3_ERR_ARGUMENT, "num_rigid_translation out of range");
+    for (int i = 0; i < n_atoms; ++i)
+        if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    const int N = n_atoms, n = 3 * N;
+    Prepared P;
+    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
+        return Carve::need(sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * (27 + 9))) +
+               Carve::need(sizeof(double) * (size_t)n * n) + 2 * Carve::need(sizeof(double) * n) +
+               Carve::need(sizeof(int)) + 4096;
+    }, P));
+    Carve cv(P.extra, P.extra_bytes);
+    const size_t slots = (size_t)N * P.nl.cap;
+    double* E3 = cv.take<double>(slots * 27 + slots * 9 + (size_t)N * (27 + 9));
+    double* E2 = E3 + slots * 27;
+    double* S2 = E2 + slots * 9;
+    double* A = cv.take<double>((size_t)n * n);
+    double* W = cv.take<double>(n);
+    double* E = cv.take<double>(n);
+    int* info = cv.take<int>(1);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "modes workspace");
+    FC3_TRY(lj_pair_blocks(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, nullptr, E2, S2));
+    FC3_TRY(lj_write_ifc2(h->stream, P.nl, E2, S2, tol, A));
+    FC3_TRY(dynmat_eigen(h, A, n, P.masses, W, E, info, freqs_sq));
+    // get_modes (dynmat.jl:123-131): the num_rigid_translation eigenvalues of smallest
+    // magnitude (stable order, as sortperm) are set to zero; the vectors are untouched
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return std::fabs(freqs_sq[a]) < std::fabs(freqs_sq[b]); });
+    for (int r = 0; r < num_rigid_translation; ++r) freqs_sq[idx[r]] = 0.0;
+    if (phi64)
+        FC3_HIP(hipMemcpyAsync(phi64, A, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
+                               h->stream));
+    if (phi32) FC3_TRY(to_f32(h->stream, A, (long)n * n, phi32));
+    return FC3_OK;
+}
+
 int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
                        const double* sw_params, double r_cut, double tol, const double* masses,
                        double* psi64, float* psi32)

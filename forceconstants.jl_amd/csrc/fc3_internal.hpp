@@ -201,8 +201,18 @@ size_t mcc3_contract_bytes(int n);
 int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n, float* P,
                   int* zc, void* ws, size_t ws_bytes);
 
+// ---- dynamical matrix and modes (modes.hip) -------------------------------------------
+// A (device n x n FP64 IFC2) -> D = A / sqrt(m_I m_J) -> eigen(Hermitian(D)) in place:
+// eigenvalues ascending into W (device) and h_w (host), eigenvectors as A's columns.
+// E: n doubles, info: one int (device scratch).  Synchronises the stream.
+int dynmat_eigen(fc3_context* h, double* A, int n, const double* d_masses, double* W, double* E,
+                 int* info, double* h_w);
+int to_f32(hipStream_t s, const double* a, long cnt, float* b);
+
 // ---- adjoints (adjoint.hip) ---------------------------------------------------------
 void release_blas(fc3_context* h);
+// the context's rocBLAS handle (created on first use) on the context's stream
+int blas_handle(fc3_context* h, void** out);
 void release_batch(fc3_context* h);
 size_t mcc3_backward_workspace_bytes(int n, int precision, bool need_dphi);
 // dpsi (n^3) and/or dphi (n x n), device FP32; uses h's workspace (sized by the above).

@@ -8,7 +8,7 @@ from . import systems
 from .api import (autotune_load, autotune_query, autotune_save, batch_graph_nodes,
                   batch_third_order_mcc3, Context, DenseForceConstants, DeviceArray, dynamical_matrix, get_modes,
                   mass_weight_third_order_, mcc3, mcc3_, mcc3_backward, mcc3_rrule,
-                  second_order, second_order_device, third_order, third_order_device, third_order_mcc3,
+                  second_order, second_order_device, third_order, third_order_device, third_order_mcc3, modes_device,
                   third_order_param_backward)
 from .systems import (AnalyticalCalculator, AutoDiffCalculator, LJ, StillingerWeberSilicon,
                       SuperCellSystem, SWParams, diamond, fcc)
@@ -19,7 +19,7 @@ __all__ = [
     "Context", "DenseForceConstants", "DeviceArray", "dynamical_matrix", "get_modes",
     "mass_weight_third_order_", "mcc3", "mcc3_", "mcc3_backward", "mcc3_rrule",
     "third_order_param_backward", "second_order", "second_order_device",
-    "third_order", "third_order_device", "third_order_mcc3", "systems", "AnalyticalCalculator",
+    "third_order", "third_order_device", "third_order_mcc3", "modes_device", "systems", "AnalyticalCalculator",
     "AutoDiffCalculator", "LJ", "StillingerWeberSilicon", "SuperCellSystem", "SWParams",
     "diamond", "fcc",
 ]

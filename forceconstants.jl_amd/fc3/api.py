@@ -253,6 +253,24 @@ def dynamical_matrix(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = N
     return DenseForceConstants(np.asfortranarray(vals), ifc2.units + " / amu", 0.0)
 
 
+def modes_device(sys: SuperCellSystem, pot, calc, num_rigid_translation: int = 3,
+                 fp64: bool = False, ctx: Optional[Context] = None):
+    """dynamical_matrix + get_modes on the GPU (fc3_modes_lj; LJ with the analytical
+    calculator): (freqs_sq host array, device eigenvectors FP32, or FP64 if fp64)."""
+    ctx = ctx or Context.default()
+    if not (isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)):
+        raise NotImplementedError(f"modes_device for {type(pot).__name__} with {type(calc).__name__}")
+    n = sys.n_dof
+    pos, box = _sys_args(sys)
+    m = np.ascontiguousarray(sys.masses, dtype=np.float64)
+    w = np.empty(n, dtype=np.float64)
+    phi = DeviceArray((n, n), np.float64 if fp64 else np.float32, ctx)
+    check(ctx.lib.fc3_modes_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma, pot.epsilon,
+                               pot.r_cut, calc.r_cut, calc.tol, _ptr(m), num_rigid_translation,
+                               _ptr(w), phi.ptr if fp64 else None, None if fp64 else phi.ptr))
+    return w, phi
+
+
 def get_modes(dynmat, num_rigid_translation: int = 3):
     """dynmat.jl:123-131: FP64 LAPACK eigendecomposition of the Hermitian matrix;
     the num_rigid_translation eigenvalues of smallest |w^2| are set to zero."""

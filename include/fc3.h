@@ -109,6 +109,19 @@ int fc3_second_order_sw(fc3_handle h, const double* positions, int n_atoms,
                         const double box[3], const double* sw_params, double r_cut,
                         double tol, double* ifc2);
 
+/* dynamical_matrix(sys, LJ, AnalyticalCalculator) then get_modes(D, num_rigid) on the
+ * device (src/dynmat.jl:5-33, 123-131): D = IFC2 / sqrt(m_I m_J), eigen(Hermitian(D))
+ * by rocSOLVER dsyevd in FP64 (upper triangle, eigenvalues ascending), then the
+ * num_rigid eigenvalues of smallest magnitude set to zero (vectors untouched).
+ *   masses: host, n_atoms;  freqs_sq: host, n (the eigenvalues, zeroed as above);
+ *   phi64 / phi32: device n x n eigenvectors (columns = modes) in FP64 / FP32, or NULL.
+ * Eigenvectors of a degenerate eigenvalue span the same space as the reference's but
+ * may be another basis of it (as between any two LAPACK builds).                   */
+int fc3_modes_lj(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                 double sigma, double epsilon, double pot_r_cut, double r_cut, double tol,
+                 const double* masses, int num_rigid_translation, double* freqs_sq,
+                 double* phi64, float* phi32);
+
 /* ---- third order ------------------------------------------------------------ */
 /* third_order(sys, LJ, AnalyticalCalculator) replaces src/ifc.jl:20-35 ->
  * src/analytical/third_order_analytical.jl:4-69, src/asr.jl:19-40,

@@ -117,11 +117,12 @@ def cpu_baseline_assembly(reps=5):
                       f"median of {reps} runs after a warm-up: {dt:.2f} s"}
 
 
-def batched_snapshots(ctx, per_size=32, replays=3):
+def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
     """The batched configuration: 3 x `per_size` LJ argon snapshots of FCC 3x3x3,
     4x4x4 and 5x5x5 (324, 768, 1500 DoF), sigma_u ~ U[0.02, 0.10] A per snapshot
     (seeded), phi per snapshot from the host FP64 eigendecomposition of its dynamical
-    matrix; assembly + BF16x3 mcc3 of all snapshots in one graph launch.  K3 outputs
+    matrix (with modes_timing, the device modes are timed beside it for one snapshot per
+    size); assembly + BF16x3 mcc3 of all snapshots in one graph launch.  K3 outputs
     rotate through two buffers per size (the whole set would need 494 GB); the error
     is checked on sampled slabs of the last two snapshots of each size."""
     import fc3
@@ -132,31 +133,23 @@ def batched_snapshots(ctx, per_size=32, replays=3):
     out_bufs, snaps, meta = {}, [], []
     t0 = time.perf_counter()
     modes = {}
+    phi_s = 0.0
     for cells in (3, 4, 5):
-        # phi on the host (IFC2 on the GPU, D and eigh on the host, as the reference does)
-        # against dynamical matrix + rocSOLVER eigensolver on the GPU, for one snapshot
-        sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, 0.05, seed=cells)
-        fc3.modes_device(sysd, S.ARGON_LJ, calc, ctx=ctx)      # warm-up (library load)
-        th = time.perf_counter()
-        fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
-        th = time.perf_counter() - th
-        td = time.perf_counter()
-        _, dphi_dev = fc3.modes_device(sysd, S.ARGON_LJ, calc, ctx=ctx)
-        ctx.synchronize()
-        td = time.perf_counter() - td
-        dphi_dev.free()
-        modes[str(sysd.n_dof)] = {"host_ms": th * 1e3, "device_ms": td * 1e3}
+        if modes_timing:
+            modes[str(3 * 4 * cells ** 3)] = modes_setup_timing(ctx, cells, calc)
         for j in range(per_size):
             seed = int(rng.integers(1 << 30))
             su = float(np.random.default_rng(seed).uniform(0.02, 0.10))
             sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, su, seed=seed)
             n = sysd.n_dof
+            tp = time.perf_counter()
             _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
-            phi32 = np.asfortranarray(phi.astype(np.float32))
+            dph = fc3.DeviceArray.from_host(np.asfortranarray(phi.astype(np.float32)), ctx=ctx)
+            phi_s += time.perf_counter() - tp
             if n not in out_bufs:
                 out_bufs[n] = [fc3.DeviceArray((n, n, n), np.float32, ctx) for _ in range(2)]
-            snaps.append((sysd, fc3.DeviceArray.from_host(phi32, ctx=ctx), out_bufs[n][j % 2]))
-            meta.append((sysd, phi32, seed, su))
+            snaps.append((sysd, dph, out_bufs[n][j % 2]))
+            meta.append((sysd, dph, seed, su))
         log(f"batched: phi for {per_size} snapshots of FCC {cells}^3 ready "
             f"({time.perf_counter() - t0:.0f} s)")
     fc3.batch_third_order_mcc3(snaps, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)  # tune + capture
@@ -177,10 +170,11 @@ def batched_snapshots(ctx, per_size=32, replays=3):
     from fc3._lib import check
     errs = []
     for i in range(nsnap - 1, -1, -1):
-        sysd, phi32, seed, su = meta[i]
+        sysd, dph, seed, su = meta[i]
         n = sysd.n_dof
         if sum(1 for e in errs if e["n"] == n) >= 2:
             continue
+        phi32 = dph.to_host()
         coo = O.lj_third_order_coo(sysd, S.ARGON_LJ, calc)
         v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
         k3 = snaps[i][2]
@@ -197,14 +191,36 @@ def batched_snapshots(ctx, per_size=32, replays=3):
     for bufs in out_bufs.values():
         for b in bufs:
             b.free()
+    for m in meta:
+        m[1].free()
     res = {"snapshots": nsnap, "sizes_dof": sorted(out_bufs), "ms_per_batch": t,
            "snapshots_per_s": nsnap / (t * 1e-3), "K3_TFLOP_per_s": flop / (t * 1e-3) / 1e12,
            "graph_nodes": nodes, "replays_timed": replays, "autotuned": tune,
            "errors_sampled": errs, "modes_setup_per_snapshot": modes,
+           "phi_setup_s": phi_s,
            "note": "assembly (FP64 blocks, mass-weighted FP32 Psi) + BF16x3 mcc3 per snapshot, "
                    "one graph launch per batch"}
     log(f"batched: {res}")
     return res
+
+
+def modes_setup_timing(ctx, cells, calc):
+    """phi of one FCC cells^3 snapshot on the host (IFC2 on the GPU, D and eigh on the host,
+    as the reference does) against dynamical matrix + rocSOLVER eigensolver on the GPU."""
+    import fc3
+    from fc3 import systems as S
+    sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, 0.05, seed=cells)
+    _, d = fc3.modes_device(sysd, S.ARGON_LJ, calc, ctx=ctx)      # warm-up (library load)
+    d.free()
+    th = time.perf_counter()
+    fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
+    th = time.perf_counter() - th
+    td = time.perf_counter()
+    _, d = fc3.modes_device(sysd, S.ARGON_LJ, calc, ctx=ctx)
+    ctx.synchronize()
+    td = time.perf_counter() - td
+    d.free()
+    return {"host_ms": th * 1e3, "device_ms": td * 1e3}
 
 
 def adjoint_timing(ctx, cells=5, reps=2):
@@ -287,6 +303,8 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-snapshot section (96 snapshots of 324/768/1500 DoF)")
+    ap.add_argument("--modes-timing", action="store_true",
+                    help="time host against device normal modes for one snapshot per size")
     ap.add_argument("--no-fused", action="store_true",
                     help="skip the fused third_order + mcc3 (no dense Psi) timing")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -442,7 +460,7 @@ def main():
         log(f"fused: {out['fused']}")
     k3.free()
     if not a.no_batched:
-        out["batched"] = batched_snapshots(ctx)
+        out["batched"] = batched_snapshots(ctx, modes_timing=a.modes_timing)
     if not a.no_adjoint:
         out["adjoint"] = adjoint_timing(ctx)
     if not a.no_cpu_baseline:

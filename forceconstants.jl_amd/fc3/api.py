@@ -254,9 +254,11 @@ def dynamical_matrix(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = N
 
 
 def modes_device(sys: SuperCellSystem, pot, calc, num_rigid_translation: int = 3,
-                 fp64: bool = False, ctx: Optional[Context] = None):
+                 fp64: bool = False, out: Optional[DeviceArray] = None,
+                 ctx: Optional[Context] = None):
     """dynamical_matrix + get_modes on the GPU (fc3_modes_lj; LJ with the analytical
-    calculator): (freqs_sq host array, device eigenvectors FP32, or FP64 if fp64)."""
+    calculator): (freqs_sq host array, device eigenvectors FP32, or FP64 if fp64),
+    written into `out` when given."""
     ctx = ctx or Context.default()
     if not (isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)):
         raise NotImplementedError(f"modes_device for {type(pot).__name__} with {type(calc).__name__}")
@@ -264,7 +266,10 @@ def modes_device(sys: SuperCellSystem, pot, calc, num_rigid_translation: int = 3
     pos, box = _sys_args(sys)
     m = np.ascontiguousarray(sys.masses, dtype=np.float64)
     w = np.empty(n, dtype=np.float64)
-    phi = DeviceArray((n, n), np.float64 if fp64 else np.float32, ctx)
+    dt = np.float64 if fp64 else np.float32
+    if out is not None and (out.shape != (n, n) or out.dtype != dt):
+        raise AssertionError("modes_device: out must be n x n of the requested precision")
+    phi = out if out is not None else DeviceArray((n, n), dt, ctx)
     check(ctx.lib.fc3_modes_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma, pot.epsilon,
                                pot.r_cut, calc.r_cut, calc.tol, _ptr(m), num_rigid_translation,
                                _ptr(w), phi.ptr if fp64 else None, None if fp64 else phi.ptr))

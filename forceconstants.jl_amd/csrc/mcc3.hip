@@ -390,37 +390,19 @@ struct Shape<3> {
     static constexpr int WM = 4, WN = 1, TM = 2, TN = 4;
 };
 
+// One tile of shape SHAPE at (m0, n0) on the four LDS stage objects of the kernel
+// (passed in, so that after inlining every access still names its own object).
 template <bool SPLIT_OUT, int SHAPE>
-__global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
+__device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int n0, __bf16* s0,
+                                           __bf16* s1, __bf16* s2, __bf16* s3)
 {
     using SH = Shape<SHAPE>;
     constexpr int TBM = SH::WM * SH::TM * 32, TBN = SH::WN * SH::TN * 32;
     static_assert(TBM == BM, "A tile is 256 rows");
     constexpr int AE = TBM * BK, BE = TBN * BK;        // bf16 elements per plane
     constexpr int CA = TBM / 32, CB = TBN / 32;        // 64-chunk copies per plane
-    // Tile order: each XCD takes a contiguous range of tiles (workgroups are dealt to the
-    // 8 XCDs round robin), walked in groups of 8 row tiles column by column, so that the
-    // tiles in flight on one XCD share fewer A and B panels (measured: 20% fewer L2
-    // misses than row-major order, 50% fewer than no XCD mapping).
-    const long wg = blockIdx.x;
-    const long tile = (wg % kXcd) * g.per_xcd + wg / kXcd;
-    if (tile >= (long)g.tiles_m * g.tiles_n) return;
-    constexpr int GM = 8;
-    const long grp = tile / (GM * (long)g.tiles_n), first = grp * GM;
-    const int gsz = (int)((g.tiles_m - first) < GM ? (g.tiles_m - first) : GM);
-    const long w = tile - grp * GM * (long)g.tiles_n;
-    const int tm = (int)(first + w % gsz), tn = (int)(w / gsz);
-    const int m0 = tm * TBM, n0 = tn * TBN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm0 = (wave % SH::WM) * (SH::TM * 32), wn0 = (wave / SH::WM) * (SH::TN * 32);
-
-    // One LDS object per stage: the compiler then sees that the fragment reads of one
-    // stage do not alias the copies in flight into the others, and adds no wait.
-    // A stage holds A hi, A lo, B hi, B lo.
-    __shared__ __attribute__((aligned(16))) __bf16 s0[2 * AE + 2 * BE];
-    __shared__ __attribute__((aligned(16))) __bf16 s1[2 * AE + 2 * BE];
-    __shared__ __attribute__((aligned(16))) __bf16 s2[2 * AE + 2 * BE];
-    __shared__ __attribute__((aligned(16))) __bf16 s3[2 * AE + 2 * BE];
 
     // Copy plan: wave w fills plane w (0 A hi, 1 A lo, 2 B hi, 3 B lo) of a stage with
     // CA or CB copies of 64 chunks; chunk P = 64 q + lane of the plane (layout:
@@ -602,6 +584,44 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
                 }
             }
         }
+}
+
+// Elements of one LDS stage of the widest (256 x 256) tile: A hi, A lo, B hi, B lo.
+constexpr int kStageE = 2 * BM * BK + 2 * 256 * BK;
+
+// The GEMM over g's M x N in 256 x 256 tiles; with NARROW = 1..3 the last tile column is
+// a 256 x 32/64/128 tile of that shape (the N % 256 columns), in the same launch, so its
+// A panel is read while the neighbouring 256-wide tiles of its row group hold it in the
+// caches.
+template <bool SPLIT_OUT, int NARROW>
+__global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
+{
+    // Tile order: each XCD takes a contiguous range of tiles (workgroups are dealt to the
+    // 8 XCDs round robin), walked in groups of 8 row tiles column by column, so that the
+    // tiles in flight on one XCD share fewer A and B panels (measured: 20% fewer L2
+    // misses than row-major order, 50% fewer than no XCD mapping).
+    const long wg = blockIdx.x;
+    const long tile = (wg % kXcd) * g.per_xcd + wg / kXcd;
+    if (tile >= (long)g.tiles_m * g.tiles_n) return;
+    constexpr int GM = 8;
+    const long grp = tile / (GM * (long)g.tiles_n), first = grp * GM;
+    const int gsz = (int)((g.tiles_m - first) < GM ? (g.tiles_m - first) : GM);
+    const long w = tile - grp * GM * (long)g.tiles_n;
+    const int tm = (int)(first + w % gsz), tn = (int)(w / gsz);
+    const int m0 = tm * BM, n0 = tn * 256;
+    // One LDS object per stage: the compiler then sees that the fragment reads of one
+    // stage do not alias the copies in flight into the others, and adds no wait.
+    __shared__ __attribute__((aligned(16))) __bf16 s0[kStageE];
+    __shared__ __attribute__((aligned(16))) __bf16 s1[kStageE];
+    __shared__ __attribute__((aligned(16))) __bf16 s2[kStageE];
+    __shared__ __attribute__((aligned(16))) __bf16 s3[kStageE];
+    if constexpr (NARROW != 0) {
+        if (tn == g.tiles_n - 1) {
+            gemm3_tile<SPLIT_OUT, NARROW>(g, m0, n0, s0, s1, s2, s3);
+            return;
+        }
+    }
+    gemm3_tile<SPLIT_OUT, 0>(g, m0, n0, s0, s1, s2, s3);
 }
 
 // Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K) -> the K-blocked A
@@ -877,19 +897,21 @@ struct OutMap {
     long R, SS, KS, CS, lo;
 };
 
-// One k_gemm3 launch of tile shape SHAPE (tbn columns per tile) over g's M x N.
-template <bool SPLIT, int SHAPE>
-int launch3(hipStream_t s, int kind, v3::G3 g, int tbn)
+// One k_gemm3 launch over g's M x N: 256-wide tile columns, the last one narrow (NARROW =
+// 1..3: 32, 64, 128 columns) when it holds N % 256 <= 128 columns.
+template <bool SPLIT, int NARROW>
+int launch3(hipStream_t s, int kind, v3::G3 g)
 {
+    const int rem = g.N % 256;
     g.tiles_m = (int)((g.M + v3::BM - 1) / v3::BM);
-    g.tiles_n = (g.N + tbn - 1) / tbn;
+    g.tiles_n = NARROW ? (g.N - rem) / 256 + 1 : (g.N + 255) / 256;
     const long tiles = (long)g.tiles_m * g.tiles_n;
     g.per_xcd = (tiles + kXcd - 1) / kXcd;
     const long grid = g.per_xcd * kXcd;
     if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
     if (g.nz < 1 || g.nz > 65535) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid y out of range");
     ProfScope prof(kind, s);
-    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, SHAPE>), dim3((unsigned)grid, (unsigned)g.nz),
+    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, NARROW>), dim3((unsigned)grid, (unsigned)g.nz),
                        dim3(v3::NT), 0, s, g);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
@@ -898,24 +920,14 @@ int launch3(hipStream_t s, int kind, v3::G3 g, int tbn)
 template <bool SPLIT>
 int launch3_cols(hipStream_t s, int kind, const v3::G3& g)
 {
-    // full 256-wide column tiles, then the remaining N % 256 columns (<= 128) with a
-    // narrow tile; a remainder above 128 keeps a 256-wide tile
+    // the N % 256 last columns: a narrow last tile column for <= 128 of them, else (or
+    // none) all columns in 256-wide tiles
     const int rem = g.N % 256;
     const int shape = rem == 0 ? 0 : rem <= 32 ? 1 : rem <= 64 ? 2 : rem <= 128 ? 3 : 0;
-    const int nmain = shape == 0 ? g.N : g.N - rem;
-    if (nmain > 0) {
-        v3::G3 m = g;
-        m.N = nmain;
-        FC3_TRY((launch3<SPLIT, 0>(s, kind, m, 256)));
-    }
-    if (shape == 0) return FC3_OK;
-    v3::G3 r = g;
-    r.N = rem;
-    r.B = g.B + 16L * nmain;   // column c of B at 16 c inside a K block
-    r.C = static_cast<char*>(g.C) + (SPLIT ? sizeof(__bf16) : sizeof(float)) * nmain * g.c_CS;
-    if (shape == 1) return launch3<SPLIT, 1>(s, kind, r, 32);
-    if (shape == 2) return launch3<SPLIT, 2>(s, kind, r, 64);
-    return launch3<SPLIT, 3>(s, kind, r, 128);
+    if (shape == 1) return launch3<SPLIT, 1>(s, kind, g);
+    if (shape == 2) return launch3<SPLIT, 2>(s, kind, g);
+    if (shape == 3) return launch3<SPLIT, 3>(s, kind, g);
+    return launch3<SPLIT, 0>(s, kind, g);
 }
 
 // C(m, c) = sum_k A(m, k) phi(k, c); A(m, k) = A[(m / a_R) a_SS + (k / 16) a_KS + 16 (m % a_R)

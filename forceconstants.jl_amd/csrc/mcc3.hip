@@ -370,7 +370,8 @@ __device__ __forceinline__ void lds_copy16(const __bf16* g, __bf16* lds)
 // waves of 2 x 9 blocks, would fit n = 2592 = 9 x 288 without an almost empty last
 // column tile, but ran 20% slower: every wave then reads all B fragments.)
 // Narrow shapes 1-3 (256 x 32, 64, 128 as 4 x 1 waves of 2 x TN blocks) take the last
-// N % 256 columns in a launch of their own, so that no 256-wide tile is spent on them.
+// N % 256 columns as the last tile column of the launch, so that no 256-wide tile is
+// spent on them.
 template <int SHAPE>
 struct Shape;
 template <>

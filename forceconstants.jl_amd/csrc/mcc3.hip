@@ -598,13 +598,14 @@ template <bool SPLIT_OUT, int NARROW>
 __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
 {
     // Tile order: each XCD takes a contiguous range of tiles (workgroups are dealt to the
-    // 8 XCDs round robin), walked in groups of 8 row tiles column by column, so that the
-    // tiles in flight on one XCD share fewer A and B panels (measured: 20% fewer L2
-    // misses than row-major order, 50% fewer than no XCD mapping).
+    // 8 XCDs round robin), walked in groups of GM row tiles column by column, so that the
+    // tiles in flight on one XCD share A and B panels.  Measured at n = 2592 (11 tile
+    // columns, 32 CUs per XCD), random operands: GM = 3 670 ms per mcc3, 2 675, 1 678,
+    // 4 678, 8 686, 16 732, 32 760; the LJ workload, same GPU: GM = 3 625 ms, 8 630 ms.
     const long wg = blockIdx.x;
     const long tile = (wg % kXcd) * g.per_xcd + wg / kXcd;
     if (tile >= (long)g.tiles_m * g.tiles_n) return;
-    constexpr int GM = 8;
+    constexpr int GM = 3;
     const long grp = tile / (GM * (long)g.tiles_n), first = grp * GM;
     const int gsz = (int)((g.tiles_m - first) < GM ? (g.tiles_m - first) : GM);
     const long w = tile - grp * GM * (long)g.tiles_n;

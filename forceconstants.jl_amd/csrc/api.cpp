@@ -478,6 +478,51 @@ int fc3_modes_lj(fc3_handle h, const double* positions, int n_atoms, const doubl
     return FC3_OK;
 }
 
+int fc3_modes_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                 const double* sw_params, double r_cut, double tol, const double* masses,
+                 int num_rigid_translation, double* freqs_sq, double* phi64, float* phi32)
+{
+    FC3_TRY(check_handle(h));
+    FC3_TRY(check_system(positions, n_atoms, box));
+    if (!sw_params || !masses || !freqs_sq)
+        return set_error(FC3_ERR_ARGUMENT, "sw_params, masses, freqs_sq must be non-null");
+    if (!(r_cut <= sw_params[2] * sw_params[1]))
+        return set_error(FC3_ERR_ARGUMENT,
+                         "For SW silicon force constant cutoff must be less than potential cutoff");
+    if (num_rigid_translation < 0 || num_rigid_translation > 3 * n_atoms)
+        return set_error(FC3_ERR_ARGUMENT, "num_rigid_translation out of range");
+    for (int i = 0; i < n_atoms; ++i)
+        if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    const int N = n_atoms, n = 3 * N;
+    Prepared P;
+    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
+        return Carve::need(sw_workspace_bytes(N, cap, false)) +
+               Carve::need(sizeof(double) * (size_t)n * n) + 2 * Carve::need(sizeof(double) * n) +
+               Carve::need(sizeof(int)) + 4096;
+    }, P));
+    Carve cv(P.extra, P.extra_bytes);
+    const size_t swb = sw_workspace_bytes(N, P.nl.cap, false);
+    char* sw_ws = cv.take<char>(swb);
+    double* A = cv.take<double>((size_t)n * n);
+    double* W = cv.take<double>(n);
+    double* E = cv.take<double>(n);
+    int* info = cv.take<int>(1);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "modes workspace");
+    FC3_TRY(sw_assemble(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl, sw_ws,
+                        swb, tol, nullptr, nullptr, nullptr, A));
+    FC3_TRY(dynmat_eigen(h, A, n, P.masses, W, E, info, freqs_sq));
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return std::fabs(freqs_sq[a]) < std::fabs(freqs_sq[b]); });
+    for (int r = 0; r < num_rigid_translation; ++r) freqs_sq[idx[r]] = 0.0;
+    if (phi64)
+        FC3_HIP(hipMemcpyAsync(phi64, A, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
+                               h->stream));
+    if (phi32) FC3_TRY(to_f32(h->stream, A, (long)n * n, phi32));
+    return FC3_OK;
+}
+
 int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
                        const double* sw_params, double r_cut, double tol, const double* masses,
                        double* psi64, float* psi32)

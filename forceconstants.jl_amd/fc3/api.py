@@ -256,11 +256,13 @@ def dynamical_matrix(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = N
 def modes_device(sys: SuperCellSystem, pot, calc, num_rigid_translation: int = 3,
                  fp64: bool = False, out: Optional[DeviceArray] = None,
                  ctx: Optional[Context] = None):
-    """dynamical_matrix + get_modes on the GPU (fc3_modes_lj; LJ with the analytical
-    calculator): (freqs_sq host array, device eigenvectors FP32, or FP64 if fp64),
-    written into `out` when given."""
+    """dynamical_matrix + get_modes on the GPU (fc3_modes_lj / fc3_modes_sw: LJ with the
+    analytical calculator, SW with the autodiff one): (freqs_sq host array, device
+    eigenvectors FP32, or FP64 if fp64), written into `out` when given."""
     ctx = ctx or Context.default()
-    if not (isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)):
+    lj = isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)
+    sw = isinstance(pot, StillingerWeberSilicon) and isinstance(calc, AutoDiffCalculator)
+    if not (lj or sw):
         raise NotImplementedError(f"modes_device for {type(pot).__name__} with {type(calc).__name__}")
     n = sys.n_dof
     pos, box = _sys_args(sys)
@@ -270,9 +272,15 @@ def modes_device(sys: SuperCellSystem, pot, calc, num_rigid_translation: int = 3
     if out is not None and (out.shape != (n, n) or out.dtype != dt):
         raise AssertionError("modes_device: out must be n x n of the requested precision")
     phi = out if out is not None else DeviceArray((n, n), dt, ctx)
-    check(ctx.lib.fc3_modes_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma, pot.epsilon,
-                               pot.r_cut, calc.r_cut, calc.tol, _ptr(m), num_rigid_translation,
-                               _ptr(w), phi.ptr if fp64 else None, None if fp64 else phi.ptr))
+    p64, p32 = (phi.ptr, None) if fp64 else (None, phi.ptr)
+    if lj:
+        check(ctx.lib.fc3_modes_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
+                                   pot.epsilon, pot.r_cut, calc.r_cut, calc.tol, _ptr(m),
+                                   num_rigid_translation, _ptr(w), p64, p32))
+    else:
+        prm = pot.params.as_abi_array()
+        check(ctx.lib.fc3_modes_sw(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), _ptr(prm), calc.r_cut,
+                                   calc.tol, _ptr(m), num_rigid_translation, _ptr(w), p64, p32))
     return w, phi
 
 

@@ -122,6 +122,12 @@ int fc3_modes_lj(fc3_handle h, const double* positions, int n_atoms, const doubl
                  const double* masses, int num_rigid_translation, double* freqs_sq,
                  double* phi64, float* phi32);
 
+/* The same for StillingerWeberSilicon with the AutoDiffCalculator (IFC2 as
+ * fc3_second_order_sw, sw_params as there).                                        */
+int fc3_modes_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
+                 const double* sw_params, double r_cut, double tol, const double* masses,
+                 int num_rigid_translation, double* freqs_sq, double* phi64, float* phi32);
+
 /* ---- third order ------------------------------------------------------------ */
 /* third_order(sys, LJ, AnalyticalCalculator) replaces src/ifc.jl:20-35 ->
  * src/analytical/third_order_analytical.jl:4-69, src/asr.jl:19-40,

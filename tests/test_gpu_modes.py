@@ -59,3 +59,25 @@ def test_modes_device_without_rigid_zeroing(ctx):
     w, _ = fc3.modes_device(sysd, LJ, calc, num_rigid_translation=0, ctx=ctx)
     assert np.all(np.diff(w) >= 0)
     assert np.max(np.abs(w - lam)) <= 1e-10 * np.abs(lam).max()
+
+
+@pytest.mark.parametrize("cells,su,seed", [((2, 2, 2), 0.03, 65), ((3, 3, 3), 0.03, 648)])
+def test_modes_device_sw_match_host(ctx, cells, su, seed):
+    pot = S.StillingerWeberSilicon()
+    calc = S.AutoDiffCalculator(1e-12, pot.r_cut)
+    sysd = S.diamond(S.SI_A, cells, S.SI_MASS, su, seed=seed)
+    n = sysd.n_dof
+    D = O.dynamical_matrix(O.sw_second_order(sysd, pot, calc), sysd.masses)
+    w_ref, _ = O.get_modes(D)
+    lam = np.linalg.eigvalsh(D)
+    w, phi = fc3.modes_device(sysd, pot, calc, fp64=True, ctx=ctx)
+    V = phi.to_host()
+    scale = float(np.abs(lam).max())
+    dw = float(np.max(np.abs(w - w_ref)))
+    res = float(np.linalg.norm(D @ V - V * lam) / np.linalg.norm(D))
+    print(f"SW diamond {cells} n={n}: max|dw| {dw:.2e} (scale {scale:.2e}), residual {res:.2e}",
+          flush=True)
+    assert dw <= 1e-10 * scale
+    assert np.array_equal(w == 0.0, w_ref == 0.0)
+    assert res <= 1e-12 * n
+    assert float(np.linalg.norm(V.T @ V - np.eye(n))) <= 1e-12 * n

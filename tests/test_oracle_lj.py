@@ -227,3 +227,30 @@ def test_coo_oracle_matches_dense():
         dense[i, j, k] = v
         assert len(v) == np.count_nonzero(psi)
         assert np.array_equal(dense, psi)
+
+
+def test_threaded_oracle_is_bit_identical_to_one_thread():
+    """The pair loop and ASR! are threaded over atoms (OpenMP, as the reference's @tasks
+    loops); every entry is still written by one iteration and every ASR sum keeps its
+    order, so the tensor must not depend on the thread count."""
+    import hashlib
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys, hashlib; sys.path[:0] = [%r, %r]\n"
+            "from fc3 import systems as S\nfrom oracle import fc3_oracle as O\n"
+            "s = S.fcc(S.ARGON_A, (3, 3, 3), S.ARGON_MASS, 0.05, seed=5)\n"
+            "c = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)\n"
+            "p = O.lj_third_order(s, S.ARGON_LJ, c)\n"
+            "print(O.lib().fco_num_threads(), hashlib.sha1(p.tobytes(order='F')).hexdigest())\n"
+            % (root, os.path.join(root, "forceconstants.jl_amd")))
+    out = {}
+    for t in ("1", "4"):
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                           timeout=300, check=True)
+        threads, digest = r.stdout.split()
+        assert threads == t
+        out[t] = digest
+    assert out["1"] == out["4"]

@@ -186,7 +186,7 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
 // z < *zc <= min(64, n)
 // (the caller sums them).  form 0: A(a, e) = X[a + n e], B(e, b) = H[b + n e];
 // form 1: A(a, e) = X[e + n^2 a], B(e, b) = H[e + n^2 b].  Applies for n % 4 == 0,
-// 16 <= n and n^2 / 64 < 65535 (aligned tensors).
+// 16 <= n (aligned tensors).
 // Modes 2 and 3 of mcc3 on the BF16X3 kernels from the gathered mode-1 result U of
 // lj_mode1_gathered: mode 2 as one batched product per atom over its neighbourhood's
 // columns (gathered phi rows), mode 3 in place; K3 FP32 in k3 (n % 4 == 0, n >= 16).

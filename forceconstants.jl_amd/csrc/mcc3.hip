@@ -628,43 +628,34 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
 
 // Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K) -> the K-blocked A
 // operand with K padded to Kp (a multiple of 16, zeros for k >= K):
-// X[(k / 16) 16 rows + 16 (r - r0) + k % 16] (hi), lo at + rows Kp.  64 rows x 64 k per
-// block through LDS; each thread then writes 4 consecutive k of one row, so a K block's
-// 64 rows go out as 2 KB contiguous.
+// X[(k / 16) 16 rows + 16 (r - r0) + k % 16] (hi), lo at + rows Kp.  Block (K block
+// blockIdx.x, 256 rows blockIdx.y): thread r loads its row's 16 values (each load 1 KB
+// contiguous over the block) and stores them as two 32-byte runs per plane, so the
+// block writes 8 KB contiguous per plane and needs no LDS.
 __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict__ psi, long n2,
                                                           int K, int Kp, long r0, long rows,
                                                           __bf16* __restrict__ X)
 {
-    __shared__ float tile[64][65];
-    const long rb = (long)blockIdx.x * 64;
-    const int kb = blockIdx.y * 64;
-    const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
-    for (int kk = ty; kk < 64; kk += 4) {
-        const int k = kb + kk;
-        const long r = rb + tx;
-        tile[kk][tx] = (k < K && r < rows) ? psi[r0 + r + n2 * k] : 0.0f;
-    }
-    __syncthreads();
-    const int rr = t >> 2, quad = t & 3;
-    const long r = rb + rr;
+    const int kb = blockIdx.x;
+    const long r = (long)blockIdx.y * 256 + threadIdx.x;
     if (r >= rows) return;
-    const long lo_off = rows * Kp;
+    const int k0 = 16 * kb;
+    bf16x8 h0, h1, l0, l1;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int k = kb + 16 * b + 4 * quad;
-        if (k >= Kp) break;
-        bf16x4 h, l;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            __bf16 hh, ll;
-            split_bf16(tile[16 * b + 4 * quad + e][rr], hh, ll);
-            h[e] = hh;
-            l[e] = ll;
-        }
-        const long off = (long)(k >> 4) * 16 * rows + 16 * r + (k & 15);
-        *reinterpret_cast<bf16x4*>(X + off) = h;
-        *reinterpret_cast<bf16x4*>(X + lo_off + off) = l;
+    for (int j = 0; j < 16; ++j) {
+        const int k = k0 + j;
+        const float x = k < K ? psi[r0 + r + n2 * k] : 0.0f;
+        __bf16 hh, ll;
+        split_bf16(x, hh, ll);
+        if (j < 8) { h0[j] = hh; l0[j] = ll; } else { h1[j - 8] = hh; l1[j - 8] = ll; }
     }
+    const long off = (long)kb * 16 * rows + 16 * r;
+    bf16x8* dh = reinterpret_cast<bf16x8*>(X + off);
+    bf16x8* dl = reinterpret_cast<bf16x8*>(X + rows * Kp + off);
+    dh[0] = h0;
+    dh[1] = h1;
+    dl[0] = l0;
+    dl[1] = l1;
 }
 
 // cnt slabs T_l (n x n, T_l[i + n j] at src[l n^2 + i + n j]) -> the K-blocked A operand
@@ -998,7 +989,7 @@ int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, cons
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
-    dim3 gt((unsigned)((rows + 63) / 64), (unsigned)((Kp + 63) / 64));
+    dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
     hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows, X);
     FC3_HIP(hipGetLastError());
     return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, o, split_out,
@@ -1282,8 +1273,7 @@ int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, int n, 
 
 bool mcc3_contract_applies(int n)
 {
-    // the row-contiguous split runs n^2 / 64 blocks in grid y
-    return v3_applies(n, FC3_PREC_BF16X3) && (long)n * n / 64 + 1 <= 65535;
+    return v3_applies(n, FC3_PREC_BF16X3);
 }
 
 // one split operand: 2 n^3 bf16, plus 16 Npad of slack that the column tiles past n read
@@ -1307,7 +1297,7 @@ int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n
         __bf16* dst = t == 0 ? SA : SB;
         if (form == 0) {
             // rows r < n, K index e < n^2 at stride n: src[r + n e]
-            dim3 gt((unsigned)((n + 63) / 64), (unsigned)((n2 + 63) / 64));
+            dim3 gt((unsigned)(n2 / 16), (unsigned)((n + 255) / 256));
             hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, src, (long)n, (int)n2,
                                (int)n2, 0L, (long)n, dst);
         } else {

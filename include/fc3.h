@@ -234,9 +234,9 @@ int fc3_autotune_load(fc3_handle h, const char* path);
  *   dphi = dL/dphi, the three mode terms                  (device n x n, or NULL; needs psi)
  * psi, phi, dk3 device FP32.  With dphi the work is 20 n^4 flops (18 without dpsi) in
  * two n^3 FP32 buffers of workspace: for FC3_PREC_BF16X3 with n % 4 == 0, n >= 16 and
- * 16-byte aligned tensors the seven mode products and (for n^2 / 64 < 65535) the
- * three n x n^2 x n contractions run on the BF16x3 kernels, with about two more n^3
- * of split scratch; otherwise they are FP32 GEMMs.  dpsi alone runs the mcc3 kernels
+ * 16-byte aligned tensors the seven mode products and the three n x n^2 x n
+ * contractions run on the BF16x3 kernels, with about two more n^3 of split scratch;
+ * otherwise they are FP32 GEMMs.  dpsi alone runs the mcc3 kernels
  * at `precision`.                                                                    */
 int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const float* dk3,
                       int n, int precision, float* dpsi, float* dphi);

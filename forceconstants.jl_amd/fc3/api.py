@@ -204,9 +204,13 @@ def third_order_mcc3(sys: SuperCellSystem, pot, calc, phi: DeviceArray, masses=N
     n = sys.n_dof
     if phi.shape != (n, n):
         raise AssertionError("third_order_mcc3: phi must be n x n")
+    if out is not None and (out.shape != (n, n, n) or out.dtype != np.float32):
+        raise AssertionError("third_order_mcc3: out must be n x n x n float32")
     k3 = out if out is not None else DeviceArray((n, n, n), np.float32, ctx)
     pos, box = _sys_args(sys)
     m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    if m is not None and m.shape != (sys.n_atoms,):
+        raise AssertionError("third_order_mcc3: masses must hold one value per atom")
     check(ctx.lib.fc3_third_order_mcc3_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
                                           pot.epsilon, pot.r_cut, calc.r_cut, calc.tol,
                                           _ptr(m) if m is not None else None, phi.ptr, k3.ptr))
@@ -314,6 +318,8 @@ def mcc3(psi: DeviceArray, phi: DeviceArray, block_size: Optional[int] = None,
     if psi.shape != (n, n, n) or phi.shape != (n, n):
         raise AssertionError("mcc3: Psi must be n x n x n and phi n x n")
     p = _prec(precision)
+    if out is not None and (out.shape != (n, n, n) or out.dtype != np.float32):
+        raise AssertionError("mcc3: out must be n x n x n float32")
     if block_size is None:
         out = out or DeviceArray((n, n, n), np.float32, ctx)
         check(ctx.lib.fc3_mcc3(ctx.h, psi.ptr, phi.ptr, n, p, out.ptr))

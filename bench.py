@@ -50,6 +50,15 @@ def profile_enable(ctx, on=True):
     check(ctx.lib.fc3_profile_enable(ctx.h, 1 if on else 0))
 
 
+def spread_slabs(w2, n, count):
+    """`count` K3 output slabs spread evenly over the spectrum, leaving out the three
+    rigid-translation modes (the smallest |omega^2|), whose slabs vanish by the acoustic
+    sum rule and so carry no information about the contraction error."""
+    rigid = set(np.argsort(np.abs(w2), kind="stable")[:3].tolist())
+    cand = np.unique(np.linspace(0, n - 1, count + 6).round().astype(int))
+    return [int(l) for l in cand if l not in rigid][:count]
+
+
 def gemm_traffic():
     """HBM bytes per k_gemm3 launch at this workload, from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy kernel), or
@@ -143,13 +152,13 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
             sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, su, seed=seed)
             n = sysd.n_dof
             tp = time.perf_counter()
-            _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
+            w2, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
             dph = fc3.DeviceArray.from_host(np.asfortranarray(phi.astype(np.float32)), ctx=ctx)
             phi_s += time.perf_counter() - tp
             if n not in out_bufs:
                 out_bufs[n] = [fc3.DeviceArray((n, n, n), np.float32, ctx) for _ in range(2)]
             snaps.append((sysd, dph, out_bufs[n][j % 2]))
-            meta.append((sysd, dph, seed, su))
+            meta.append((sysd, dph, seed, su, w2))
         log(f"batched: phi for {per_size} snapshots of FCC {cells}^3 ready "
             f"({time.perf_counter() - t0:.0f} s)")
     fc3.batch_third_order_mcc3(snaps, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)  # tune + capture
@@ -170,7 +179,7 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
     from fc3._lib import check
     errs = []
     for i in range(nsnap - 1, -1, -1):
-        sysd, dph, seed, su = meta[i]
+        sysd, dph, seed, su, w2 = meta[i]
         n = sysd.n_dof
         if sum(1 for e in errs if e["n"] == n) >= 2:
             continue
@@ -180,7 +189,7 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
         k3 = snaps[i][2]
         num = den = 0.0
         buf = np.empty((n, n), dtype=np.float32, order="F")
-        for l in (0, n // 2, n - 1):
+        for l in spread_slabs(w2, n, 8):
             ref = O.mcc3_fp64_slab((coo[0], coo[1], coo[2], v32), phi32.astype(np.float64), l)
             check(ctx.lib.fc3_memcpy_d2h(ctx.h, buf.ctypes.data_as(ctypes.c_void_p),
                                          ctypes.c_void_p(k3.ptr.value + 4 * n * n * l), 4 * n * n))
@@ -310,6 +319,11 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+backward timing at FCC 5x5x5 (1500 DoF)")
     a = ap.parse_args()
+    if a.modes_timing:
+        print("--modes-timing is disabled: a profiled run of device modes followed by the batch "
+              "aborted the GPU queue, and that sequence stays off until its cause is fixed",
+              file=sys.stderr)
+        sys.exit(2)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {a.gpus} requested)", file=sys.stderr)
@@ -328,7 +342,7 @@ def main():
 
     # phi: IFC2 on the GPU, FP64 eigendecomposition of the dynamical matrix on the host
     dyn = fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx)
-    _, phi = fc3.get_modes(dyn)
+    w2, phi = fc3.get_modes(dyn)
     phi32 = np.asfortranarray(phi.astype(np.float32))
     dphi = fc3.DeviceArray.from_host(phi32, ctx=ctx)
     log("phi ready")
@@ -362,7 +376,7 @@ def main():
     v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
     coo32 = (coo[0], coo[1], coo[2], v32)
     phi64 = phi32.astype(np.float64)
-    slabs = [0, n // 2 + 7, n - 1]
+    slabs = spread_slabs(w2, n, 16)
     ref = {l: O.mcc3_fp64_slab(coo32, phi64, l) for l in slabs}
     log("oracle slabs ready")
     flop = 6.0 * n ** 4

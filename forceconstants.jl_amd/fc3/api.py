@@ -129,6 +129,30 @@ class DenseForceConstants:
         return self.values.shape
 
 
+def _check_buf(arr, shape, dtype, name: str, ctx: Optional[Context] = None) -> DeviceArray:
+    """AssertionError unless `arr` is a live DeviceArray of exactly this shape and dtype
+    (on ctx's device when ctx is given).  The kernels write through the pointer as given,
+    so a smaller or differently typed buffer would be written out of bounds; every
+    caller-supplied buffer passes through here before any library call."""
+    if not isinstance(arr, DeviceArray) or not arr.ptr:
+        raise AssertionError(f"{name} must be a live DeviceArray")
+    shape = tuple(int(x) for x in shape)
+    if arr.shape != shape or arr.dtype != np.dtype(dtype):
+        raise AssertionError(f"{name} must be {shape} {np.dtype(dtype).name}, "
+                             f"got {arr.shape} {arr.dtype.name}")
+    if ctx is not None and arr.ctx.device != ctx.device:
+        raise AssertionError(f"{name} lives on device {arr.ctx.device}, the call on {ctx.device}")
+    return arr
+
+
+def _check_masses(masses, n_atoms: int, name: str = "masses") -> np.ndarray:
+    """Host masses as n_atoms contiguous doubles (the C side reads exactly that many)."""
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    if m.shape != (int(n_atoms),):
+        raise AssertionError(f"{name} must hold one value per atom ({n_atoms}), got shape {m.shape}")
+    return m
+
+
 def _units(pot, order: int) -> str:
     e, l = ("kcal/mol", "Angstrom") if isinstance(pot, LJ) else ("eV", "Angstrom")
     return f"{e} / {l}^{order}"
@@ -144,7 +168,8 @@ def second_order_device(sys: SuperCellSystem, pot, calc, out: Optional[DeviceArr
                         ctx: Optional[Context] = None) -> DeviceArray:
     ctx = ctx or Context.default()
     n = sys.n_dof
-    out = out or DeviceArray((n, n), np.float64, ctx)
+    out = _check_buf(out, (n, n), np.float64, "out", ctx) if out is not None else \
+        DeviceArray((n, n), np.float64, ctx)
     pos, box = _sys_args(sys)
     if isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator):
         st = ctx.lib.fc3_second_order_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
@@ -175,7 +200,14 @@ def third_order_device(sys: SuperCellSystem, pot, calc, psi64: Optional[DeviceAr
     `masses` is given) on the device."""
     ctx = ctx or Context.default()
     pos, box = _sys_args(sys)
-    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    n = sys.n_dof
+    if psi64 is None and psi32 is None:
+        raise AssertionError("third_order_device: psi64 and psi32 are both None")
+    if psi64 is not None:
+        _check_buf(psi64, (n, n, n), np.float64, "psi64", ctx)
+    if psi32 is not None:
+        _check_buf(psi32, (n, n, n), np.float32, "psi32", ctx)
+    m = None if masses is None else _check_masses(masses, sys.n_atoms)
     mp = _ptr(m) if m is not None else None
     p64 = psi64.ptr if psi64 is not None else None
     p32 = psi32.ptr if psi32 is not None else None
@@ -202,15 +234,12 @@ def third_order_mcc3(sys: SuperCellSystem, pot, calc, phi: DeviceArray, masses=N
     if not (isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)):
         raise NotImplementedError(f"third_order_mcc3 for {type(pot).__name__} with {type(calc).__name__}")
     n = sys.n_dof
-    if phi.shape != (n, n):
-        raise AssertionError("third_order_mcc3: phi must be n x n")
-    if out is not None and (out.shape != (n, n, n) or out.dtype != np.float32):
-        raise AssertionError("third_order_mcc3: out must be n x n x n float32")
+    _check_buf(phi, (n, n), np.float32, "third_order_mcc3: phi", ctx)
+    if out is not None:
+        _check_buf(out, (n, n, n), np.float32, "third_order_mcc3: out", ctx)
     k3 = out if out is not None else DeviceArray((n, n, n), np.float32, ctx)
     pos, box = _sys_args(sys)
-    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
-    if m is not None and m.shape != (sys.n_atoms,):
-        raise AssertionError("third_order_mcc3: masses must hold one value per atom")
+    m = None if masses is None else _check_masses(masses, sys.n_atoms)
     check(ctx.lib.fc3_third_order_mcc3_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
                                           pot.epsilon, pot.r_cut, calc.r_cut, calc.tol,
                                           _ptr(m) if m is not None else None, phi.ptr, k3.ptr))
@@ -233,15 +262,20 @@ def mass_weight_third_order_(ifc3, masses, ctx: Optional[Context] = None):
     sqrt(m_i m_j m_k) in place.  Accepts a DeviceArray (FP64) or a
     DenseForceConstants whose host values are weighted on the GPU."""
     ctx = ctx or Context.default()
-    m = np.ascontiguousarray(masses, dtype=np.float64)
+    m = np.ascontiguousarray(masses, dtype=np.float64).reshape(-1)
+    n = ifc3.shape[0]
+    # D = Int(N_modes / length(masses)); @assert D ∈ [1,2,3] (third_order_analytical.jl:186-190)
+    dim, rem = divmod(n, max(len(m), 1))
+    if len(m) == 0 or rem or dim not in (1, 2, 3):
+        raise AssertionError("D ∈ [1, 2, 3]")
     if isinstance(ifc3, DeviceArray):
-        n = ifc3.shape[0]
-        dim = n // len(m)
+        _check_buf(ifc3, (n, n, n), np.float64, "mass_weight_third_order!: Psi", ctx)
         check(ctx.lib.fc3_mass_weight_third_order(ctx.h, ifc3.ptr, len(m), dim, _ptr(m)))
         return ifc3
+    if tuple(ifc3.shape) != (n, n, n):
+        raise AssertionError("mass_weight_third_order!: Psi must be n x n x n")
     d = DeviceArray.from_host(np.asfortranarray(ifc3.values, dtype=np.float64), ctx=ctx)
-    n = d.shape[0]
-    check(ctx.lib.fc3_mass_weight_third_order(ctx.h, d.ptr, len(m), n // len(m), _ptr(m)))
+    check(ctx.lib.fc3_mass_weight_third_order(ctx.h, d.ptr, len(m), dim, _ptr(m)))
     ifc3.values = d.to_host()
     d.free()
     ifc3.units = ifc3.units + " / amu"
@@ -270,11 +304,11 @@ def modes_device(sys: SuperCellSystem, pot, calc, num_rigid_translation: int = 3
         raise NotImplementedError(f"modes_device for {type(pot).__name__} with {type(calc).__name__}")
     n = sys.n_dof
     pos, box = _sys_args(sys)
-    m = np.ascontiguousarray(sys.masses, dtype=np.float64)
+    m = _check_masses(sys.masses, sys.n_atoms)
     w = np.empty(n, dtype=np.float64)
     dt = np.float64 if fp64 else np.float32
-    if out is not None and (out.shape != (n, n) or out.dtype != dt):
-        raise AssertionError("modes_device: out must be n x n of the requested precision")
+    if out is not None:
+        _check_buf(out, (n, n), dt, "modes_device: out", ctx)
     phi = out if out is not None else DeviceArray((n, n), dt, ctx)
     p64, p32 = (phi.ptr, None) if fp64 else (None, phi.ptr)
     if lj:
@@ -315,11 +349,11 @@ def mcc3(psi: DeviceArray, phi: DeviceArray, block_size: Optional[int] = None,
     n = phi.shape[0]
     if psi.dtype != np.float32 or phi.dtype != np.float32:
         raise TypeError("mcc3 takes Float32 device arrays")
-    if psi.shape != (n, n, n) or phi.shape != (n, n):
-        raise AssertionError("mcc3: Psi must be n x n x n and phi n x n")
+    _check_buf(psi, (n, n, n), np.float32, "mcc3: Psi")
+    _check_buf(phi, (n, n), np.float32, "mcc3: phi", ctx)
     p = _prec(precision)
-    if out is not None and (out.shape != (n, n, n) or out.dtype != np.float32):
-        raise AssertionError("mcc3: out must be n x n x n float32")
+    if out is not None:
+        _check_buf(out, (n, n, n), np.float32, "mcc3: out", ctx)
     if block_size is None:
         out = out or DeviceArray((n, n, n), np.float32, ctx)
         check(ctx.lib.fc3_mcc3(ctx.h, psi.ptr, phi.ptr, n, p, out.ptr))
@@ -334,8 +368,8 @@ def mcc3_(psi: DeviceArray, phi: DeviceArray, precision="fp32") -> DeviceArray:
     """mcc3!(Psi, phi): K3 overwrites Psi; returns the same DeviceArray."""
     ctx = psi.ctx
     n = phi.shape[0]
-    if psi.shape != (n, n, n) or phi.shape != (n, n):
-        raise AssertionError("mcc3!: Psi must be n x n x n and phi n x n")
+    _check_buf(psi, (n, n, n), np.float32, "mcc3!: Psi")
+    _check_buf(phi, (n, n), np.float32, "mcc3!: phi", ctx)
     check(ctx.lib.fc3_mcc3_inplace(ctx.h, psi.ptr, phi.ptr, n, _prec(precision)))
     return psi
 
@@ -348,8 +382,9 @@ def mcc3_backward(psi: DeviceArray, phi: DeviceArray, dk3: DeviceArray, precisio
     returns (dL/dPsi or None, dL/dphi or None) as device arrays."""
     ctx = psi.ctx
     n = phi.shape[0]
-    if psi.shape != (n, n, n) or dk3.shape != (n, n, n) or phi.shape != (n, n):
-        raise AssertionError("mcc3 backward: Psi, dK3 must be n x n x n and phi n x n")
+    _check_buf(psi, (n, n, n), np.float32, "mcc3 backward: Psi")
+    _check_buf(phi, (n, n), np.float32, "mcc3 backward: phi", ctx)
+    _check_buf(dk3, (n, n, n), np.float32, "mcc3 backward: dK3", ctx)
     gp = DeviceArray((n, n, n), np.float32, ctx) if dpsi else None
     gf = DeviceArray((n, n), np.float32, ctx) if dphi else None
     check(ctx.lib.fc3_mcc3_backward(ctx.h, psi.ptr, phi.ptr, dk3.ptr, n, _prec(precision),
@@ -378,9 +413,10 @@ def third_order_param_backward(sys: SuperCellSystem, pot, calc, dpsi: DeviceArra
     ctx = ctx or dpsi.ctx
     pos, L = _sys_args(sys)
     n = sys.n_dof
-    if dpsi.shape != (n, n, n):
-        raise AssertionError("dpsi must be 3N x 3N x 3N")
-    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    if not isinstance(dpsi, DeviceArray) or dpsi.dtype not in (np.float32, np.float64):
+        raise TypeError("dpsi must be a float32 or float64 DeviceArray")
+    _check_buf(dpsi, (n, n, n), dpsi.dtype, "dpsi", ctx)
+    m = None if masses is None else _check_masses(masses, sys.n_atoms)
     if dpsi.dtype == np.float32:
         d32, d64 = dpsi.ptr, None
     elif dpsi.dtype == np.float64:
@@ -422,12 +458,10 @@ def batch_third_order_mcc3(snapshots, pot, calc, precision="bf16x3", autotune: b
     keep = []
     for i, (sys, phi, k3) in enumerate(snaps):
         n = sys.n_dof
-        if phi.shape != (n, n) or k3.shape != (n, n, n):
-            raise AssertionError(f"snapshot {i}: phi must be n x n and k3 n x n x n (n = {n})")
-        if phi.dtype != np.float32 or k3.dtype != np.float32:
-            raise TypeError("phi and k3 must be float32 device arrays")
+        _check_buf(phi, (n, n), np.float32, f"snapshot {i}: phi", ctx)
+        _check_buf(k3, (n, n, n), np.float32, f"snapshot {i}: k3", ctx)
         pos, L = _sys_args(sys)
-        m = np.ascontiguousarray(sys.masses, dtype=np.float64)
+        m = _check_masses(sys.masses, sys.n_atoms)
         keep += [pos, m]
         arr[i].positions = _ptr(pos)
         arr[i].n_atoms = sys.n_atoms

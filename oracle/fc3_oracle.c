@@ -327,19 +327,25 @@ int fco_lj_ifc3_columns(const double* pos, int N, const double* L, double sigma,
 }
 
 /* mass_weight_third_order!, third_order_analytical.jl:185-214 (D = 3). */
-void fco_mass_weight3(double* psi, int N, const double* m)
+/* mass_weight_third_order! (third_order_analytical.jl:185-214) for D = n / N in 1:3:
+ * ii = D*i + a, jj = D*j + b, kk = D*k + c; nonzero entries divided by
+ * sqrt(m_i m_j m_k), the product taken left to right as the reference writes it. */
+void fco_mass_weight3_d(double* psi, int N, int D, const double* m)
 {
-    const long n = 3L * N, n2 = n * n;
+    const long n = (long)D * N, n2 = n * n;
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < N; ++j)
             for (int k = 0; k < N; ++k)
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b)
-                        for (int c = 0; c < 3; ++c) {
-                            long idx = (3L * i + a) + n * (3L * j + b) + n2 * (3L * k + c);
+                for (int a = 0; a < D; ++a)
+                    for (int b = 0; b < D; ++b)
+                        for (int c = 0; c < D; ++c) {
+                            long idx = ((long)D * i + a) + n * ((long)D * j + b) +
+                                       n2 * ((long)D * k + c);
                             if (psi[idx] != 0) psi[idx] /= sqrt((m[i] * m[j]) * m[k]);
                         }
 }
+
+void fco_mass_weight3(double* psi, int N, const double* m) { fco_mass_weight3_d(psi, N, 3, m); }
 
 /* ========================================================================= */
 /* Stillinger-Weber.  Derivative blocks are computed by order-3 hyper-dual

@@ -56,6 +56,8 @@ def lib():
                                           ctypes.c_double, ctypes.c_int, _i64p, _i64p, _dp]
         L.fco_mass_weight3.argtypes = [_dp, ctypes.c_int, _dp]
         L.fco_mass_weight3.restype = None
+        L.fco_mass_weight3_d.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.fco_mass_weight3_d.restype = None
         L.fco_sw_blocks3.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
         L.fco_sw_blocks3.restype = None
         L.fco_sw_blocks2.argtypes = [_dp, _dp, ctypes.c_double, _dp]
@@ -161,10 +163,14 @@ def coo_mass_weight(coo, masses):
 
 
 def mass_weight_third_order(psi: np.ndarray, masses) -> np.ndarray:
-    """mass_weight_third_order! on a Fortran-ordered FP64 array (in place)."""
+    """mass_weight_third_order! on a Fortran-ordered FP64 array (in place), with
+    D = size(psi, 1) / length(masses) in 1:3 as the reference asserts."""
     assert psi.flags.f_contiguous and psi.dtype == np.float64
     m = np.ascontiguousarray(masses, dtype=np.float64)
-    lib().fco_mass_weight3(_p(psi), len(m), _p(m))
+    D, rem = divmod(psi.shape[0], len(m))
+    if rem or D not in (1, 2, 3):
+        raise AssertionError("D ∈ [1, 2, 3]")
+    lib().fco_mass_weight3_d(_p(psi), len(m), D, _p(m))
     return psi
 
 

@@ -94,3 +94,57 @@ def test_autotune_table_roundtrip(ctx, tmp_path):
         assert fc3.autotune_query(768, "bf16x3", ctx=other)[:2] == (v, t)
     finally:
         other.close()
+
+
+def _spread_slabs(w2, n, count):
+    """`count` output slabs spread over the spectrum, the three rigid modes left out."""
+    rigid = set(np.argsort(np.abs(w2), kind="stable")[:3].tolist())
+    cand = np.unique(np.linspace(0, n - 1, count + 6).round().astype(int))
+    return [int(l) for l in cand if l not in rigid][:count]
+
+
+@pytest.mark.parametrize("variant", [4, 3])
+def test_batch_forced_variant_timed_sizes_match_fp64(ctx, tmp_path, variant):
+    """The sizes of the timed batch (FCC 3x3x3, 4x4x4, 5x5x5: 324, 768, 1500 DoF) with the
+    configuration forced through a loaded tuning table instead of the tuner's choice:
+    variant 4 (the fused path, no dense Psi) and variant 3 (dense Psi, pre-split BF16x3
+    kernels).  Each snapshot's K3 is compared with the FP64 contraction of the oracle's
+    mass-weighted FP32 Psi on 12 slabs spread over the spectrum (rigid modes left out)."""
+    spec = [((3, 3, 3), 31), ((4, 4, 4), 32), ((5, 5, 5), 33)]
+    table = tmp_path / "forced.txt"
+    table.write_text("".join(f"{3 * 4 * c[0] ** 3} 1 {variant} 16384 1.0\n" for c, _ in spec))
+    other = fc3.Context(ctx.device)
+    try:
+        fc3.autotune_load(str(table), ctx=other)
+        snaps, modes = [], []
+        for cells, seed in spec:
+            su = float(np.random.default_rng(seed).uniform(0.02, 0.10))
+            sysd = S.fcc(S.ARGON_A, cells, S.ARGON_MASS, su, seed=seed)
+            f2 = O.lj_second_order(sysd, LJ, CALC)
+            w2, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+            phi32 = np.asfortranarray(phi.astype(np.float32))
+            n = sysd.n_dof
+            snaps.append((sysd, fc3.DeviceArray.from_host(phi32, ctx=other),
+                          fc3.DeviceArray((n, n, n), np.float32, other)))
+            modes.append((w2, phi32))
+        fc3.batch_third_order_mcc3(snaps, LJ, CALC, precision="bf16x3", autotune=False, ctx=other)
+        other.synchronize()
+        for (sysd, dphi, k3), (w2, phi32), (cells, seed) in zip(snaps, modes, spec):
+            n = sysd.n_dof
+            assert fc3.autotune_query(n, "bf16x3", ctx=other)[0] == variant
+            coo = O.lj_third_order_coo(sysd, LJ, CALC)
+            v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
+            coo32 = (coo[0], coo[1], coo[2], v32)
+            got = k3.to_host()
+            num = den = 0.0
+            for l in _spread_slabs(w2, n, 12):
+                ref = O.mcc3_fp64_slab(coo32, phi32.astype(np.float64), l)
+                num += float(np.sum((got[:, :, l].astype(np.float64) - ref) ** 2))
+                den += float(np.sum(ref * ref))
+            err = (num / den) ** 0.5
+            print(f"forced variant {variant}: FCC {cells} (n={n}) rel fro err {err:.3e}", flush=True)
+            assert err <= TOL["bf16x3"]
+            dphi.free()
+            k3.free()
+    finally:
+        other.close()

@@ -254,3 +254,27 @@ def test_threaded_oracle_is_bit_identical_to_one_thread():
         assert threads == t
         out[t] = digest
     assert out["1"] == out["4"]
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+def test_oracle_mass_weight_dims(D):
+    """mass_weight_third_order! for D = n / N_atoms in 1:3 (third_order_analytical.jl:
+    185-214): unit masses leave Psi unchanged, equal masses m scale nonzeros by m^-1.5,
+    zeros stay zero, and the atom of index ii is ii // D (a per-atom mass pattern)."""
+    N = 5
+    n = D * N
+    rng = np.random.default_rng(D)
+    psi = rng.standard_normal((n, n, n))
+    psi[np.abs(psi) < 0.5] = 0.0
+    psi = np.asfortranarray(psi)
+    assert np.array_equal(O.mass_weight_third_order(psi.copy(order="F"), np.ones(N)), psi)
+    got = O.mass_weight_third_order(psi.copy(order="F"), np.full(N, 4.0))
+    assert np.allclose(got, psi / 8.0, rtol=1e-15, atol=0)
+    m = np.array([1.0, 4.0, 9.0, 16.0, 25.0])
+    got = O.mass_weight_third_order(psi.copy(order="F"), m)
+    idx = np.arange(n) // D
+    s = np.sqrt(m)[idx]
+    assert np.allclose(got, psi / (s[:, None, None] * s[None, :, None] * s[None, None, :]),
+                       rtol=1e-14, atol=0)
+    with pytest.raises(AssertionError):
+        O.mass_weight_third_order(psi.copy(order="F"), np.ones(N + 1))

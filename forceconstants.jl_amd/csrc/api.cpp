@@ -183,7 +183,11 @@ int fc3_destroy(fc3_handle h)
     hipStreamSynchronize(h->stream);
     release_blas(h);
     release_batch(h);
+    if (h->aux) hipStreamSynchronize(h->aux);
     if (h->ws) hipFree(h->ws);
+    for (auto e : h->aux_ev)
+        if (e) hipEventDestroy(e);
+    if (h->aux) hipStreamDestroy(h->aux);
     for (auto& r : h->prof.records) {
         hipEventDestroy(r.start);
         hipEventDestroy(r.stop);
@@ -402,15 +406,17 @@ int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int di
                                 const double* masses)
 {
     FC3_TRY(check_handle(h));
-    if (dim < 1 || dim > 3) return set_error(FC3_ERR_ARGUMENT, "D must be in 1:3");
-    if (dim != 3) return set_error(FC3_ERR_UNSUPPORTED, "only D = 3 is supported");
+    // @assert D ∈ [1,2,3] (third_order_analytical.jl:190)
+    if (dim < 1 || dim > 3) return set_error(FC3_ERR_ARGUMENT, "D ∈ [1, 2, 3]");
     if (!psi64 || !masses || n_atoms < 1)
         return set_error(FC3_ERR_ARGUMENT, "psi64/masses must be non-null, n_atoms >= 1");
+    if ((long long)n_atoms * dim > 65535)
+        return set_error(FC3_ERR_UNSUPPORTED, "D*n_atoms must be <= 65535");
     FC3_TRY(ensure_workspace(h, sizeof(double) * n_atoms + 256));
     double* dm = static_cast<double*>(h->ws);
     FC3_HIP(hipMemcpyAsync(dm, masses, sizeof(double) * n_atoms, hipMemcpyHostToDevice,
                            h->stream));
-    FC3_TRY(mass_weight3(h->stream, psi64, n_atoms, dm));
+    FC3_TRY(mass_weight3(h->stream, psi64, n_atoms, dim, dm));
     return FC3_OK;
 }
 
@@ -445,11 +451,13 @@ int fc3_modes_lj(fc3_handle h, const double* positions, int n_atoms, const doubl
     for (int i = 0; i < n_atoms; ++i)
         if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
     const int N = n_atoms, n = 3 * N;
+    size_t sv = 0;
+    FC3_TRY(dsyevd_workspace_bytes(h, n, &sv));
     Prepared P;
     FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
         return Carve::need(sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * (27 + 9))) +
                Carve::need(sizeof(double) * (size_t)n * n) + 2 * Carve::need(sizeof(double) * n) +
-               Carve::need(sizeof(int)) + 4096;
+               Carve::need(sizeof(int)) + Carve::need(sv) + 4096;
     }, P));
     Carve cv(P.extra, P.extra_bytes);
     const size_t slots = (size_t)N * P.nl.cap;
@@ -460,10 +468,11 @@ int fc3_modes_lj(fc3_handle h, const double* positions, int n_atoms, const doubl
     double* W = cv.take<double>(n);
     double* E = cv.take<double>(n);
     int* info = cv.take<int>(1);
+    char* svw = cv.take<char>(sv);
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "modes workspace");
     FC3_TRY(lj_pair_blocks(h->stream, P.pos, box, sigma, epsilon, P.nl, E3, nullptr, E2, S2));
     FC3_TRY(lj_write_ifc2(h->stream, P.nl, E2, S2, tol, A));
-    FC3_TRY(dynmat_eigen(h, A, n, P.masses, W, E, info, freqs_sq));
+    FC3_TRY(dynmat_eigen(h, A, n, P.masses, W, E, info, freqs_sq, svw, sv));
     // get_modes (dynmat.jl:123-131): the num_rigid_translation eigenvalues of smallest
     // magnitude (stable order, as sortperm) are set to zero; the vectors are untouched
     std::vector<int> idx(n);
@@ -494,11 +503,13 @@ int fc3_modes_sw(fc3_handle h, const double* positions, int n_atoms, const doubl
     for (int i = 0; i < n_atoms; ++i)
         if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
     const int N = n_atoms, n = 3 * N;
+    size_t sv = 0;
+    FC3_TRY(dsyevd_workspace_bytes(h, n, &sv));
     Prepared P;
     FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
         return Carve::need(sw_workspace_bytes(N, cap, false)) +
                Carve::need(sizeof(double) * (size_t)n * n) + 2 * Carve::need(sizeof(double) * n) +
-               Carve::need(sizeof(int)) + 4096;
+               Carve::need(sizeof(int)) + Carve::need(sv) + 4096;
     }, P));
     Carve cv(P.extra, P.extra_bytes);
     const size_t swb = sw_workspace_bytes(N, P.nl.cap, false);
@@ -507,10 +518,11 @@ int fc3_modes_sw(fc3_handle h, const double* positions, int n_atoms, const doubl
     double* W = cv.take<double>(n);
     double* E = cv.take<double>(n);
     int* info = cv.take<int>(1);
+    char* svw = cv.take<char>(sv);
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "modes workspace");
     FC3_TRY(sw_assemble(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl, sw_ws,
                         swb, tol, nullptr, nullptr, nullptr, A));
-    FC3_TRY(dynmat_eigen(h, A, n, P.masses, W, E, info, freqs_sq));
+    FC3_TRY(dynmat_eigen(h, A, n, P.masses, W, E, info, freqs_sq, svw, sv));
     std::vector<int> idx(n);
     for (int i = 0; i < n; ++i) idx[i] = i;
     std::stable_sort(idx.begin(), idx.end(),
@@ -582,30 +594,64 @@ int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n, in
     FC3_TRY(check_handle(h));
     if (!psi || !phi || !k3_host)
         return set_error(FC3_ERR_ARGUMENT, "psi, phi, k3_host must be non-null");
+    // @assert size(phi)[1] % block_size == 0 && block_size > 0  (mcc.jl:91-92)
     if (block_size <= 0 || n % block_size != 0)
         return set_error(FC3_ERR_BLOCK_SIZE, "size(phi)[1] % block_size == 0 && block_size > 0");
-    // Compute the full K3 on the device, then stream it to the host slab by slab
-    // (block_size slabs of n^2 floats per copy).
-    const size_t bytes = sizeof(float) * (size_t)n * n * n;
-    float* k3 = nullptr;
-    hipError_t e = hipMalloc(&k3, bytes);
-    if (e != hipSuccess)
-        return set_error(FC3_ERR_OUT_OF_MEMORY, std::string("mcc3 blocked: ") + hipGetErrorString(e));
-    int st = fc3_mcc3(h, psi, phi, n, precision, k3);
-    if (st == FC3_OK) {
-        const size_t slab = sizeof(float) * (size_t)n * n * block_size;
-        for (int l0 = 0; l0 < n && st == FC3_OK; l0 += block_size) {
-            e = hipMemcpyAsync(reinterpret_cast<char*>(k3_host) + slab / block_size * l0,
-                               reinterpret_cast<char*>(k3) + slab / block_size * l0, slab,
-                               hipMemcpyDeviceToHost, h->stream);
-            if (e != hipSuccess) st = set_error(FC3_ERR_HIP, hipGetErrorString(e));
-        }
-        e = hipStreamSynchronize(h->stream);
-        if (st == FC3_OK && e != hipSuccess) st = set_error(FC3_ERR_HIP, hipGetErrorString(e));
+    if (precision != FC3_PREC_FP32 && precision != FC3_PREC_BF16X3)
+        return set_error(FC3_ERR_ARGUMENT, "unknown precision");
+    if ((long long)n * n >= (1LL << 31))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: n^2 must be < 2^31");
+    // The reference's block_size bounds device memory (mcc.jl:85).  Here the device holds
+    // two blocks of bs output slabs (one being computed, one being copied to the host)
+    // plus bounded scratch, never the whole K3.  bs is a multiple of block_size of about
+    // 4 GiB per block (fewer passes over Psi), block_size itself when that is larger.
+    const size_t slab = sizeof(float) * (size_t)n * n;
+    const long bmem = std::max<long>(1, (long)((size_t(4) << 30) / slab));
+    long bsl = block_size >= bmem ? block_size : (bmem / block_size) * block_size;
+    const int bs = (int)std::min<long>(bsl, n);
+    const bool v3 = mcc3_blocked_v3(n, precision, psi);
+    const size_t blk = Carve::need(slab * bs);
+    const size_t scratch = mcc3_blocked_bytes(n, v3, bs);
+    FC3_TRY(ensure_workspace(h, 2 * blk + scratch + 1024));
+    if (!h->aux) {
+        FC3_HIP(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+        for (auto& e : h->aux_ev) FC3_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    hipStreamSynchronize(h->stream);
-    hipFree(k3);
-    return st;
+    Carve cv(h->ws, h->ws_bytes);
+    float* dbuf[2] = {cv.take<float>((size_t)n * n * bs), cv.take<float>((size_t)n * n * bs)};
+    char* sw = cv.take<char>(scratch);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 blocked workspace");
+    hipEvent_t* computed = h->aux_ev;       // block in dbuf[b] computed
+    hipEvent_t* copied = h->aux_ev + 2;     // dbuf[b] copied to the host
+    const int nblk = (n + bs - 1) / bs;
+    int st = FC3_OK;
+    hipError_t e = hipSuccess;
+    for (int t = 0; t <= nblk && st == FC3_OK && e == hipSuccess; ++t) {
+        if (t < nblk) {   // compute block t into dbuf[t % 2] once its previous copy is done
+            const int b = t % 2, l0 = t * bs, cnt = std::min(bs, n - l0);
+            if (t >= 2) e = hipStreamWaitEvent(h->stream, copied[b], 0);
+            if (e == hipSuccess)
+                st = mcc3_blocked_slabs(h->stream, psi, phi, n, v3, bs, l0, cnt, dbuf[b], sw,
+                                        scratch);
+            if (st == FC3_OK && e == hipSuccess) e = hipEventRecord(computed[b], h->stream);
+        }
+        if (t >= 1 && st == FC3_OK && e == hipSuccess) {   // copy block t - 1 out
+            const int b = (t - 1) % 2, l0 = (t - 1) * bs, cnt = std::min(bs, n - l0);
+            e = hipStreamWaitEvent(h->aux, computed[b], 0);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(k3_host + (size_t)n * n * l0, dbuf[b], slab * cnt,
+                                   hipMemcpyDeviceToHost, h->aux);
+            if (e == hipSuccess) e = hipEventRecord(copied[b], h->aux);
+        }
+    }
+    const hipError_t e1 = hipStreamSynchronize(h->stream);
+    const hipError_t e2 = hipStreamSynchronize(h->aux);
+    if (st != FC3_OK) return st;
+    for (hipError_t x : {e, e1, e2})
+        if (x != hipSuccess)
+            return set_error(x == hipErrorOutOfMemory ? FC3_ERR_OUT_OF_MEMORY : FC3_ERR_HIP,
+                             std::string("mcc3 blocked: ") + hipGetErrorString(x));
+    return FC3_OK;
 }
 
 int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const float* dk3, int n,

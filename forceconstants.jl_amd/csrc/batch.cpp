@@ -86,6 +86,16 @@ struct Params {
 // Variant 4: the fused path (no n^3 Psi: sparse mode 1 from the pair blocks, mode 2
 // over each atom's neighbourhood columns, mode 3), for BF16X3 where it applies.
 constexpr int kVariantFused = 4;
+// Launch sizes (workgroups per GEMM launch) the tuner tries for the dense kernels; 0 is
+// the library default.  A loaded table may name only these.
+constexpr long kTileCands[] = {8192L, 16384L, 32768L};
+bool tiles_known(long t)
+{
+    if (t == 0) return true;
+    for (long c : kTileCands)
+        if (t == c) return true;
+    return false;
+}
 
 bool fused_applies(long n, int cap, int precision, const void* k3)
 {
@@ -153,7 +163,7 @@ static int tune_size(fc3_context* h, BatchState* B, char* base, const SnapPlan& 
     if (fused_applies(P.n, P.cap, prm.precision, sn.k3))
         cands.push_back(Mcc3Config{kVariantFused, 0});
     if (mcc3_variant_applies((int)P.n, prm.precision, 3))
-        for (long t : {8192L, 16384L, 32768L}) cands.push_back(Mcc3Config{3, t});
+        for (long t : kTileCands) cands.push_back(Mcc3Config{3, t});
     else
         cands.push_back(Mcc3Config{2, 0});
     BatchState::Tuned best;
@@ -267,11 +277,15 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
         if (it != B->tuned.end()) P[i].cfg = it->second.cfg;
         // size the mcc3 workspace for every candidate a tuning run may try
         for (int v : {2, 3})
-            for (long t : {8192L, 16384L, 32768L}) {
+            for (long t : kTileCands) {
                 Mcc3Config c{v, t};
                 if (!mcc3_variant_applies((int)P[i].n, precision, v)) continue;
                 max_mws = std::max(max_mws, mcc3_workspace_bytes((int)P[i].n, precision, false, &c));
             }
+        // and for the configuration this snapshot will actually run
+        if (P[i].cfg.variant != kVariantFused)
+            max_mws = std::max(max_mws,
+                               mcc3_workspace_bytes((int)P[i].n, precision, false, &P[i].cfg));
         if (fused_applies(P[i].n, P[i].cap, precision, snaps[i].k3))
             max_mws = std::max(max_mws, fused_bytes(P[i].n, P[i].cap));
         else if (P[i].cfg.variant == kVariantFused)
@@ -427,6 +441,12 @@ int fc3_autotune_load(fc3_handle h, const char* path)
         const bool fused_ok = var == kVariantFused && prec == FC3_PREC_BF16X3 && n % 4 == 0 &&
                               n >= 16;
         if (!fused_ok && !mcc3_variant_applies(n, prec, var)) continue;
+        if (!tiles_known(tiles)) {
+            std::fclose(f);
+            return set_error(FC3_ERR_ARGUMENT, std::string(path) + ": tiles " +
+                                                   std::to_string(tiles) +
+                                                   " is not a launch size the tuner uses");
+        }
         BatchState::Tuned t;
         t.cfg = Mcc3Config{var, tiles};
         t.ms = ms;

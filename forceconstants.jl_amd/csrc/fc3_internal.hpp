@@ -54,6 +54,10 @@ struct fc3_context {
     // Workspace: reserved (size computed per call), grown with a stream sync.
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    // Copy stream and its events (created on first use by fc3_mcc3_blocked): device ->
+    // host copies of one block overlap the computation of the next.
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace fc3 {
@@ -122,7 +126,8 @@ struct Ifc3Blocks {
 int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b,
                         double tol, const double* d_masses, double* psi64, float* psi32);
 
-int mass_weight3(hipStream_t s, double* psi, int N, const double* d_masses);
+// Psi[ii, jj, kk] /= sqrt((m[ii / D] m[jj / D]) m[kk / D]) where nonzero, n = D N.
+int mass_weight3(hipStream_t s, double* psi, int N, int D, const double* d_masses);
 
 // Sparse mode-1 product fused to the pair assembly (no dense Psi): for every atom K
 // and its three slabs kk = 3K + g, U_K(g n + p, t) = sum_i phiT[p + n i] Psi32[i, jj_t, kk]
@@ -154,7 +159,7 @@ int sw_param_grad(hipStream_t s, const double* d_pos, const double box[3], const
 
 // ---- mcc3 (mcc3.hip) --------------------------------------------------------------
 // Kernel family and launch sizing of one mcc3: variant 2 = FP32-operand kernels (any n),
-// 3 = pre-split BF16x3 kernels (BF16X3 with n % 16 == 0), 0 = default (3 when it
+// 3 = pre-split BF16x3 kernels (BF16X3 with n % 4 == 0, n >= 16), 0 = default (3 when it
 // applies); tiles = target workgroups per GEMM launch (0 = default).
 struct Mcc3Config {
     int variant = 0;
@@ -194,6 +199,14 @@ int mcc3_gather_k(int cap);
 size_t mcc3_from_gathered_bytes(int n, int cap);
 int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, int n, const int* count,
                        const int* nbr, int cap, float* k3, void* ws, size_t ws_bytes);
+// Bounded-memory mcc3 (fc3_mcc3_blocked): K3 slabs [l0, l0 + cnt), cnt <= bs, into `out`
+// (n^2 cnt floats, device) from the whole Psi, bitwise equal to the same slabs of
+// mcc3_run.  v3 = mcc3_blocked_v3(n, precision, psi) selects the BF16X3 kernels (else
+// FP32); the workspace (mcc3_blocked_bytes) holds phi's prepared copy and bounded scratch.
+bool mcc3_blocked_v3(int n, int precision, const void* psi);
+size_t mcc3_blocked_bytes(int n, bool v3, int bs);
+int mcc3_blocked_slabs(hipStream_t s, const float* psi, const float* phi, int n, bool v3, int bs,
+                       int l0, int cnt, float* out, void* ws, size_t ws_bytes);
 // out[j + n i] = in[i + n j] (device, n x n)
 int transpose_square(hipStream_t s, const float* in, int n, float* out);
 bool mcc3_contract_applies(int n);
@@ -205,8 +218,11 @@ int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n
 // A (device n x n FP64 IFC2) -> D = A / sqrt(m_I m_J) -> eigen(Hermitian(D)) in place:
 // eigenvalues ascending into W (device) and h_w (host), eigenvectors as A's columns.
 // E: n doubles, info: one int (device scratch).  Synchronises the stream.
+// solver_ws: rocSOLVER's scratch (dsyevd_workspace_bytes), attached to the rocBLAS handle
+// for this call only.
+int dsyevd_workspace_bytes(fc3_context* h, int n, size_t* bytes);
 int dynmat_eigen(fc3_context* h, double* A, int n, const double* d_masses, double* W, double* E,
-                 int* info, double* h_w);
+                 int* info, double* h_w, void* solver_ws, size_t solver_ws_bytes);
 int to_f32(hipStream_t s, const double* a, long cnt, float* b);
 
 // ---- adjoints (adjoint.hip) ---------------------------------------------------------

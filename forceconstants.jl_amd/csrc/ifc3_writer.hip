@@ -201,17 +201,17 @@ void dispatch_writer(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b,
 // In-place mass weighting of a dense FP64 tensor: only nonzero entries are
 // rewritten (third_order_analytical.jl:195-210).
 __global__ void __launch_bounds__(256)
-k_mass_weight3(double* __restrict__ psi, int N, const double* __restrict__ m)
+k_mass_weight3(double* __restrict__ psi, int N, int D, const double* __restrict__ m)
 {
-    const long n = 3L * N;
+    const long n = (long)D * N;
     const int kk = blockIdx.y;
-    const double mk = m[kk / 3];
+    const double mk = m[kk / D];
     const long j0 = (long)blockIdx.x;  // one column per block row-chunk
     double* col = psi + (long)kk * n * n + j0 * n;
-    const double mj = m[j0 / 3];
+    const double mj = m[j0 / D];
     for (long i = threadIdx.x; i < n; i += blockDim.x) {
         const double v = col[i];
-        if (v != 0.0) col[i] = v / sqrt((m[i / 3] * mj) * mk);
+        if (v != 0.0) col[i] = v / sqrt((m[i / D] * mj) * mk);
     }
 }
 
@@ -384,10 +384,10 @@ int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b
     return FC3_OK;
 }
 
-int mass_weight3(hipStream_t s, double* psi, int N, const double* d_masses)
+int mass_weight3(hipStream_t s, double* psi, int N, int D, const double* d_masses)
 {
-    const long n = 3L * N;
-    hipLaunchKernelGGL(k_mass_weight3, dim3((unsigned)n, (unsigned)n), dim3(256), 0, s, psi, N,
+    const long n = (long)D * N;
+    hipLaunchKernelGGL(k_mass_weight3, dim3((unsigned)n, (unsigned)n), dim3(256), 0, s, psi, N, D,
                        d_masses);
     FC3_HIP(hipGetLastError());
     return FC3_OK;

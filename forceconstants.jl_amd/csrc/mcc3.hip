@@ -739,11 +739,16 @@ struct PhiPrep {
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-size_t phi_bytes(int n) { return Carve::need((size_t)round_up(n, BK) * round_up(n, v2::BN) * 4); }
-
-int prep_phi(hipStream_t s, const float* phi, int n, Carve& cv, PhiPrep& out)
+// extra: zero columns past the padded width, for B operands that start at a column
+// offset (a tile reads up to 256 columns from its first one)
+size_t phi_bytes(int n, int extra = 0)
 {
-    const int Kpad = round_up(n, BK), Npad = round_up(n, v2::BN);
+    return Carve::need((size_t)round_up(n, BK) * (round_up(n, v2::BN) + extra) * 4);
+}
+
+int prep_phi(hipStream_t s, const float* phi, int n, Carve& cv, PhiPrep& out, int extra = 0)
+{
+    const int Kpad = round_up(n, BK), Npad = round_up(n, v2::BN) + extra;
     const long e = (long)Kpad * Npad;
     float* pf = cv.take<float>(e);
     out.b = pf;
@@ -853,11 +858,14 @@ __global__ void k_prep_phi3(const float* __restrict__ phi, int n, int Kp, int Np
     P[e + o] = l;
 }
 
-size_t phi3_bytes(int n) { return Carve::need((size_t)2 * 2 * round_up(n, v3::BN) * padk(n)); }
-
-int prep_phi3(hipStream_t s, const float* phi, int n, Carve& cv, Split3& out)
+size_t phi3_bytes(int n, int extra = 0)
 {
-    const int Npad = round_up(n, v3::BN), Kp = padk(n);
+    return Carve::need((size_t)2 * 2 * (round_up(n, v3::BN) + extra) * padk(n));
+}
+
+int prep_phi3(hipStream_t s, const float* phi, int n, Carve& cv, Split3& out, int extra = 0)
+{
+    const int Npad = round_up(n, v3::BN) + extra, Kp = padk(n);
     const long e = (long)Kp * Npad;
     __bf16* p = cv.take<__bf16>(2 * e);
     out.b = p;
@@ -985,7 +993,7 @@ long mode3_rows(int n, size_t cap_bytes, long tiles_per_launch)
 
 // Psi rows [r0, r0 + rows) -> X (split, K-blocked, K padded) -> C(m, l) for those rows.
 int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, const Split3& B,
-                __bf16* X, const OutMap& o, bool split_out)
+                __bf16* X, const OutMap& o, bool split_out, int ncols = 0)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
@@ -993,7 +1001,7 @@ int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, cons
     hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows, X);
     FC3_HIP(hipGetLastError());
     return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, o, split_out,
-                 rows, n, Kp);
+                 rows, ncols > 0 ? ncols : n, Kp);
 }
 
 // Modes 1 and 2 on slabs [l0, l0 + cnt): U holds them as the mode-1 A operand (slab
@@ -1385,6 +1393,95 @@ int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int pre
                       rows, n, n));
         FC3_HIP(hipMemcpy2DAsync(psi + r0, sizeof(float) * n2, W, sizeof(float) * rows,
                                  sizeof(float) * rows, n, hipMemcpyDeviceToDevice, s));
+    }
+    return FC3_OK;
+}
+
+// ---- bounded device memory: mcc3(Psi, phi, block_size) (src/mcc.jl:89-120) ----------------
+// K3 slabs [l0, l0 + cnt) from the whole Psi: mode 3 against phi's columns l0 .. l0 + cnt
+// only, then modes 1 and 2 on those slabs.  The device holds the block's output slabs and
+// bounded scratch (512 MiB of slab scratch, 2 GiB of split Psi rows); Psi is read, and for
+// BF16X3 split, once per block.  A GEMM tile's result depends only on its own rows and
+// columns and the full K loop, never on which other tiles a launch holds, so every slab
+// is bitwise the slab the unblocked mcc3_run computes.
+namespace {
+constexpr size_t kBlkSlabScratch = size_t(512) << 20, kBlkRowScratch = size_t(2) << 30;
+constexpr int kBlkExtraCols = 256;   // a tile reads up to 256 phi columns from its first
+
+long blk_rows(int n)
+{
+    const size_t Kp = padk(n);
+    long u = (long)(kBlkRowScratch / (4 * Kp)) / n;   // whole columns j per row block
+    u = std::max(1L, std::min(u, (long)n));
+    const long blocks = (n + u - 1) / u;
+    return (n + blocks - 1) / blocks * (long)n;
+}
+}  // namespace
+
+bool mcc3_blocked_v3(int n, int precision, const void* psi)
+{
+    return v3_applies(n, precision) && aligned16(psi);
+}
+
+size_t mcc3_blocked_bytes(int n, bool v3, int bs)
+{
+    const size_t n2 = (size_t)n * n;
+    if (v3) {
+        const size_t Kp = padk(n), P = (size_t)n * Kp;
+        const size_t nb = std::min<size_t>(bs, slab_batch3(n, kBlkSlabScratch, kTilesPerLaunch));
+        return phi3_bytes(n, kBlkExtraCols) + Carve::need(4 * P * nb) +
+               Carve::need(4 * (size_t)blk_rows(n) * Kp) +
+               ((int)Kp != n ? Carve::need(4 * P * bs) : 0) + 4096;
+    }
+    const size_t nb = std::min<size_t>(bs, slab_batch(n));
+    return phi_bytes(n, kBlkExtraCols) + Carve::need(4 * n2 * nb) + 4096;
+}
+
+int mcc3_blocked_slabs(hipStream_t s, const float* psi, const float* phi, int n, bool v3, int bs,
+                       int l0, int cnt, float* out, void* ws, size_t ws_bytes)
+{
+    if (cnt < 1 || cnt > bs || l0 < 0 || l0 + cnt > n)
+        return set_error(FC3_ERR_ARGUMENT, "mcc3 blocked: slab range out of bounds");
+    const long n2 = (long)n * n;
+    Carve cv(ws, ws_bytes);
+    if (v3) {
+        if (!aligned16(out)) return set_error(FC3_ERR_ARGUMENT, "mcc3 blocked: unaligned block");
+        Split3 B;
+        FC3_TRY(prep_phi3(s, phi, n, cv, B, kBlkExtraCols));
+        const int Kp = padk(n);
+        const long P = (long)n * Kp;
+        const int nb = (int)std::min<long>(bs, slab_batch3(n, kBlkSlabScratch, kTilesPerLaunch));
+        const long R = blk_rows(n);
+        __bf16* S = cv.take<__bf16>((size_t)2 * P * nb);
+        __bf16* X = cv.take<__bf16>((size_t)2 * R * Kp);
+        // U lives in the output block when its slabs have K3's size, else in scratch
+        __bf16* U = Kp == n ? reinterpret_cast<__bf16*>(out) : cv.take<__bf16>((size_t)2 * P * bs);
+        if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 blocked workspace too small");
+        FC3_TRY(zero_kpad(s, U, n, cnt));
+        FC3_TRY(zero_kpad(s, S, n, nb));
+        const Split3 Bc{B.b + 16L * l0, B.lo, B.ks};   // B(k, c) = phi[k, l0 + c]
+        for (long r0 = 0; r0 < n2; r0 += R) {
+            const long rows = (n2 - r0) < R ? (n2 - r0) : R;
+            const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * P, P};
+            FC3_TRY(mode3_block(s, psi, n, r0, rows, Bc, X, oU, true, cnt));
+        }
+        for (int c0 = 0; c0 < cnt; c0 += nb) {
+            const int c = (cnt - c0) < nb ? (cnt - c0) : nb;
+            FC3_TRY(modes12_v3(s, U + 2 * P * c0, n, c, B, S, out + n2 * c0));
+        }
+        return FC3_OK;
+    }
+    PhiPrep B;
+    FC3_TRY(prep_phi(s, phi, n, cv, B, kBlkExtraCols));
+    const int nb = std::min(bs, slab_batch(n));
+    float* S = cv.take<float>((size_t)n2 * nb);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 blocked workspace too small");
+    const PhiPrep Bc{B.b + (long)l0 * B.ldb, B.ldb};    // B(k, c) = phi[k, l0 + c]
+    // mode 3: U[(i, j), c] = sum_k Psi[(i, j), k] phi[k, l0 + c] -> the output block
+    FC3_TRY(gemm2(s, FC3_KERNEL_GEMM_MODE3, psi, 1, n2, Bc, out, n2, 0, n2, n2, cnt, n));
+    for (int c0 = 0; c0 < cnt; c0 += nb) {
+        const int c = (cnt - c0) < nb ? (cnt - c0) : nb;
+        FC3_TRY(modes12(s, out, n, c0, c, B, S));
     }
     return FC3_OK;
 }

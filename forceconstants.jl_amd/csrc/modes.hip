@@ -4,6 +4,9 @@
 //   triangle as Julia's Hermitian wrapper reads it.
 // The eigensolver is rocSOLVER's divide-and-conquer dsyevd in FP64 on the library's
 // stream; the caller zeroes the rigid-translation eigenvalues as get_modes does.
+// rocSOLVER's device scratch is carved from the library's workspace (sized by a rocBLAS
+// size query) and handed to the rocBLAS handle for the call only, so the handle never
+// allocates or frees device memory of its own between the library's calls.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -28,8 +31,25 @@ __global__ void k_to_f32(const double* __restrict__ a, long cnt, float* __restri
 
 }  // namespace
 
+int dsyevd_workspace_bytes(fc3_context* h, int n, size_t* bytes)
+{
+    void* hb = nullptr;
+    FC3_TRY(blas_handle(h, &hb));
+    rocblas_handle b = static_cast<rocblas_handle>(hb);
+    size_t sz = 0;
+    if (rocblas_start_device_memory_size_query(b) != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS device memory size query could not start");
+    // in query mode the call only reports its scratch size (no launch, no access)
+    (void)rocsolver_dsyevd(b, rocblas_evect_original, rocblas_fill_upper, n, nullptr, n, nullptr,
+                           nullptr, nullptr);
+    if (rocblas_stop_device_memory_size_query(b, &sz) != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS device memory size query failed");
+    *bytes = sz;
+    return FC3_OK;
+}
+
 int dynmat_eigen(fc3_context* h, double* A, int n, const double* d_masses, double* W, double* E,
-                 int* info, double* h_w)
+                 int* info, double* h_w, void* solver_ws, size_t solver_ws_bytes)
 {
     hipStream_t s = h->stream;
     const long nn = (long)n * n;
@@ -37,10 +57,20 @@ int dynmat_eigen(fc3_context* h, double* A, int n, const double* d_masses, doubl
     FC3_HIP(hipGetLastError());
     void* hb = nullptr;
     FC3_TRY(blas_handle(h, &hb));
-    const rocblas_status st = rocsolver_dsyevd(static_cast<rocblas_handle>(hb), rocblas_evect_original,
-                                               rocblas_fill_upper, n, A, n, W, E, info);
+    rocblas_handle b = static_cast<rocblas_handle>(hb);
+    if (solver_ws_bytes &&
+        rocblas_set_workspace(b, solver_ws, solver_ws_bytes) != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS: setting the eigensolver workspace failed");
+    const rocblas_status st = rocsolver_dsyevd(b, rocblas_evect_original, rocblas_fill_upper, n, A,
+                                               n, W, E, info);
+    // back to rocBLAS-managed memory, which the handle holds none of now: the workspace
+    // above belongs to the library and may be reallocated by its next call
+    const rocblas_status st2 = solver_ws_bytes ? rocblas_set_workspace(b, nullptr, 0)
+                                               : rocblas_status_success;
     if (st != rocblas_status_success)
         return set_error(FC3_ERR_HIP, std::string("rocSOLVER dsyevd: ") + rocblas_status_to_string(st));
+    if (st2 != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS: releasing the eigensolver workspace failed");
     int hinfo = 0;
     FC3_HIP(hipMemcpyAsync(h_w, W, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     FC3_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));

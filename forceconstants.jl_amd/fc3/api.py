@@ -17,6 +17,7 @@ goes through libfc3.so; nothing here computes force constants on the CPU.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass
 from typing import Optional, Tuple
 
@@ -45,6 +46,7 @@ class Context:
         check(self.lib.fc3_create(ctypes.byref(h), int(device)))
         self.h = h
         self.device = device
+        self._arrays = weakref.WeakSet()   # live DeviceArrays allocated on this handle
 
     @classmethod
     def default(cls) -> "Context":
@@ -53,7 +55,11 @@ class Context:
         return cls._default
 
     def close(self):
+        """Free the device arrays still alive on this handle, then the handle (its stream
+        and workspace).  Arrays freed here read as freed afterwards (ptr None)."""
         if getattr(self, "h", None):
+            for a in list(self._arrays):
+                a.free()
             self.lib.fc3_destroy(self.h)
             self.h = None
 
@@ -82,9 +88,12 @@ class DeviceArray:
         self.shape = tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        if not self.ctx.h:
+            raise AssertionError("DeviceArray on a closed Context")
         p = ctypes.c_void_p()
         check(self.ctx.lib.fc3_device_alloc(self.ctx.h, ctypes.byref(p), self.nbytes))
         self.ptr = p
+        self.ctx._arrays.add(self)
         if zero:
             check(self.ctx.lib.fc3_memset_zero(self.ctx.h, self.ptr, self.nbytes))
 
@@ -102,9 +111,11 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        """Release the allocation (after the handle's queued work).  Idempotent; an array
+        whose Context was closed was already freed by close()."""
+        if self.ptr and self.ctx.h:
             self.ctx.lib.fc3_device_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:

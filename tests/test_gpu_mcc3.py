@@ -199,3 +199,55 @@ def test_config_sw_diamond333(ctx, prec):
     err = rel_fro(k3, ref)
     print(f"SW diamond 3x3x3 {prec}: rel fro err {err:.3e}", flush=True)
     assert err <= TOL[prec]
+
+
+@pytest.mark.parametrize("n,block_size,prec", [(1536, 256, "bf16x3"), (1500, 250, "bf16x3"),
+                                               (1500, 250, "fp32")])
+def test_mcc3_blocked_bounded_memory_bitwise(n, block_size, prec):
+    """mcc3(Psi, phi, block_size) (src/mcc.jl:89-120) exists to bound device memory.  At
+    n = 1500/1536 (K3 = 13.5/14.5 GB) the call may grow device memory by at most two blocks
+    of output slabs plus 3 GiB of scratch, well below the whole K3, and its host result
+    must equal the unblocked device mcc3 bit for bit (every slab runs the same per-element
+    arithmetic).  Several blocks run (6 at both sizes), and n = 1500 (n % 16 != 0) takes
+    the separate split-operand scratch.  Psi is a rank-3 tensor uploaded slab by slab."""
+    other = fc3.Context(0)   # its close() frees the arrays below
+    try:
+        rng = np.random.default_rng(n + block_size)
+        a, b, c = (rng.standard_normal((3, n)) for _ in range(3))
+        phi32 = np.linalg.qr(rng.standard_normal((n, n)))[0].astype(np.float32)
+        psi = fc3.DeviceArray((n, n, n), np.float32, other)
+        slab_bytes = 4 * n * n
+        for k in range(n):
+            s = np.asfortranarray(np.einsum("ti,tj,t->ij", a, b, c[:, k]).astype(np.float32))
+            fc3._lib.check(other.lib.fc3_memcpy_h2d(other.h, ctypes.c_void_p(psi.ptr.value + slab_bytes * k),
+                                                    s.ctypes.data_as(ctypes.c_void_p), slab_bytes))
+        dphi = fc3.DeviceArray.from_host(phi32, ctx=other)
+        other.synchronize()
+        free0 = other.memory()[0]
+        host = fc3.mcc3(psi, dphi, block_size, precision=prec)
+        growth = free0 - other.memory()[0]
+        bmem = max(1, (4 << 30) // slab_bytes)
+        bs = min(n, block_size if block_size >= bmem else (bmem // block_size) * block_size)
+        print(f"blocked n={n} block_size={block_size} {prec}: {(n + bs - 1) // bs} blocks of {bs} "
+              f"slabs, device growth {growth / 2**30:.2f} GiB (K3 {4 * n**3 / 2**30:.2f} GiB)",
+              flush=True)
+        assert (n + bs - 1) // bs >= 2
+        # two output blocks, the split mode-1 operand of a block when n % 16 != 0 (its K is
+        # padded, so it cannot live in the output block), and 3 GiB of bounded scratch
+        kp = -(-n // 16) * 16
+        split = bs * 4 * n * kp if (prec == "bf16x3" and kp != n) else 0
+        assert growth <= 2 * bs * slab_bytes + split + (3 << 30)
+        assert growth < 0.7 * 4 * n ** 3
+        dev = fc3.mcc3(psi, dphi, precision=prec).to_host()
+        assert np.array_equal(host, dev)
+        # and the result itself, on a few slabs, against the rank-3 closed form
+        p64 = phi32.astype(np.float64)
+        A, B, C = a @ p64, b @ p64, c @ p64
+        num = den = 0.0
+        for l in (0, bs - 1, bs, n - 1):
+            ref = np.einsum("ti,tj,t->ij", A, B, C[:, l])
+            num += float(np.sum((host[:, :, l].astype(np.float64) - ref) ** 2))
+            den += float(np.sum(ref * ref))
+        assert (num / den) ** 0.5 <= TOL[prec]
+    finally:
+        other.close()

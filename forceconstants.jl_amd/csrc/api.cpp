@@ -172,6 +172,11 @@ int fc3_create(fc3_handle* out, int device)
         delete h;
         return set_error(FC3_ERR_HIP, std::string("fc3_create: ") + hipGetErrorString(e));
     }
+    // the static scratch of the allocation-free entry points, looked up on this device
+    if (const int st = noalloc_init_device(device); st != FC3_OK) {
+        fc3_destroy(h);
+        return st;
+    }
     *out = h;
     return FC3_OK;
 }

@@ -72,6 +72,11 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// Allocation-free entry points (noalloc.hip): size of the static device scratch, and the
+// once-per-device lookup of its address (fc3_create calls it with the device current).
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+int noalloc_init_device(int device);
+
 // Ensure the handle's workspace holds at least `bytes`.
 int ensure_workspace(fc3_context* h, size_t bytes);
 

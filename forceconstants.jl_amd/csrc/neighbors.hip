@@ -66,6 +66,8 @@ __global__ void __launch_bounds__(256) k_nbr_fill(const double* __restrict__ pos
     }
 }
 
+__global__ void k_zero_int(int* __restrict__ p) { *p = 0; }
+
 // rev[a][s] = slot of a in the list of b = nbr[a][s] (lists are symmetric and sorted).
 __global__ void k_nbr_rev(const int* __restrict__ count, const int* __restrict__ nbr, int N,
                           int cap, int* __restrict__ rev)
@@ -91,7 +93,8 @@ __global__ void k_nbr_rev(const int* __restrict__ count, const int* __restrict__
 int neighbor_count(hipStream_t s, const double* d_pos, int N, const double box[3],
                    double r_cut, int* d_count, int* d_maxcount, int* h_max)
 {
-    FC3_HIP(hipMemsetAsync(d_maxcount, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, d_maxcount);   // a fill kernel, not a memset node
+    FC3_HIP(hipGetLastError());
     const int per_block = 256 / kWave;
     dim3 grid((N + per_block - 1) / per_block);
     hipLaunchKernelGGL(k_nbr_count, grid, dim3(256), 0, s, d_pos, N, box[0], box[1], box[2],

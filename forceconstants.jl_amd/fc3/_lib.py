@@ -67,6 +67,8 @@ SIGNATURES = {
     "fc3_second_order_sw": (_c_int, [_handle, _dp, _c_int, _dp, _dp, _c_double, _c_double, _vp]),
     "fc3_third_order_lj": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double, _c_double,
                                     _c_double, _c_double, _dp, _vp, _vp]),
+    "fc3_third_order_lj_noalloc": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double,
+                                            _c_double, _c_double, _c_double, _dp, _vp, _vp]),
     "fc3_third_order_sw": (_c_int, [_handle, _dp, _c_int, _dp, _dp, _c_double, _c_double, _dp,
                                     _vp, _vp]),
     "fc3_modes_lj": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double, _c_double,

@@ -143,6 +143,21 @@ int fc3_third_order_lj(fc3_handle h, const double* positions, int n_atoms,
                        double pot_r_cut, double r_cut, double tol,
                        const double* masses, double* psi64, float* psi32);
 
+/* Allocation-free variant of fc3_third_order_lj: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, checks and results (bit for bit) as fc3_third_order_lj,
+ * but the scratch (positions, neighbour lists, pair and self blocks) lives in a static
+ * 64 MiB device array per GPU instead of the handle's workspace, and the call makes no
+ * device allocation.  Systems whose scratch exceeds it (296 bytes per neighbour slot,
+ * N * cap slots for N atoms with at most cap neighbours each, plus 252 bytes per atom)
+ * return FC3_ERR_CAPACITY before any launch that writes the outputs.
+ * All calls on one GPU share that array: calls that overlap on one GPU race.  It waits
+ * for the neighbour count on the host, as fc3_third_order_lj does.                 */
+int fc3_third_order_lj_noalloc(fc3_handle h, const double* positions, int n_atoms,
+                               const double box[3], double sigma, double epsilon,
+                               double pot_r_cut, double r_cut, double tol,
+                               const double* masses, double* psi64, float* psi32);
+
 /* Fused third_order + mass weighting + mcc3 for LJ without the dense tensor: the
  * K3 = mcc3(Psi32, phi) that fc3_third_order_lj (psi32, masses) followed by fc3_mcc3
  * (FC3_PREC_BF16X3) gives, with the mode-1 product taken over Psi's nonzeros straight

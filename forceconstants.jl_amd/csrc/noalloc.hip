@@ -6,8 +6,8 @@
 // allocate (serving loops, harnesses that forbid it) use this variant instead: the same
 // kernels, with that scratch carved from `scratch`, a static device array of
 // kNoallocScratchBytes (64 MiB) that every GPU the library runs on holds until the
-// process ends.  The
-// caller passes the outputs, as for fc3_third_order_lj, and the results are identical.
+// process ends.  The caller passes the outputs, as for fc3_third_order_lj, and the
+// results are identical.
 //
 // The array's name is a host-side stand-in, not a device address: its address on each
 // device is taken once, with that device current, by fc3_create (noalloc_init_device)

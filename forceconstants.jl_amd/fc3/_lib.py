@@ -78,6 +78,11 @@ SIGNATURES = {
     "fc3_third_order_mcc3_lj": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double,
                                          _c_double, _c_double, _c_double, _dp, _vp, _vp]),
     "fc3_mass_weight_third_order": (_c_int, [_handle, _vp, _c_int, _c_int, _dp]),
+    "fc3_save_third_order_txt": (_c_int, [_handle, _vp, _c_int, _c_int, ctypes.c_char_p,
+                                          ctypes.c_char_p]),
+    "fc3_save_second_order_txt": (_c_int, [_handle, _vp, _c_int, _c_int, ctypes.c_char_p,
+                                           ctypes.c_char_p]),
+    "fc3_format_float": (_c_int, [_c_double, ctypes.c_char_p, _c_int]),
     "fc3_mcc3": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _vp]),
     "fc3_mcc3_inplace": (_c_int, [_handle, _vp, _vp, _c_int, _c_int]),
     "fc3_mcc3_blocked": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _c_int, _fp]),

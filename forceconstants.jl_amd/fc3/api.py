@@ -385,6 +385,61 @@ def mcc3_(psi: DeviceArray, phi: DeviceArray, precision="fp32") -> DeviceArray:
     return psi
 
 
+# ---- output formats (src/io.jl:9-80) ---------------------------------------------------
+
+def format_float(x: float) -> str:
+    """x as the text files print it: Julia's print of a Float64."""
+    L = _lib.load()
+    buf = ctypes.create_string_buffer(40)
+    check(L.fc3_format_float(float(x), buf, len(buf)))
+    return buf.value.decode()
+
+
+def _save(fc, N_atoms, D, outpath, filename, fmt, order, units, ctx):
+    import os
+    if fmt == "JLD2":
+        raise NotImplementedError("fmt = :JLD2 needs an HDF5 writer, which this build does not "
+                                  "have; use fmt = 'txt'")
+    if fmt != "txt":
+        raise ValueError(f"Invalid format, {fmt}")
+    n = int(N_atoms) * int(D)
+    if tuple(fc.shape) != (n,) * order:     # io.jl:10 / :40
+        raise AssertionError("Incorrect dimensions")
+    path = os.path.join(outpath, filename + ".txt")
+    if isinstance(fc, DeviceArray):
+        ctx = ctx or fc.ctx
+        _check_buf(fc, (n,) * order, np.float64, "force constants", ctx)
+        d, owned = fc, False
+    else:
+        ctx = ctx or Context.default()
+        units = fc.units if units is None else units
+        d, owned = DeviceArray.from_host(np.asfortranarray(fc.values, dtype=np.float64), ctx=ctx), True
+    fn = ctx.lib.fc3_save_third_order_txt if order == 3 else ctx.lib.fc3_save_second_order_txt
+    try:
+        check(fn(ctx.h, d.ptr, int(N_atoms), int(D), (units or "").encode(), path.encode()))
+    finally:
+        if owned:
+            d.free()
+    return path
+
+
+def save_second_order(Phi, N_atoms: int, D: int, outpath: str, filename: str = "second_order",
+                      fmt: str = "JLD2", units: Optional[str] = None,
+                      ctx: Optional[Context] = None) -> str:
+    """save_second_order(Phi, N_atoms, D, outpath; filename, fmt) (src/io.jl:9-37).
+    Phi: DenseForceConstants (host values) or a DeviceArray (FP64, n x n).  fmt "txt"
+    writes `outpath/filename.txt`; the nonzeros are found on the GPU."""
+    return _save(Phi, N_atoms, D, outpath, filename, fmt, 2, units, ctx)
+
+
+def save_third_order(Psi, N_atoms: int, D: int, outpath: str, filename: str = "third_order",
+                     fmt: str = "JLD2", units: Optional[str] = None,
+                     ctx: Optional[Context] = None) -> str:
+    """save_third_order(Psi, N_atoms, D, outpath; filename, fmt) (src/io.jl:39-69).
+    Psi: DenseForceConstants (host values) or a DeviceArray (FP64, n x n x n)."""
+    return _save(Psi, N_atoms, D, outpath, filename, fmt, 3, units, ctx)
+
+
 # ---- adjoints (reverse mode; the targets of the Julia rrules) -------------------------
 
 def mcc3_backward(psi: DeviceArray, phi: DeviceArray, dk3: DeviceArray, precision="fp32",

@@ -187,6 +187,25 @@ int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms,
 int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int dim,
                                 const double* masses);
 
+/* ---- text output (save_second_order / save_third_order, fmt = :txt) ----------- */
+/* save_third_order(Psi, N_atoms, D, outpath; fmt = :txt) replaces src/io.jl:39-62: the
+ * device tensor psi (FP64, n^3, n = dim * n_atoms) written to the file `path` as the
+ * header "#N atoms, Sum Ψ = s, Units: units" (s = round(sum(Psi), sigdigits = 7)) and
+ * one line "i a j b k c value" (1-based) per nonzero, atoms i, j, k outermost and
+ * components a, b, c innermost, values printed as Julia prints a Float64 (shortest
+ * decimal that reads back exactly).  The nonzeros are found and ordered on the
+ * device; only they cross to the host.  units: UTF-8 text, or NULL for none.      */
+int fc3_save_third_order_txt(fc3_handle h, const double* psi, int n_atoms, int dim,
+                             const char* units, const char* path);
+/* save_second_order(Phi, N_atoms, D, outpath; fmt = :txt) replaces src/io.jl:9-28, the
+ * same for the device matrix ifc2 (FP64, n^2): "#N atoms, Sum Φ = s, Units: units"
+ * and lines "i a j b value".                                                         */
+int fc3_save_second_order_txt(fc3_handle h, const double* ifc2, int n_atoms, int dim,
+                              const char* units, const char* path);
+/* x as the text files print it (Julia's print of a Float64), NUL-terminated into buf
+ * (len >= 33 always suffices).  Host only.                                          */
+int fc3_format_float(double x, char* buf, int len);
+
 /* ---- modal coupling constants ----------------------------------------------- */
 /* mcc3(Psi::CuArray{Float32,3}, phi::CuArray{Float32,2}) replaces src/mcc.jl:29-51
  * (and mcc3_tens_opt, src/mcc.jl:11-20):

@@ -261,3 +261,68 @@ def mcc3_fp64_slab(psi_sparse, phi: np.ndarray, l: int) -> np.ndarray:
     import scipy.sparse as sp
     M = sp.coo_matrix((w, (i, j)), shape=(n, n)).tocsr()
     return F.T @ (M @ F)
+
+
+# ---- text output (src/io.jl:9-62) ------------------------------------------------------
+
+def julia_float_str(x: float) -> str:
+    """print(::Float64) as Julia does it (Base.Ryu.writeshortest with print's defaults),
+    restated from Python's shortest round-trip digits (repr): digits d_1..d_m, pt =
+    (decimal exponent of d_1) + 1; plain notation for -4 < pt <= 6 (for pt >= m only
+    when x is that integer up to 0.05), '.0' after integers; else 'd_1.d_2..d_m' + 'e' +
+    str(pt - 1), with 'd_1.0' when m = 1."""
+    import math
+    from decimal import Decimal
+    if x != x:
+        return "NaN"
+    neg = math.copysign(1.0, x) < 0
+    s = "-" if neg else ""
+    if math.isinf(x):
+        return s + "Inf"
+    if x == 0.0:
+        return s + "0.0"
+    _, digs, exp = Decimal(repr(abs(x))).as_tuple()
+    digs = list(digs)
+    while len(digs) > 1 and digs[-1] == 0:
+        digs.pop()
+        exp += 1
+    m = len(digs)
+    d = "".join(map(str, digs))
+    pt = exp + m
+    plain = -4 < pt <= 6
+    if plain and pt >= m:
+        y = 10.0 ** (pt - m)
+        if abs(math.fmod(x + 0.05, y) % y - 0.05) > 0.05:
+            plain = False
+    if plain:
+        if pt <= 0:
+            return s + "0." + "0" * (-pt) + d
+        if pt < m:
+            return s + d[:pt] + "." + d[pt:]
+        return s + d + "0" * (pt - m) + ".0"
+    return s + d[0] + "." + (d[1:] or "0") + "e" + str(pt - 1)
+
+
+def save_txt_lines(values: np.ndarray, N_atoms: int, D: int):
+    """The data lines save_second_order / save_third_order (fmt = :txt) print for a dense
+    tensor: one 'i a j b [k c] value' line (1-based) per nonzero, in the order of the
+    reference's loops, atoms i, j(, k) outermost and components a, b(, c) innermost
+    (io.jl:16-27, 46-61)."""
+    v = np.asarray(values)
+    order = v.ndim
+    idx = np.nonzero(v)
+    atoms = [ix // D for ix in idx]
+    comps = [ix % D for ix in idx]
+    key = np.zeros(len(idx[0]), dtype=np.int64)
+    for a in atoms:
+        key = key * N_atoms + a
+    for c in comps:
+        key = key * D + c
+    perm = np.argsort(key, kind="stable")
+    lines = []
+    for t in perm:
+        cols = []
+        for q in range(order):
+            cols += [str(int(atoms[q][t]) + 1), str(int(comps[q][t]) + 1)]
+        lines.append(" ".join(cols) + " " + julia_float_str(float(v[tuple(ix[t] for ix in idx)])))
+    return lines

@@ -73,44 +73,80 @@ def gemm_traffic():
     return None
 
 
-def cpu_baseline_mcc3(n=1024, seed=0):
-    """Host FP32 three-mode contraction with numpy/OpenBLAS on the host cores: the
-    reference has no CPU mcc3, so this restates its three mode products on the CPU
-    (per-slab GEMMs phi^T Psi_k phi, then one GEMM for the last mode)."""
+def cpu_model():
+    """The host CPU's model name as /proc/cpuinfo gives it (None if unavailable)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def blas_threads():
     try:
         from threadpoolctl import threadpool_info
-        threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+        return max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
     except Exception:
-        threads = os.cpu_count() or 1
-    rng = np.random.default_rng(seed)
-    psi = rng.standard_normal((n, n, n), dtype=np.float32)   # psi[k] is the slab Psi[:, :, k]
-    phi = rng.standard_normal((n, n), dtype=np.float32)
+        return os.cpu_count() or 1
+
+
+def cpu_baseline_mcc3(coo32, phi32, n, width=24, reps=5):
+    """The reference has no CPU mcc3, so this restates its three mode products on the host
+    with numpy/OpenBLAS FP32 GEMMs on all cores, on this workload's own mass-weighted FP32
+    Psi and phi, as the device orders them (mode 3, then modes 1 and 2 per slab).  Bounded
+    sample, the same fraction width / n of each product: mode 3 for the rows (i, j) with j
+    in one band of `width` columns (dense, all n output modes: 2 width n^3 flop) and modes
+    1 and 2 for `width` K3 slabs (4 width n^3 flop), from the dense slabs U_l = Psi x3
+    phi[:, l] prepared beforehand.  Its flop rate is that of the whole 6 n^4 contraction.
+    One warm-up, then the median of `reps` runs."""
+    import scipy.sparse as sp
+    i, j, k, v = coo32
+    i = i.astype(np.int64)
+    j = j.astype(np.int64)
+    k = k.astype(np.int64)
+    phi = np.asfortranarray(phi32, dtype=np.float32)
     phiT = np.ascontiguousarray(phi.T)
-    t0 = time.perf_counter()
-    T = np.empty((n, n, n), dtype=np.float32)
-    for k in range(n):
-        T[k] = phiT @ psi[k] @ phi                            # modes 1 and 2 on slab k
-        if k % 256 == 255:
-            log(f"cpu mcc3 slab {k + 1}/{n}")
-    K3 = phiT @ T.reshape(n, n * n)                          # mode 3
-    dt = time.perf_counter() - t0
-    del T, K3, psi
-    return {"value": 6.0 * n ** 4 / dt / 1e12, "unit": "TFLOP/s", "cores": int(threads),
-            "kind": "port", "sample": f"FP32 mcc3 of a random {n}^3 tensor (numpy/OpenBLAS "
-            f"GEMMs on {threads} threads), {dt:.1f} s"}
+    c0 = n // 2
+    sel = (j >= c0) & (j < c0 + width)
+    A = np.zeros((n * width, n), dtype=np.float32, order="F")       # Psi rows of the band
+    A[i[sel] + n * (j[sel] - c0), k[sel]] = v[sel].astype(np.float32)
+    M = sp.csr_matrix((v, (i + n * j, k)), shape=(n * n, n))
+    Ub = np.asarray(M @ phi[:, c0:c0 + width].astype(np.float64), dtype=np.float32)
+    Us = [np.ascontiguousarray(Ub[:, c]).reshape(n, n, order="F") for c in range(width)]
+    del M, Ub
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        _ = A @ phi                                   # mode 3 for the band's rows
+        for Ul in Us:
+            _ = phiT @ Ul @ phi                       # modes 1 and 2 of one slab
+        dt = time.perf_counter() - t0
+        if r:
+            times.append(dt)
+    dt = float(np.median(times))
+    th = blas_threads()
+    return {"value": 6.0 * width * n ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": int(th),
+            "kind": "port", "cpu": cpu_model(),
+            "sample": f"FP32 mcc3 (numpy/OpenBLAS, {th} threads) of this workload's Psi and phi "
+                      f"at {n} DoF: mode 3 for the rows of {width} Psi columns and modes 1-2 "
+                      f"for {width} K3 slabs (6 x {width} n^3 flop, {width}/{n} of the whole), "
+                      f"median of {reps} runs after a warm-up: {dt:.2f} s"}
 
 
-def cpu_baseline_assembly(reps=5):
+def cpu_baseline_assembly(cells=5, reps=5):
     """The oracle's restatement of the reference loop (dense FP64 third_order!, ASR!,
-    apply_tols!) on FCC 4x4x4 (768 DoF), threaded over atoms on all OpenMP threads as
+    apply_tols!) on LJ argon FCC cells^3, threaded over atoms on all OpenMP threads as
     the reference's @tasks loops are (third_order_analytical.jl:14-15, asr.jl:19-40);
     one warm-up, then the median of `reps` runs."""
     from fc3 import systems as S
     from oracle import fc3_oracle as O
-    sysd = S.fcc(S.ARGON_A, (4, 4, 4), S.ARGON_MASS, 0.05, seed=768)
+    sysd = S.fcc(S.ARGON_A, (cells,) * 3, S.ARGON_MASS, 0.05, seed=768)
     calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
     threads = int(O.lib().fco_num_threads())
-    log(f"cpu assembly (oracle, FCC 4x4x4, {threads} threads)")
+    n = sysd.n_dof
+    log(f"cpu assembly (oracle, FCC {cells}^3, {threads} threads)")
     times = []
     for r in range(reps + 1):
         t0 = time.perf_counter()
@@ -120,10 +156,40 @@ def cpu_baseline_assembly(reps=5):
         if r:
             times.append(dt)
     dt = float(np.median(times))
-    n = sysd.n_dof
     return {"value": 8.0 * n ** 3 / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"LJ FCC 4x4x4 ({n} DoF) third_order, dense FP64, OpenMP over atoms, "
-                      f"median of {reps} runs after a warm-up: {dt:.2f} s"}
+            "cpu": cpu_model(),
+            "sample": f"LJ FCC {cells}x{cells}x{cells} ({n} DoF) third_order, dense FP64, "
+                      f"OpenMP over atoms, median of {reps} runs after a warm-up: {dt:.2f} s"}
+
+
+def sw_assembly(ctx, steps=3):
+    """IFC3 assembly for Stillinger-Weber silicon on diamond 3x3x3 (648 DoF, sigma_u =
+    0.03 A, seed 648): FP64 Psi + mass-weighted FP32 Psi in one pass, 12 n^3 bytes."""
+    import fc3
+    from fc3 import systems as S
+    pot = S.StillingerWeberSilicon()
+    calc = S.AutoDiffCalculator(1e-12, pot.r_cut)
+    sysd = S.diamond(S.SI_A, (3, 3, 3), S.SI_MASS, 0.03, seed=648)
+    n = sysd.n_dof
+    d64 = fc3.DeviceArray((n, n, n), np.float64, ctx)
+    d32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, pot, calc, psi64=d64, psi32=d32, masses=sysd.masses, ctx=ctx)
+    ctx.synchronize()
+    profile_enable(ctx)
+    ctx.timer_start()
+    for _ in range(steps):
+        fc3.third_order_device(sysd, pot, calc, psi64=d64, psi32=d32, masses=sysd.masses, ctx=ctx)
+    t = ctx.timer_stop() / steps
+    w_ms, w_cnt = profile(ctx, 0)
+    profile_enable(ctx, False)
+    d64.free()
+    d32.free()
+    b = 12.0 * n ** 3
+    res = {"config": f"SW silicon diamond 3x3x3 ({n} DoF)", "ms_per_step": t,
+           "GB_per_s": b / (t * 1e-3) / 1e9, "writer_kernel_ms": w_ms / max(w_cnt, 1),
+           "writer_GB_per_s": b / (w_ms / max(w_cnt, 1) * 1e-3) / 1e9, "bytes_per_step": b}
+    log(f"sw assembly {res}")
+    return res
 
 
 def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
@@ -318,6 +384,8 @@ def main():
                     help="skip the fused third_order + mcc3 (no dense Psi) timing")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+backward timing at FCC 5x5x5 (1500 DoF)")
+    ap.add_argument("--no-sw", action="store_true",
+                    help="skip the Stillinger-Weber assembly timing (diamond 3x3x3, 648 DoF)")
     a = ap.parse_args()
     if a.modes_timing:
         print("--modes-timing is disabled: a profiled run of device modes followed by the batch "
@@ -397,7 +465,14 @@ def main():
 
     results = {}
     runs = [("bf16x3", a.steps)] + ([("fp32", a.fp32_steps)] if a.fp32_steps > 0 else [])
-    for prec, steps in runs:
+    if a.fp32_steps > 0:
+        # BF16x3 with every mode-3 block multiplied: the same arithmetic as the FP32 form,
+        # which does not skip Psi's all-zero blocks (the like-for-like precision ratio)
+        runs.append(("bf16x3_dense_mode3", a.fp32_steps))
+    for key, steps in runs:
+        prec = "bf16x3" if key.startswith("bf16x3") else key
+        if key == "bf16x3_dense_mode3":
+            os.environ["FC3_MCC3_DENSE_MODE3"] = "1"
         for _ in range(a.warmup):
             fc3.mcc3(d32, dphi, precision=prec, out=k3)
         ctx.synchronize()
@@ -409,32 +484,49 @@ def main():
         m3_ms, m3_c = profile(ctx, 1)
         m12_ms, m12_c = profile(ctx, 2)
         profile_enable(ctx, False)
+        os.environ.pop("FC3_MCC3_DENSE_MODE3", None)
         err = slab_error()
         mul = 3.0 if prec == "bf16x3" else 1.0
-        results[prec] = {
+        results[key] = {
             "ms_per_step": t, "TFLOP_per_s": flop / (t * 1e-3) / 1e12, "rel_fro_err": err,
             "err_sample": f"K3 slabs {slabs} vs FP64 host contraction of the same FP32 Psi, phi",
             "gemm_ms_per_step": (m3_ms + m12_ms) / steps,
             "gemm_launches_per_step": (m3_c + m12_c) // steps,
+            "mode3_launches_per_step": m3_c / steps, "mode12_launches_per_step": m12_c / steps,
             "mode3_ms_per_step": m3_ms / steps, "mode12_ms_per_step": m12_ms / steps,
             # MFMA flops the kernels issue (3 bf16 products per FP32-equivalent product)
             "mfma_flop_per_step": mul * flop,
         }
-        log(f"mcc3 {prec}: {results[prec]}")
+        log(f"mcc3 {key}: {results[key]}")
 
     bf = results["bf16x3"]
-    # Dominant kernel: the BF16x3 mode-product GEMM (k_gemm3, all three modes; about 95%
-    # of the contraction's device time).  Algorithmic work = 3 bf16 products x 6 n^4
-    # over the step's launches; achieved = that per launch / average launch duration,
-    # both from HIP events on the library's stream.
-    launches = max(bf["gemm_launches_per_step"], 1)
-    f_launch = 3.0 * flop / launches
-    ms_launch = bf["gemm_ms_per_step"] / launches
+    # Dominant kernel: the BF16x3 mode-product GEMM k_gemm3 on the dense operands of modes
+    # 1 and 2 (mode 3 multiplies only Psi's nonzero blocks and takes a few percent of the
+    # time).  Algorithmic work of those launches = 3 bf16 products x 4 n^4 (two of the three
+    # mode products); achieved = that per launch / their average launch duration, both from
+    # HIP events on the library's stream.
+    m12 = max(1, int(round(bf["mode12_launches_per_step"])))
+    f_launch = 3.0 * 4.0 * n ** 4 / m12
+    ms_launch = bf["mode12_ms_per_step"] / m12
     achieved = f_launch / (ms_launch * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_gemm3 (BF16x3 mode products, all modes)",
+    traffic = gemm_traffic()
+    roofline = {"bound": "mfma", "kernel": "k_gemm3 (BF16x3), the mode-1 and mode-2 launches",
                 "achieved": achieved, "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / (PEAK["bf16"] / 1e12), "traffic": gemm_traffic(),
-                "flop_per_launch": f_launch, "ms_per_launch": ms_launch, "launches_per_step": launches}
+                "frac": achieved / (PEAK["bf16"] / 1e12), "traffic": traffic,
+                "traffic_source": ("profiles/k_gemm3_lj_fcc666_traffic.md: rocprofv3 --pmc "
+                                   "FETCH_SIZE and WRITE_SIZE in separate runs of this workload, "
+                                   "copy-kernel calibrated; not measured in this run")
+                if traffic is not None else None,
+                "flop_per_launch": f_launch, "ms_per_launch": ms_launch, "launches_per_step": m12,
+                "mode3": {"launches_per_step": bf["mode3_launches_per_step"],
+                          "ms_per_step": bf["mode3_ms_per_step"],
+                          "note": "mode 3 multiplies only the 256-row x 16-K blocks of Psi that "
+                                  "hold a nonzero (bitwise the same result)"}}
+    roofline_assembly = {"bound": "hbm", "kernel": "k_write_ifc3_pairwise",
+                         "achieved": asm["writer_GB_per_s"], "peak": PEAK["hbm"] / 1e9,
+                         "unit": "GB/s", "frac": asm["writer_GB_per_s"] / (PEAK["hbm"] / 1e9),
+                         "bytes_per_launch": asm["bytes_per_step"],
+                         "ms_per_launch": asm["writer_kernel_ms"]}
 
     out = {
         "metric": METRIC, "value": bf["TFLOP_per_s"], "unit": "TFLOP/s", "n_gpus": 1,
@@ -445,9 +537,13 @@ def main():
                                "assembly + mcc3 (BF16x3 headline, FP32 alongside)",
                    "n_dof": n, "n_atoms": sysd.n_atoms},
         "roofline": roofline,
+        "roofline_assembly": roofline_assembly,
         "mcc3": results,
         "fp32_vs_bf16x3_speedup": (results["fp32"]["ms_per_step"] / bf["ms_per_step"]
                                    if "fp32" in results else None),
+        "fp32_vs_bf16x3_dense_mode3_speedup": (
+            results["fp32"]["ms_per_step"] / results["bf16x3_dense_mode3"]["ms_per_step"]
+            if "bf16x3_dense_mode3" in results else None),
         "assembly": asm,
     }
     d32.free()
@@ -477,9 +573,11 @@ def main():
         out["batched"] = batched_snapshots(ctx, modes_timing=a.modes_timing)
     if not a.no_adjoint:
         out["adjoint"] = adjoint_timing(ctx)
+    if not a.no_sw:
+        out["sw_assembly"] = sw_assembly(ctx)
     if not a.no_cpu_baseline:
         log("cpu baselines")
-        out["cpu_baseline"] = cpu_baseline_mcc3()
+        out["cpu_baseline"] = cpu_baseline_mcc3(coo32, phi32, n)
         out["cpu_baseline_assembly"] = cpu_baseline_assembly()
     else:
         out["cpu_baseline"] = None

@@ -16,6 +16,7 @@
 //                once into K-blocked hi/lo planes (each GEMM writes its output already
 //                split for the next one) and copied global -> LDS without registers.
 #include <algorithm>
+#include <cstdlib>
 
 #include "bf16_split.hpp"
 #include "fc3_internal.hpp"
@@ -329,7 +330,25 @@ struct G3 {
     // either way it writes C + z c_ZS.  An unsplit product: zk = 1, kc = K / BK, nz = 1.
     int kc, zk, nz;
     long a_ZS, b_ZS, c_ZS;
+    // sparse K (k_gemm3<..., true>, unsplit products only): row tile tm multiplies only
+    // the K blocks kb with kmask[tm kmask_ld + kb] != 0, in ascending order; the others
+    // hold only zeros in A, so the result is the same
+    const unsigned char* kmask;
+    int kmask_ld;
+    int kmask_div;       // row tile tm reads mask row tm / kmask_div
+    // optional outer level of the A row map: A row m -> (m / a_R2) a_SS2 + row m % a_R2
+    // of the two-level map above (a_R2 = 2^30 when unused)
+    int a_R2;
+    long a_SS2;
+    // epilogue (256 x 256 tiles): 0 = each lane stores its accumulators directly; 1 = C is
+    // contiguous along rows (c_KS == 16) and 2 = along columns within 16-row groups
+    // (c_CS == 16): the tile goes out through LDS so that consecutive lanes store
+    // consecutive addresses
+    int epi;
 };
+
+// K blocks a sparse-K tile can list (K <= 65536)
+constexpr int kMaxKBlocks = 4096;
 
 // Workgroup barrier for the copy ring: s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding,
 // expcnt unconstrained), then s_barrier.  After it, every wave's copies older than its
@@ -393,9 +412,109 @@ struct Shape<3> {
 
 // One tile of shape SHAPE at (m0, n0) on the four LDS stage objects of the kernel
 // (passed in, so that after inlining every access still names its own object).
-template <bool SPLIT_OUT, int SHAPE>
+// Four C values of rows row..row + 3 (one 16-row group, row % 4 == 0) and column col,
+// at the address of the layout in g, as FP32 or split into bf16 hi/lo planes.
+template <bool SPLIT_OUT>
+__device__ __forceinline__ void store4(const G3& g, int kz, int row, int col, float x0, float x1,
+                                       float x2, float x3)
+{
+    const unsigned sl = (unsigned)row / (unsigned)g.c_R;
+    const int rr = row - (int)sl * g.c_R;   // rr % 4 == 0: the 4 rows share a K block
+    const long off = (long)sl * g.c_SS + (long)(rr >> 4) * g.c_KS + (rr & 15) + (long)col * g.c_CS;
+    if constexpr (SPLIT_OUT) {
+        __bf16* C = static_cast<__bf16*>(g.C) + kz * g.c_ZS;
+        bf16x4 h, l;
+        __bf16 hh, ll;
+        split_bf16(x0, hh, ll); h[0] = hh; l[0] = ll;
+        split_bf16(x1, hh, ll); h[1] = hh; l[1] = ll;
+        split_bf16(x2, hh, ll); h[2] = hh; l[2] = ll;
+        split_bf16(x3, hh, ll); h[3] = hh; l[3] = ll;
+        *reinterpret_cast<bf16x4*>(C + off) = h;
+        *reinterpret_cast<bf16x4*>(C + off + g.c_lo) = l;
+    } else {
+        float* C = static_cast<float*>(g.C) + kz * g.c_ZS;
+        *reinterpret_cast<float4*>(C + off) = make_float4(x0, x1, x2, x3);
+    }
+}
+
+// LDS visibility of this wave's stores to its other lanes (and the order of the wave's
+// LDS accesses around it)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The 128 x 128 wave tile of a 256 x 256 tile, written through the wave's own LDS stage
+// buffer `st` (>= 18 KB; the stages are free once every wave's copies have landed and
+// its fragment reads are done, which the barrier at the top ensures).  Each element goes
+// to the address the direct epilogue would give it (store4); only the lane that stores
+// it changes, so that a wave's store covers contiguous memory:
+//   epi 1 (rows contiguous): 32-column strips; lane = (column pair, 4-row chunk of 128),
+//          one instruction = 2 columns x 128 rows;
+//   epi 2 (columns contiguous in 16-row groups): 32-row strips; lane = (column, 4-row
+//          chunk of a 16-row group), one instruction = 16 columns x 16 rows.
+template <bool SPLIT_OUT>
+__device__ __forceinline__ void staged_epilogue(const G3& g, int m0, int n0, int wm0, int wn0,
+                                                int kz, const f32x16 (&acc)[4][4], float* st)
+{
+    const int lane = threadIdx.x & 63;
+    __syncthreads();
+    if (g.epi == 1) {
+        constexpr int PR = 132;   // [column][row] pitch in floats: 128 rows + 4
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 32 * i + 8 * q + 4 * (lane >> 5), c = lane & 31;
+                    *reinterpret_cast<float4*>(st + c * PR + r) =
+                        make_float4(acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
+                                    acc[i][j][4 * q + 3]);
+                }
+            wave_lds_sync();
+#pragma unroll 1
+            for (int t = 0; t < 16; ++t) {
+                const int c = 2 * t + (lane >> 5), r = 4 * (lane & 31);
+                const float4 v = *reinterpret_cast<const float4*>(st + c * PR + r);
+                const int row = m0 + wm0 + r, col = n0 + wn0 + 32 * j + c;
+                if (row < g.M && col < g.N) store4<SPLIT_OUT>(g, kz, row, col, v.x, v.y, v.z, v.w);
+            }
+            wave_lds_sync();
+        }
+    } else {
+        constexpr int PC = 36;    // [column][row] pitch in floats: 32 rows + 4
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 8 * q + 4 * (lane >> 5), c = 32 * j + (lane & 31);
+                    *reinterpret_cast<float4*>(st + c * PC + r) =
+                        make_float4(acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
+                                    acc[i][j][4 * q + 3]);
+                }
+            wave_lds_sync();
+#pragma unroll 1
+            for (int t = 0; t < 16; ++t) {
+                const int c = 16 * (t & 7) + (lane >> 2), r = 16 * (t >> 3) + 4 * (lane & 3);
+                const float4 v = *reinterpret_cast<const float4*>(st + c * PC + r);
+                const int row = m0 + wm0 + 32 * i + r, col = n0 + wn0 + c;
+                if (row < g.M && col < g.N) store4<SPLIT_OUT>(g, kz, row, col, v.x, v.y, v.z, v.w);
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
+// SPK: the K steps are the nk_list K blocks klist[0..nk_list) (sparse K), not 0..K/16.
+template <bool SPLIT_OUT, int SHAPE, bool SPK = false>
 __device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int n0, __bf16* s0,
-                                           __bf16* s1, __bf16* s2, __bf16* s3)
+                                           __bf16* s1, __bf16* s2, __bf16* s3,
+                                           const unsigned short* klist = nullptr, int nk_list = 0)
 {
     using SH = Shape<SHAPE>;
     constexpr int TBM = SH::WM * SH::TM * 32, TBN = SH::WN * SH::TN * 32;
@@ -422,8 +541,10 @@ __device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int 
         if (is_a) {
             int m = m0 + (q < CA ? r : 0);
             if (m >= g.M) m = g.M - 1;        // rows past M feed only discarded rows of C
-            const unsigned sl = (unsigned)m / (unsigned)g.a_R;
-            const long arow = (long)sl * g.a_SS + 16L * (m - (int)sl * g.a_R);
+            const unsigned g2 = (unsigned)m / (unsigned)g.a_R2;
+            const int m2 = m - (int)g2 * g.a_R2;
+            const unsigned sl = (unsigned)m2 / (unsigned)g.a_R;
+            const long arow = (long)g2 * g.a_SS2 + (long)sl * g.a_SS + 16L * (m2 - (int)sl * g.a_R);
             src[q] = g.A + arow + (wave ? g.a_lo : 0) + 8 * c;
         } else {
             src[q] = g.B + (wave == 3 ? g.b_lo : 0) + 16L * (n0 + (q < CB ? r : 0)) + 8 * c;
@@ -431,8 +552,8 @@ __device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int 
     }
     const long kstride = is_a ? g.a_KS : g.b_KS;   // elements per K block
     const int kz = blockIdx.y, kbeg = g.zk ? kz * g.kc : 0;
-    // >= 1: the host launches no empty chunk
-    const int nk = g.zk ? min(g.kc, g.K / BK - kbeg) : g.K / BK;
+    // >= 1: the host launches no empty chunk; a sparse-K tile may have none (zero output)
+    const int nk = SPK ? nk_list : g.zk ? min(g.kc, g.K / BK - kbeg) : g.K / BK;
     const long zoff = kbeg * kstride + (long)kz * (is_a ? g.a_ZS : g.b_ZS);
 #pragma unroll
     for (int q = 0; q < CMAX; ++q) src[q] += zoff;
@@ -441,7 +562,8 @@ __device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int 
     // last step again (into a stage whose fragments are read but not used), so that
     // every step has the same number of copies in flight
     auto issue = [&](__bf16* st, int k) {
-        const long kk = (k < nk ? k : nk - 1) * kstride;
+        const int kc = k < nk ? k : nk - 1;
+        const long kk = (SPK ? (long)klist[kc] : (long)kc) * kstride;
 #pragma unroll
         for (int q = 0; q < CMAX; ++q)
             if (q < ncopy) lds_copy16(src[q] + kk, st + wofs + 512 * q);
@@ -526,23 +648,25 @@ __device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int 
 
     // prologue: steps 0..3 in flight; once step 0 has landed its fragments are read, and
     // the second barrier retires those reads (stage 0 may be refilled) with step 1 landed
-    issue(s0, 0);
-    issue(s1, 1);
-    issue(s2, 2);
-    issue(s3, 3);
-    FC3_RING_BARRIER(3)
-    FC3_READ(s0, ah0, al0, bh0, bl0)
-    FC3_RING_BARRIER(2)
-    int k = 0;
-    for (; k + 4 <= nk; k += 4) {
-        FC3_EVEN(k, s1, s0)
-        FC3_ODD(k + 1, s2, s1)
-        FC3_EVEN(k + 2, s3, s2)
-        FC3_ODD(k + 3, s0, s3)
-    }
-    if (k < nk) FC3_EVEN(k, s1, s0)
-    if (k + 1 < nk) FC3_ODD(k + 1, s2, s1)
-    if (k + 2 < nk) FC3_EVEN(k + 2, s3, s2)
+    // (a sparse-K tile without K blocks skips the loop: its output is zero)
+    if (!SPK || nk > 0) {
+        issue(s0, 0);
+        issue(s1, 1);
+        issue(s2, 2);
+        issue(s3, 3);
+        FC3_RING_BARRIER(3)
+        FC3_READ(s0, ah0, al0, bh0, bl0)
+        FC3_RING_BARRIER(2)
+        int k = 0;
+        for (; k + 4 <= nk; k += 4) {
+            FC3_EVEN(k, s1, s0)
+            FC3_ODD(k + 1, s2, s1)
+            FC3_EVEN(k + 2, s3, s2)
+            FC3_ODD(k + 3, s0, s3)
+        }
+        if (k < nk) FC3_EVEN(k, s1, s0)
+        if (k + 1 < nk) FC3_ODD(k + 1, s2, s1)
+        if (k + 2 < nk) FC3_EVEN(k + 2, s3, s2)
 #undef FC3_EVEN
 #undef FC3_ODD
 #undef FC3_STEP4
@@ -550,9 +674,20 @@ __device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int 
 #undef FC3_READ
 #undef FC3_RING_BARRIER
 #undef FC3_MMA3
-    wait_vmcnt0();   // no copy may still be landing when the workgroup's LDS is released
+        wait_vmcnt0();   // no copy may still be landing when the workgroup's LDS is released
 
+    }
     // epilogue: register r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+    if constexpr (SHAPE == 0) {
+        if (g.epi != 0) {
+            staged_epilogue<SPLIT_OUT>(g, m0, n0, wm0, wn0, kz, acc,
+                                       reinterpret_cast<float*>(wave == 0   ? s0
+                                                                : wave == 1 ? s1
+                                                                : wave == 2 ? s2
+                                                                            : s3));
+            return;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < SH::TM; ++i)
 #pragma unroll
@@ -594,7 +729,7 @@ constexpr int kStageE = 2 * BM * BK + 2 * 256 * BK;
 // a 256 x 32/64/128 tile of that shape (the N % 256 columns), in the same launch, so its
 // A panel is read while the neighbouring 256-wide tiles of its row group hold it in the
 // caches.
-template <bool SPLIT_OUT, int NARROW>
+template <bool SPLIT_OUT, int NARROW, bool SPK = false>
 __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
 {
     // Tile order: each XCD takes a contiguous range of tiles (workgroups are dealt to the
@@ -617,13 +752,41 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
     __shared__ __attribute__((aligned(16))) __bf16 s1[kStageE];
     __shared__ __attribute__((aligned(16))) __bf16 s2[kStageE];
     __shared__ __attribute__((aligned(16))) __bf16 s3[kStageE];
-    if constexpr (NARROW != 0) {
-        if (tn == g.tiles_n - 1) {
-            gemm3_tile<SPLIT_OUT, NARROW>(g, m0, n0, s0, s1, s2, s3);
-            return;
+    if constexpr (SPK) {
+        // this row tile's nonzero K blocks, in ascending order (wave 0 compacts the mask)
+        __shared__ unsigned short klist[kMaxKBlocks];
+        __shared__ int kcount;
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            const unsigned char* mk = g.kmask + (long)(tm / g.kmask_div) * g.kmask_ld;
+            int c = 0;
+            for (int b0 = 0; b0 < g.kmask_ld; b0 += 64) {
+                const int kb = b0 + lane;
+                const bool f = kb < g.kmask_ld && mk[kb] != 0;
+                const unsigned long long m = __ballot(f);
+                if (f) klist[c + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)kb;
+                c += __popcll(m);
+            }
+            if (lane == 0) kcount = c;
         }
+        __syncthreads();
+        const int nkl = kcount;
+        if constexpr (NARROW != 0) {
+            if (tn == g.tiles_n - 1) {
+                gemm3_tile<SPLIT_OUT, NARROW, true>(g, m0, n0, s0, s1, s2, s3, klist, nkl);
+                return;
+            }
+        }
+        gemm3_tile<SPLIT_OUT, 0, true>(g, m0, n0, s0, s1, s2, s3, klist, nkl);
+    } else {
+        if constexpr (NARROW != 0) {
+            if (tn == g.tiles_n - 1) {
+                gemm3_tile<SPLIT_OUT, NARROW>(g, m0, n0, s0, s1, s2, s3);
+                return;
+            }
+        }
+        gemm3_tile<SPLIT_OUT, 0>(g, m0, n0, s0, s1, s2, s3);
     }
-    gemm3_tile<SPLIT_OUT, 0>(g, m0, n0, s0, s1, s2, s3);
 }
 
 // Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K) -> the K-blocked A
@@ -632,23 +795,36 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
 // blockIdx.x, 256 rows blockIdx.y): thread r loads its row's 16 values (each load 1 KB
 // contiguous over the block) and stores them as two 32-byte runs per plane, so the
 // block writes 8 KB contiguous per plane and needs no LDS.
+// With kmask, the block also records whether any of its values is nonzero, at
+// kmask[blockIdx.y nkb + blockIdx.x] (the mode-3 GEMM's row tile and K block), and an
+// all-zero block is not written at all: the GEMM skips it.
 __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict__ psi, long n2,
                                                           int K, int Kp, long r0, long rows,
-                                                          __bf16* __restrict__ X)
+                                                          __bf16* __restrict__ X,
+                                                          unsigned char* __restrict__ kmask,
+                                                          int nkb)
 {
     const int kb = blockIdx.x;
     const long r = (long)blockIdx.y * 256 + threadIdx.x;
-    if (r >= rows) return;
+    const bool valid = r < rows;
     const int k0 = 16 * kb;
     bf16x8 h0, h1, l0, l1;
+    bool nz = false;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const int k = k0 + j;
-        const float x = k < K ? psi[r0 + r + n2 * k] : 0.0f;
+        const float x = (valid && k < K) ? psi[r0 + r + n2 * k] : 0.0f;
+        nz |= x != 0.0f;
         __bf16 hh, ll;
         split_bf16(x, hh, ll);
         if (j < 8) { h0[j] = hh; l0[j] = ll; } else { h1[j - 8] = hh; l1[j - 8] = ll; }
     }
+    if (kmask) {   // every thread of the block reaches this barrier
+        const int any = __syncthreads_or(nz ? 1 : 0);
+        if (threadIdx.x == 0) kmask[(long)blockIdx.y * nkb + kb] = any ? 1 : 0;
+        if (!any) return;
+    }
+    if (!valid) return;
     const long off = (long)kb * 16 * rows + 16 * r;
     bf16x8* dh = reinterpret_cast<bf16x8*>(X + off);
     bf16x8* dl = reinterpret_cast<bf16x8*>(X + rows * Kp + off);
@@ -656,6 +832,73 @@ __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict
     dh[1] = h1;
     dl[0] = l0;
     dl[1] = l1;
+}
+
+// Mode-3 operand in the row order of U's K-blocked layout: Psi rows (i, j), j in [j0, j0 +
+// u) (r0 = n j0), as X rows m = (i / 16) 16 u + 16 (j - j0) + i % 16 for i < Kp (rows
+// i >= n, the K padding of U, are zero), split and K-blocked over k like
+// k_split_transpose, with rows = u Kp.  A 256-row GEMM tile is then 16 i x 16 j (for
+// u >= 16): its output column is one contiguous run of U.  Block (K block blockIdx.x, row
+// tile blockIdx.y); kmask and the skipped all-zero blocks as k_split_transpose.
+// With pat, rows (i, j) that hold a nonzero set pat[i + n j] = 1 (Psi's (i, j) pattern,
+// which the mode-1 operand U_l(i, j) = sum_k Psi[i, j, k] phi[k, l] shares for every l).
+__global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restrict__ psi, long n2,
+                                                            int n, int Kp, long r0, int u,
+                                                            __bf16* __restrict__ X,
+                                                            unsigned char* __restrict__ kmask,
+                                                            int nkb, unsigned char* __restrict__ pat)
+{
+    const int kb = blockIdx.x;
+    const long rows = (long)u * Kp;
+    const long m = (long)blockIdx.y * 256 + threadIdx.x;
+    const long ib = m / (16L * u), rem = m - ib * 16L * u;
+    const int jj = (int)(rem >> 4), i = (int)(ib * 16 + (rem & 15));
+    const bool inb = m < rows, valid = inb && i < n;
+    const int k0 = 16 * kb;
+    bf16x8 h0, h1, l0, l1;
+    bool nz = false;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int k = k0 + t;
+        const float x = (valid && k < n) ? psi[r0 + i + (long)n * jj + n2 * k] : 0.0f;
+        nz |= x != 0.0f;
+        __bf16 hh, ll;
+        split_bf16(x, hh, ll);
+        if (t < 8) { h0[t] = hh; l0[t] = ll; } else { h1[t - 8] = hh; l1[t - 8] = ll; }
+    }
+    if (pat && nz) pat[(long)i + (long)n * (r0 / n + jj)] = 1;   // same value from every writer
+    if (kmask) {   // every thread of the block reaches this barrier
+        const int any = __syncthreads_or(nz ? 1 : 0);
+        if (threadIdx.x == 0) kmask[(long)blockIdx.y * nkb + kb] = any ? 1 : 0;
+        if (!any) return;
+    }
+    if (!inb) return;
+    const long off = (long)kb * 16 * rows + 16 * m;
+    bf16x8* dh = reinterpret_cast<bf16x8*>(X + off);
+    bf16x8* dl = reinterpret_cast<bf16x8*>(X + rows * Kp + off);
+    dh[0] = h0;
+    dh[1] = h1;
+    dl[0] = l0;
+    dl[1] = l1;
+}
+
+// bytes [0, cnt) of p set to v
+__global__ void k_fill_u8(unsigned char* __restrict__ p, long cnt, unsigned char v)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (long)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
+// Mode-1 block mask from Psi's (i, j) pattern: T[jb nkb + kb] = 1 when some pat[i + n j]
+// with i in K block kb (i < n) and j in [16 jb, 16 jb + 16) is set.  Block (kb, jb).
+__global__ void __launch_bounds__(256) k_pattern_blocks(const unsigned char* __restrict__ pat, int n,
+                                                         int nkb, unsigned char* __restrict__ T)
+{
+    const int kb = blockIdx.x, jb = blockIdx.y;
+    const int i = 16 * kb + (threadIdx.x & 15), j = 16 * jb + (threadIdx.x >> 4);
+    const bool f = i < n && j < n && pat[(long)i + (long)n * j] != 0;
+    const int any = __syncthreads_or(f ? 1 : 0);
+    if (threadIdx.x == 0) T[(long)jb * nkb + kb] = any ? 1 : 0;
 }
 
 // cnt slabs T_l (n x n, T_l[i + n j] at src[l n^2 + i + n j]) -> the K-blocked A operand
@@ -900,7 +1143,7 @@ struct OutMap {
 
 // One k_gemm3 launch over g's M x N: 256-wide tile columns, the last one narrow (NARROW =
 // 1..3: 32, 64, 128 columns) when it holds N % 256 <= 128 columns.
-template <bool SPLIT, int NARROW>
+template <bool SPLIT, int NARROW, bool SPK = false>
 int launch3(hipStream_t s, int kind, v3::G3 g)
 {
     const int rem = g.N % 256;
@@ -912,30 +1155,31 @@ int launch3(hipStream_t s, int kind, v3::G3 g)
     if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
     if (g.nz < 1 || g.nz > 65535) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid y out of range");
     ProfScope prof(kind, s);
-    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, NARROW>), dim3((unsigned)grid, (unsigned)g.nz),
+    hipLaunchKernelGGL((v3::k_gemm3<SPLIT, NARROW, SPK>), dim3((unsigned)grid, (unsigned)g.nz),
                        dim3(v3::NT), 0, s, g);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
 
-template <bool SPLIT>
+template <bool SPLIT, bool SPK = false>
 int launch3_cols(hipStream_t s, int kind, const v3::G3& g)
 {
     // the N % 256 last columns: a narrow last tile column for <= 128 of them, else (or
     // none) all columns in 256-wide tiles
     const int rem = g.N % 256;
     const int shape = rem == 0 ? 0 : rem <= 32 ? 1 : rem <= 64 ? 2 : rem <= 128 ? 3 : 0;
-    if (shape == 1) return launch3<SPLIT, 1>(s, kind, g);
-    if (shape == 2) return launch3<SPLIT, 2>(s, kind, g);
-    if (shape == 3) return launch3<SPLIT, 3>(s, kind, g);
-    return launch3<SPLIT, 0>(s, kind, g);
+    if (shape == 1) return launch3<SPLIT, 1, SPK>(s, kind, g);
+    if (shape == 2) return launch3<SPLIT, 2, SPK>(s, kind, g);
+    if (shape == 3) return launch3<SPLIT, 3, SPK>(s, kind, g);
+    return launch3<SPLIT, 0, SPK>(s, kind, g);
 }
 
 // C(m, c) = sum_k A(m, k) phi(k, c); A(m, k) = A[(m / a_R) a_SS + (k / 16) a_KS + 16 (m % a_R)
 // + k % 16], lo at + a_lo.
 int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_SS, long a_KS,
           const Split3& B, const OutMap& o, bool split_out, long M, int N, int K, int kc = 0,
-          long c_ZS = 0, int batch = 0, long a_ZS = 0, long b_ZS = 0)
+          long c_ZS = 0, int batch = 0, long a_ZS = 0, long b_ZS = 0,
+          const unsigned char* kmask = nullptr, int kmask_div = 1, long a_R2 = 0, long a_SS2 = 0)
 {
     if (M >= (1L << 31) || a_R >= (1L << 31) || o.R >= (1L << 31))
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
@@ -943,6 +1187,7 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
                     a_KS % 8 == 0 && B.lo % 8 == 0 && B.ks % 8 == 0 && o.R % 4 == 0 &&
                     o.SS % 4 == 0 && o.KS % 4 == 0 && o.CS % 4 == 0 && o.lo % 4 == 0 &&
                     aligned16(A) && aligned16(B.b) && aligned16(o.C) && a_ZS % 8 == 0 &&
+                    a_SS2 % 8 == 0 &&
                     b_ZS % 8 == 0 && c_ZS % 4 == 0;
     if (!ok) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: v3 GEMM operand layout");
     v3::G3 g;
@@ -957,6 +1202,18 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
         g.nz = (K / v3::BK + g.kc - 1) / g.kc;
     }
     g.a_ZS = a_ZS; g.b_ZS = b_ZS; g.c_ZS = c_ZS;
+    g.kmask = kmask;
+    g.kmask_ld = K / v3::BK;
+    g.kmask_div = kmask_div > 0 ? kmask_div : 1;
+    if (a_R2 >= (1L << 30)) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: A row groups too large");
+    g.a_R2 = a_R2 > 0 ? (int)a_R2 : (1 << 30);
+    g.a_SS2 = a_R2 > 0 ? a_SS2 : 0;
+    g.epi = o.KS == 16 ? 1 : o.CS == 16 ? 2 : 0;
+    if (kmask) {   // sparse K: unsplit, unbatched products with split output
+        if (!split_out || batch > 0 || g.nz != 1 || g.kmask_ld > v3::kMaxKBlocks)
+            return set_error(FC3_ERR_UNSUPPORTED, "mcc3: sparse-K GEMM shape");
+        return launch3_cols<true, true>(s, kind, g);
+    }
     return split_out ? launch3_cols<true>(s, kind, g) : launch3_cols<false>(s, kind, g);
 }
 
@@ -975,7 +1232,10 @@ int slab_batch3(int n, size_t cap_bytes, long tiles_per_launch)
     const long cap = (long)(cap_bytes / (4 * (size_t)n * padk(n)));
     b = std::max(1L, std::min({b, cap, (long)n}));
     const long batches = (n + b - 1) / b;
-    return (int)((n + batches - 1) / batches);
+    b = (n + batches - 1) / batches;
+    // a multiple of 16 when n is one: every batch can then take the sparse mode 1
+    if (n % 16 == 0 && b >= 16) b = b / 16 * 16;
+    return (int)b;
 }
 
 // Mode-3 row block: whole columns j of Psi(i, j, :) (a multiple of n rows); the split
@@ -992,31 +1252,90 @@ long mode3_rows(int n, size_t cap_bytes, long tiles_per_launch)
 }
 
 // Psi rows [r0, r0 + rows) -> X (split, K-blocked, K padded) -> C(m, l) for those rows.
+// With kmask (mask_bytes(rows, n) bytes of scratch; split output only) the split records
+// which 256-row x 16-K blocks hold a nonzero and the GEMM multiplies only those: a force-
+// constant tensor is mostly zero blocks (LJ FCC 6x6x6: 3.5% of them are not), and the
+// skipped products are all exactly zero, so U is the same bit for bit.
+size_t mask_bytes(long rows, int n) { return Carve::need((size_t)((rows + 255) / 256) * (padk(n) / 16)); }
+
 int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, const Split3& B,
-                __bf16* X, const OutMap& o, bool split_out, int ncols = 0)
+                __bf16* X, const OutMap& o, bool split_out, int ncols = 0,
+                unsigned char* kmask = nullptr)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
+    if (kmask && (!split_out || Kp / 16 > v3::kMaxKBlocks)) kmask = nullptr;
     dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
-    hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows, X);
+    hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows, X,
+                       kmask, Kp / 16);
     FC3_HIP(hipGetLastError());
     return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, o, split_out,
-                 rows, ncols > 0 ? ncols : n, Kp);
+                 rows, ncols > 0 ? ncols : n, Kp, 0, 0, 0, 0, 0, kmask);
+}
+
+// Mode 3 of the out-of-place forms, Psi rows (i, j) with j in [j0, j0 + u) -> U's slabs
+// l < ncols: the rows go through k_split_transpose_u, so that a GEMM tile is 16 i x 16 j
+// and its output columns are contiguous runs of U (written through the staged epilogue);
+// U's K-padding rows i in [n, Kp) receive zeros.  X: u Kp rows (x_rows), kmask:
+// mask_bytes(u Kp, n) bytes or null (then every block is multiplied).
+long x_rows(long R, int n) { return R / n * padk(n); }
+
+int mode3_block_u(hipStream_t s, const float* psi, int n, long j0, int u, const Split3& B,
+                  __bf16* X, __bf16* U, long P, int ncols, unsigned char* kmask,
+                  unsigned char* pat = nullptr)
+{
+    const long n2 = (long)n * n;
+    const int Kp = padk(n);
+    if (kmask && Kp / 16 > v3::kMaxKBlocks) kmask = nullptr;
+    const long rows = (long)u * Kp;
+    dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
+    hipLaunchKernelGGL(v3::k_split_transpose_u, gt, dim3(256), 0, s, psi, n2, n, Kp, n * j0, u, X,
+                       kmask, Kp / 16, pat);
+    FC3_HIP(hipGetLastError());
+    // C(m, c), m = (i / 16) 16 u + 16 (j - j0) + i % 16 -> U_c(i, j) = U[c 2P + (i / 16) 16 n +
+    // 16 j + i % 16]
+    const OutMap oU{U + 16 * j0, 16L * u, 16L * n, 16, 2 * P, P};
+    return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, oU, true, rows,
+                 ncols, Kp, 0, 0, 0, 0, 0, kmask);
+}
+
+// Mode 3 skips the all-zero K blocks of Psi unless FC3_MCC3_DENSE_MODE3 is set to a
+// nonzero value (then every block is multiplied: the reference for checking that the
+// skipping changes nothing).
+bool mode3_skips_zero_blocks()
+{
+    const char* e = std::getenv("FC3_MCC3_DENSE_MODE3");
+    return !(e && *e && std::atoi(e) != 0);
 }
 
 // Modes 1 and 2 on slabs [l0, l0 + cnt): U holds them as the mode-1 A operand (slab
 // stride 2P bf16 with P = n Kp, lo at +P, K-blocked); S is scratch of the same layout; T
 // receives the FP32 slabs K3_l[b + n c].
+//
+// mask1 (n % 16 == 0 and cnt % 16 == 0; else ignored): the mode-1 block mask of
+// k_pattern_blocks, T[jb kbs + kb] for 16-row groups jb of j and K blocks kb of i.  Mode 1
+// then takes its rows in the order m = jb (16 cnt) + 16 l + j % 16 (16 j x 16 slabs per
+// 256-row tile), so that a tile's K blocks are those of 16 j only, and multiplies only the
+// K blocks T marks: U_l(i, j) is exactly zero wherever Psi[i, j, :] is (for every l), so
+// S is the same bit for bit.
 int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, __bf16* S,
-               float* T)
+               float* T, const unsigned char* mask1 = nullptr)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
     const long P = (long)n * Kp;
-    // A(l n + j, i) = U_l(i, j);  C(l n + j, b) = S_l(j, b) -> mode-2 A(l n + b, j)
-    const OutMap oS{S, n, 2 * P, 16L * n, 16, P};
-    FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, n, 2 * P, 16L * n, B, oS, true, (long)cnt * n,
-                  n, Kp));
+    if (mask1 && n % 16 == 0 && cnt % 16 == 0) {
+        // A(m, i) = U_l(i, j) = U[l 2P + (i / 16) 16 n + 16 j + i % 16], m as above;
+        // C(m, b) = S_l(j, b) at S[l 2P + (j / 16) 16 n + 16 b + j % 16] (mode-2 A operand)
+        const OutMap oS1{S, 16L * cnt, 16L * n, 2 * P, 16, P};
+        FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, 16, 2 * P, 16L * n, B, oS1, true,
+                      (long)cnt * n, n, Kp, 0, 0, 0, 0, 0, mask1, cnt / 16, 16L * cnt, 256));
+    } else {
+        // A(l n + j, i) = U_l(i, j);  C(l n + j, b) = S_l(j, b) -> mode-2 A(l n + b, j)
+        const OutMap oS{S, n, 2 * P, 16L * n, 16, P};
+        FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, n, 2 * P, 16L * n, B, oS, true,
+                      (long)cnt * n, n, Kp));
+    }
     // A(l n + b, j) = S_l(j, b);  C(l n + b, c) = K3_l[b + n c]
     const OutMap oT{T, n, n2, 16, n, 0};
     return gemm3(s, FC3_KERNEL_GEMM_MODE12, S, P, n, 2 * P, 16L * n, B, oT, false, (long)cnt * n,
@@ -1031,7 +1350,9 @@ size_t mcc3_v3_workspace_bytes(int n, bool inplace, long tiles)
     const size_t slabs = Carve::need(2 * sizeof(__bf16) * P * nb);
     const size_t u = (!inplace && padk(n) != n) ? Carve::need(2 * sizeof(__bf16) * P * n) : 0;
     return phi3_bytes(n) + slabs * (inplace ? 2 : 1) + u +
-           Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n)) + 4096;
+           Carve::need(2 * sizeof(__bf16) * (size_t)x_rows(R, n) * padk(n)) +
+           mask_bytes(x_rows(R, n), n) + Carve::need((size_t)n * n) +
+           Carve::need((size_t)(n / 16 + 1) * (padk(n) / 16)) + 4096;
 }
 
 int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3, void* ws,
@@ -1046,22 +1367,35 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
     const int nb = slab_batch3(n, kScratchSlabs, tiles);
     const long R = mode3_rows(n, kScratchRows, tiles);
     __bf16* S = cv.take<__bf16>((size_t)2 * P * nb);
-    __bf16* X = cv.take<__bf16>((size_t)2 * R * Kp);
+    __bf16* X = cv.take<__bf16>((size_t)2 * x_rows(R, n) * Kp);
     // U lives in the K3 buffer when its slabs have K3's size, else in scratch
     __bf16* U = Kp == n ? reinterpret_cast<__bf16*>(k3) : cv.take<__bf16>((size_t)2 * P * n);
+    const bool skip = mode3_skips_zero_blocks();
+    unsigned char* mask = skip ? cv.take<unsigned char>(mask_bytes(x_rows(R, n), n)) : nullptr;
+    // Psi's (i, j) pattern and the mode-1 block mask (n % 16 == 0 only)
+    const bool skip1 = skip && n % 16 == 0;
+    unsigned char* pat = skip1 ? cv.take<unsigned char>((size_t)n * n) : nullptr;
+    unsigned char* mask1 = skip1 ? cv.take<unsigned char>((size_t)(n / 16) * (Kp / 16)) : nullptr;
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 workspace too small");
-    FC3_TRY(zero_kpad(s, U, n, n));
     FC3_TRY(zero_kpad(s, S, n, nb));
+    if (pat) {
+        hipLaunchKernelGGL(v3::k_fill_u8, dim3(1024), dim3(256), 0, s, pat, n2, (unsigned char)0);
+        FC3_HIP(hipGetLastError());
+    }
+    // mode 3 by blocks of whole columns j (R is a multiple of n); it also writes U's
+    // K padding (zeros)
     for (long r0 = 0; r0 < n2; r0 += R) {
         const long rows = (n2 - r0) < R ? (n2 - r0) : R;
-        // C((i, j) - r0, l) -> U_l(i, j) = U[l 2P + (i / 16) 16 n + 16 j + i % 16]; a row
-        // block starts at a multiple of n (R is), so j = (m + r0) / n with m = row
-        const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * P, P};
-        FC3_TRY(mode3_block(s, psi, n, r0, rows, B, X, oU, true));
+        FC3_TRY(mode3_block_u(s, psi, n, r0 / n, (int)(rows / n), B, X, U, P, n, mask, pat));
+    }
+    if (mask1) {
+        hipLaunchKernelGGL(v3::k_pattern_blocks, dim3((unsigned)(Kp / 16), (unsigned)(n / 16)),
+                           dim3(256), 0, s, pat, n, Kp / 16, mask1);
+        FC3_HIP(hipGetLastError());
     }
     for (int l0 = 0; l0 < n; l0 += nb) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0));
+        FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0, mask1));
     }
     return FC3_OK;
 }
@@ -1307,7 +1641,7 @@ int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n
             // rows r < n, K index e < n^2 at stride n: src[r + n e]
             dim3 gt((unsigned)(n2 / 16), (unsigned)((n + 255) / 256));
             hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, src, (long)n, (int)n2,
-                               (int)n2, 0L, (long)n, dst);
+                               (int)n2, 0L, (long)n, dst, (unsigned char*)nullptr, 0);
         } else {
             const long q = n3 / 4;
             hipLaunchKernelGGL(v3::k_split_rows, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
@@ -1430,7 +1764,8 @@ size_t mcc3_blocked_bytes(int n, bool v3, int bs)
         const size_t Kp = padk(n), P = (size_t)n * Kp;
         const size_t nb = std::min<size_t>(bs, slab_batch3(n, kBlkSlabScratch, kTilesPerLaunch));
         return phi3_bytes(n, kBlkExtraCols) + Carve::need(4 * P * nb) +
-               Carve::need(4 * (size_t)blk_rows(n) * Kp) +
+               Carve::need(4 * (size_t)x_rows(blk_rows(n), n) * Kp) +
+               mask_bytes(x_rows(blk_rows(n), n), n) +
                ((int)Kp != n ? Carve::need(4 * P * bs) : 0) + 4096;
     }
     const size_t nb = std::min<size_t>(bs, slab_batch(n));
@@ -1453,17 +1788,17 @@ int mcc3_blocked_slabs(hipStream_t s, const float* psi, const float* phi, int n,
         const int nb = (int)std::min<long>(bs, slab_batch3(n, kBlkSlabScratch, kTilesPerLaunch));
         const long R = blk_rows(n);
         __bf16* S = cv.take<__bf16>((size_t)2 * P * nb);
-        __bf16* X = cv.take<__bf16>((size_t)2 * R * Kp);
+        __bf16* X = cv.take<__bf16>((size_t)2 * x_rows(R, n) * Kp);
         // U lives in the output block when its slabs have K3's size, else in scratch
         __bf16* U = Kp == n ? reinterpret_cast<__bf16*>(out) : cv.take<__bf16>((size_t)2 * P * bs);
+        unsigned char* mask =
+            mode3_skips_zero_blocks() ? cv.take<unsigned char>(mask_bytes(x_rows(R, n), n)) : nullptr;
         if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 blocked workspace too small");
-        FC3_TRY(zero_kpad(s, U, n, cnt));
         FC3_TRY(zero_kpad(s, S, n, nb));
         const Split3 Bc{B.b + 16L * l0, B.lo, B.ks};   // B(k, c) = phi[k, l0 + c]
         for (long r0 = 0; r0 < n2; r0 += R) {
             const long rows = (n2 - r0) < R ? (n2 - r0) : R;
-            const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * P, P};
-            FC3_TRY(mode3_block(s, psi, n, r0, rows, Bc, X, oU, true, cnt));
+            FC3_TRY(mode3_block_u(s, psi, n, r0 / n, (int)(rows / n), Bc, X, U, P, cnt, mask));
         }
         for (int c0 = 0; c0 < cnt; c0 += nb) {
             const int c = (cnt - c0) < nb ? (cnt - c0) : nb;

@@ -251,3 +251,41 @@ def test_mcc3_blocked_bounded_memory_bitwise(n, block_size, prec):
         assert (num / den) ** 0.5 <= TOL[prec]
     finally:
         other.close()
+
+
+@pytest.mark.parametrize("cells", [(4, 4, 4), (5, 5, 5)])
+def test_mcc3_mode3_zero_block_skipping_is_bitwise_neutral(ctx, cells, monkeypatch):
+    """BF16X3 mode 3 multiplies only the 256-row x 16-K blocks of Psi that hold a nonzero
+    (an LJ tensor is mostly zero blocks: 32% of them are nonzero at 768 DoF, 10% at 1500).
+    The skipped products are exactly zero, so K3 must equal, bit for bit, the result with
+    every block multiplied (FC3_MCC3_DENSE_MODE3=1), for both split-operand layouts
+    (n = 768, n % 16 == 0: U in the K3 buffer; n = 1500: U in scratch), unblocked and
+    blocked."""
+    sysd = S.fcc(S.ARGON_A, cells, S.ARGON_MASS, 0.05, seed=3)
+    calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
+    n = sysd.n_dof
+    _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
+    dphi = fc3.DeviceArray.from_host(phi, dtype=np.float32, ctx=ctx)
+    psi = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, S.ARGON_LJ, calc, psi32=psi, masses=sysd.masses, ctx=ctx)
+    monkeypatch.setenv("FC3_MCC3_DENSE_MODE3", "1")
+    dense = fc3.mcc3(psi, dphi, precision="bf16x3").to_host()
+    monkeypatch.delenv("FC3_MCC3_DENSE_MODE3")
+    sparse = fc3.mcc3(psi, dphi, precision="bf16x3").to_host()
+    assert np.array_equal(sparse, dense)
+    blocked = fc3.mcc3(psi, dphi, n // 4, precision="bf16x3")
+    assert np.array_equal(blocked, dense)
+    # and the value against the FP64 contraction on a few slabs
+    coo = O.lj_third_order_coo(sysd, S.ARGON_LJ, calc)
+    v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
+    p64 = phi.astype(np.float32).astype(np.float64)
+    num = den = 0.0
+    for l in (n // 3, n // 2, n - 1):
+        ref = O.mcc3_fp64_slab((coo[0], coo[1], coo[2], v32), p64, l)
+        num += float(np.sum((sparse[:, :, l].astype(np.float64) - ref) ** 2))
+        den += float(np.sum(ref * ref))
+    print(f"mode-3 block skipping FCC {cells}: bitwise equal, rel err {(num / den) ** 0.5:.2e}",
+          flush=True)
+    assert (num / den) ** 0.5 <= TOL["bf16x3"]
+    psi.free()
+    dphi.free()

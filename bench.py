@@ -60,8 +60,8 @@ def spread_slabs(w2, n, count):
 
 
 def gemm_traffic():
-    """HBM bytes per k_gemm3 launch at this workload, from the committed rocprofv3 PMC
-    summary (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy kernel), or
+    """HBM bytes per mode-2 k_gemm3 launch at this workload, from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a copy kernel), or
     None if that summary is absent."""
     path = os.path.join(ROOT, "profiles", "k_gemm3_lj_fcc666_traffic.md")
     try:
@@ -466,13 +466,13 @@ def main():
     results = {}
     runs = [("bf16x3", a.steps)] + ([("fp32", a.fp32_steps)] if a.fp32_steps > 0 else [])
     if a.fp32_steps > 0:
-        # BF16x3 with every mode-3 block multiplied: the same arithmetic as the FP32 form,
-        # which does not skip Psi's all-zero blocks (the like-for-like precision ratio)
-        runs.append(("bf16x3_dense_mode3", a.fp32_steps))
+        # BF16x3 with every block multiplied: the same arithmetic as the FP32 form, which
+        # does not skip the operands' all-zero blocks (the like-for-like precision ratio)
+        runs.append(("bf16x3_dense", a.fp32_steps))
     for key, steps in runs:
         prec = "bf16x3" if key.startswith("bf16x3") else key
-        if key == "bf16x3_dense_mode3":
-            os.environ["FC3_MCC3_DENSE_MODE3"] = "1"
+        if key == "bf16x3_dense":
+            os.environ["FC3_MCC3_DENSE"] = "1"
         for _ in range(a.warmup):
             fc3.mcc3(d32, dphi, precision=prec, out=k3)
         ctx.synchronize()
@@ -483,45 +483,51 @@ def main():
         t = ctx.timer_stop() / steps
         m3_ms, m3_c = profile(ctx, 1)
         m12_ms, m12_c = profile(ctx, 2)
+        m2_ms, m2_c = profile(ctx, 5)
         profile_enable(ctx, False)
-        os.environ.pop("FC3_MCC3_DENSE_MODE3", None)
+        os.environ.pop("FC3_MCC3_DENSE", None)
         err = slab_error()
         mul = 3.0 if prec == "bf16x3" else 1.0
         results[key] = {
             "ms_per_step": t, "TFLOP_per_s": flop / (t * 1e-3) / 1e12, "rel_fro_err": err,
             "err_sample": f"K3 slabs {slabs} vs FP64 host contraction of the same FP32 Psi, phi",
-            "gemm_ms_per_step": (m3_ms + m12_ms) / steps,
-            "gemm_launches_per_step": (m3_c + m12_c) // steps,
-            "mode3_launches_per_step": m3_c / steps, "mode12_launches_per_step": m12_c / steps,
-            "mode3_ms_per_step": m3_ms / steps, "mode12_ms_per_step": m12_ms / steps,
+            "gemm_ms_per_step": (m3_ms + m12_ms + m2_ms) / steps,
+            "gemm_launches_per_step": (m3_c + m12_c + m2_c) // steps,
+            "mode3_launches_per_step": m3_c / steps, "mode3_ms_per_step": m3_ms / steps,
+            # BF16x3: mode 1 under "mode12" and mode 2 on its own; FP32: both under "mode12"
+            "mode12_launches_per_step": m12_c / steps, "mode12_ms_per_step": m12_ms / steps,
+            "mode2_launches_per_step": m2_c / steps, "mode2_ms_per_step": m2_ms / steps,
             # MFMA flops the kernels issue (3 bf16 products per FP32-equivalent product)
             "mfma_flop_per_step": mul * flop,
         }
         log(f"mcc3 {key}: {results[key]}")
 
     bf = results["bf16x3"]
-    # Dominant kernel: the BF16x3 mode-product GEMM k_gemm3 on the dense operands of modes
-    # 1 and 2 (mode 3 multiplies only Psi's nonzero blocks and takes a few percent of the
-    # time).  Algorithmic work of those launches = 3 bf16 products x 4 n^4 (two of the three
-    # mode products); achieved = that per launch / their average launch duration, both from
-    # HIP events on the library's stream.
-    m12 = max(1, int(round(bf["mode12_launches_per_step"])))
-    f_launch = 3.0 * 4.0 * n ** 4 / m12
-    ms_launch = bf["mode12_ms_per_step"] / m12
+    # Dominant kernel: k_gemm3 in mode 2, the one dense mode product (modes 3 and 1 skip
+    # the all-zero blocks of Psi and of U).  Algorithmic work of those launches = 3 bf16
+    # products x 2 n^4; achieved = that per launch / their average launch duration, both
+    # from HIP events on the library's stream.
+    m2 = max(1, int(round(bf["mode2_launches_per_step"])))
+    f_launch = 3.0 * 2.0 * n ** 4 / m2
+    ms_launch = bf["mode2_ms_per_step"] / m2
     achieved = f_launch / (ms_launch * 1e-3) / 1e12
     traffic = gemm_traffic()
-    roofline = {"bound": "mfma", "kernel": "k_gemm3 (BF16x3), the mode-1 and mode-2 launches",
+    roofline = {"bound": "mfma", "kernel": "k_gemm3 (BF16x3), the mode-2 launches",
                 "achieved": achieved, "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / (PEAK["bf16"] / 1e12), "traffic": traffic,
                 "traffic_source": ("profiles/k_gemm3_lj_fcc666_traffic.md: rocprofv3 --pmc "
                                    "FETCH_SIZE and WRITE_SIZE in separate runs of this workload, "
                                    "copy-kernel calibrated; not measured in this run")
                 if traffic is not None else None,
-                "flop_per_launch": f_launch, "ms_per_launch": ms_launch, "launches_per_step": m12,
-                "mode3": {"launches_per_step": bf["mode3_launches_per_step"],
-                          "ms_per_step": bf["mode3_ms_per_step"],
-                          "note": "mode 3 multiplies only the 256-row x 16-K blocks of Psi that "
-                                  "hold a nonzero (bitwise the same result)"}}
+                "flop_per_launch": f_launch, "ms_per_launch": ms_launch, "launches_per_step": m2,
+                "other_modes": {
+                    "mode1_launches_per_step": bf["mode12_launches_per_step"],
+                    "mode1_ms_per_step": bf["mode12_ms_per_step"],
+                    "mode3_launches_per_step": bf["mode3_launches_per_step"],
+                    "mode3_ms_per_step": bf["mode3_ms_per_step"],
+                    "note": "modes 3 and 1 multiply only the 16-wide K blocks of Psi and of U "
+                            "that hold a nonzero (the same K3 bit for bit); dense: all blocks "
+                            "(mcc3.bf16x3_dense)"}}
     roofline_assembly = {"bound": "hbm", "kernel": "k_write_ifc3_pairwise",
                          "achieved": asm["writer_GB_per_s"], "peak": PEAK["hbm"] / 1e9,
                          "unit": "GB/s", "frac": asm["writer_GB_per_s"] / (PEAK["hbm"] / 1e9),
@@ -541,9 +547,9 @@ def main():
         "mcc3": results,
         "fp32_vs_bf16x3_speedup": (results["fp32"]["ms_per_step"] / bf["ms_per_step"]
                                    if "fp32" in results else None),
-        "fp32_vs_bf16x3_dense_mode3_speedup": (
-            results["fp32"]["ms_per_step"] / results["bf16x3_dense_mode3"]["ms_per_step"]
-            if "bf16x3_dense_mode3" in results else None),
+        "fp32_vs_bf16x3_dense_speedup": (
+            results["fp32"]["ms_per_step"] / results["bf16x3_dense"]["ms_per_step"]
+            if "bf16x3_dense" in results else None),
         "assembly": asm,
     }
     d32.free()

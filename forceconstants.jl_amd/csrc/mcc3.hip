@@ -1299,12 +1299,12 @@ int mode3_block_u(hipStream_t s, const float* psi, int n, long j0, int u, const 
                  ncols, Kp, 0, 0, 0, 0, 0, kmask);
 }
 
-// Mode 3 skips the all-zero K blocks of Psi unless FC3_MCC3_DENSE_MODE3 is set to a
-// nonzero value (then every block is multiplied: the reference for checking that the
+// Modes 3 and 1 skip the all-zero K blocks of their operands unless FC3_MCC3_DENSE is set
+// to a nonzero value (then every block is multiplied: the reference for checking that the
 // skipping changes nothing).
 bool mode3_skips_zero_blocks()
 {
-    const char* e = std::getenv("FC3_MCC3_DENSE_MODE3");
+    const char* e = std::getenv("FC3_MCC3_DENSE");
     return !(e && *e && std::atoi(e) != 0);
 }
 
@@ -1338,7 +1338,7 @@ int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, 
     }
     // A(l n + b, j) = S_l(j, b);  C(l n + b, c) = K3_l[b + n c]
     const OutMap oT{T, n, n2, 16, n, 0};
-    return gemm3(s, FC3_KERNEL_GEMM_MODE12, S, P, n, 2 * P, 16L * n, B, oT, false, (long)cnt * n,
+    return gemm3(s, FC3_KERNEL_GEMM_MODE2, S, P, n, 2 * P, 16L * n, B, oT, false, (long)cnt * n,
                  n, Kp);
 }
 

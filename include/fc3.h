@@ -78,10 +78,11 @@ int fc3_timer_stop(fc3_handle h, float* ms);   /* waits for the stream */
 typedef enum {
     FC3_KERNEL_IFC3_WRITE = 0,   /* dense IFC3 writer (ASR + tol + mass weight fused) */
     FC3_KERNEL_GEMM_MODE3 = 1,   /* mcc3 mode-3 GEMM (M = n^2) */
-    FC3_KERNEL_GEMM_MODE12 = 2,  /* mcc3 per-slab mode-1 and mode-2 GEMMs */
+    FC3_KERNEL_GEMM_MODE12 = 2,  /* mcc3 per-slab mode-1 GEMMs (and mode 2 on the FP32 kernels) */
     FC3_KERNEL_PAIR_BLOCKS = 3,  /* pair / triplet block evaluation */
     FC3_KERNEL_NEIGHBORS = 4,    /* neighbour list construction */
-    FC3_KERNEL_COUNT = 5
+    FC3_KERNEL_GEMM_MODE2 = 5,   /* mcc3 per-slab mode-2 GEMMs on the BF16x3 kernels */
+    FC3_KERNEL_COUNT = 6
 } fc3_kernel_kind;
 /* Enable (1) or disable (0) event timing of each kernel launch on this handle;
  * enabling clears previous records.  Not a reference interface: measurement aid. */

@@ -256,11 +256,12 @@ def test_mcc3_blocked_bounded_memory_bitwise(n, block_size, prec):
 @pytest.mark.parametrize("cells", [(4, 4, 4), (5, 5, 5)])
 def test_mcc3_mode3_zero_block_skipping_is_bitwise_neutral(ctx, cells, monkeypatch):
     """BF16X3 mode 3 multiplies only the 256-row x 16-K blocks of Psi that hold a nonzero
-    (an LJ tensor is mostly zero blocks: 32% of them are nonzero at 768 DoF, 10% at 1500).
-    The skipped products are exactly zero, so K3 must equal, bit for bit, the result with
-    every block multiplied (FC3_MCC3_DENSE_MODE3=1), for both split-operand layouts
-    (n = 768, n % 16 == 0: U in the K3 buffer; n = 1500: U in scratch), unblocked and
-    blocked."""
+    (an LJ tensor is mostly zero blocks: 9% of them are nonzero at 768 DoF, 3% at 1500),
+    and for n % 16 == 0 mode 1 only the blocks of U where Psi's (i, j) pattern is not
+    empty.  The skipped products are exactly zero, so K3 must equal, bit for bit, the
+    result with every block multiplied (FC3_MCC3_DENSE=1), for both split-operand layouts
+    (n = 768, n % 16 == 0, both modes skip, U in the K3 buffer; n = 1500: mode 3 skips, U
+    in scratch), unblocked and blocked."""
     sysd = S.fcc(S.ARGON_A, cells, S.ARGON_MASS, 0.05, seed=3)
     calc = S.AnalyticalCalculator(1e-12, S.ARGON_LJ.r_cut)
     n = sysd.n_dof
@@ -268,9 +269,9 @@ def test_mcc3_mode3_zero_block_skipping_is_bitwise_neutral(ctx, cells, monkeypat
     dphi = fc3.DeviceArray.from_host(phi, dtype=np.float32, ctx=ctx)
     psi = fc3.DeviceArray((n, n, n), np.float32, ctx)
     fc3.third_order_device(sysd, S.ARGON_LJ, calc, psi32=psi, masses=sysd.masses, ctx=ctx)
-    monkeypatch.setenv("FC3_MCC3_DENSE_MODE3", "1")
+    monkeypatch.setenv("FC3_MCC3_DENSE", "1")
     dense = fc3.mcc3(psi, dphi, precision="bf16x3").to_host()
-    monkeypatch.delenv("FC3_MCC3_DENSE_MODE3")
+    monkeypatch.delenv("FC3_MCC3_DENSE")
     sparse = fc3.mcc3(psi, dphi, precision="bf16x3").to_host()
     assert np.array_equal(sparse, dense)
     blocked = fc3.mcc3(psi, dphi, n // 4, precision="bf16x3")

@@ -162,6 +162,51 @@ def cpu_baseline_assembly(cells=5, reps=5):
                       f"OpenMP over atoms, median of {reps} runs after a warm-up: {dt:.2f} s"}
 
 
+def unique_timing(ctx, d32, dphi, k3, n, slabs, a, full_ms):
+    """Symmetric-unique K3 (mcc3_unique, BF16x3) on the headline Psi and phi: time per
+    call, and a bitwise check of its entries
+    against the full K3 that k3 holds (the BF16x3 runs above) on the sampled slabs."""
+    import ctypes
+    import fc3
+    from fc3._lib import check
+    du = fc3.DeviceArray((fc3.unique_count(n),), np.float32, ctx)
+    for _ in range(a.warmup):
+        fc3.mcc3_unique(d32, dphi, precision="bf16x3", out=du)
+    ctx.synchronize()
+    profile_enable(ctx)
+    ctx.timer_start()
+    for _ in range(a.steps):
+        fc3.mcc3_unique(d32, dphi, precision="bf16x3", out=du)
+    t = ctx.timer_stop() / a.steps
+    m3_ms, _ = profile(ctx, 1)
+    m1_ms, _ = profile(ctx, 2)
+    m2_ms, _ = profile(ctx, 5)
+    profile_enable(ctx, False)
+    buf = np.empty((n, n), dtype=np.float32, order="F")
+    mism = 0
+    for l in slabs:
+        check(ctx.lib.fc3_memcpy_d2h(ctx.h, buf.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.c_void_p(k3.ptr.value + 4 * n * n * l), 4 * n * n))
+        tri = np.ascontiguousarray(buf[: l + 1, : l + 1].T[np.tril_indices(l + 1)])
+        got = np.empty_like(tri)
+        lo = int(fc3.unique_index(0, 0, l))
+        check(ctx.lib.fc3_memcpy_d2h(ctx.h, got.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.c_void_p(du.ptr.value + 4 * lo), 4 * tri.size))
+        mism += int(np.count_nonzero(got.view(np.uint32) != tri.view(np.uint32)))
+    du.free()
+    ctx.release_workspace()   # the n^3 intermediate of the unique form
+    res = {"ms_per_step": t, "entries": fc3.unique_count(n),
+           "K3_equiv_TFLOP_per_s": 6.0 * n ** 4 / (t * 1e-3) / 1e12,
+           "speedup_vs_full_bf16x3": full_ms / t,
+           "mode3_ms_per_step": m3_ms / a.steps, "mode1_ms_per_step": m1_ms / a.steps,
+           "mode2_ms_per_step": m2_ms / a.steps,
+           "bitwise_mismatches": mism, "checked_slabs": slabs,
+           "note": "K3[a,b,l] for a <= b <= l (BF16x3): modes 1 and 2 run only the GEMM tiles "
+                   "holding such entries; checked bit for bit against the full K3"}
+    log(f"unique {res}")
+    return res
+
+
 def sw_assembly(ctx, steps=3):
     """IFC3 assembly for Stillinger-Weber silicon on diamond 3x3x3 (648 DoF, sigma_u =
     0.03 A, seed 648): FP64 Psi + mass-weighted FP32 Psi in one pass, 12 n^3 bytes."""
@@ -384,6 +429,8 @@ def main():
                     help="skip the fused third_order + mcc3 (no dense Psi) timing")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+backward timing at FCC 5x5x5 (1500 DoF)")
+    ap.add_argument("--no-unique", action="store_true",
+                    help="skip the symmetric-unique K3 (mcc3_unique) section")
     ap.add_argument("--no-sw", action="store_true",
                     help="skip the Stillinger-Weber assembly timing (diamond 3x3x3, 648 DoF)")
     a = ap.parse_args()
@@ -552,6 +599,8 @@ def main():
             if "bf16x3_dense" in results else None),
         "assembly": asm,
     }
+    if not a.no_unique:
+        out["unique"] = unique_timing(ctx, d32, dphi, k3, n, slabs, a, bf["ms_per_step"])
     d32.free()
     if not a.no_fused:
         # third_order + mcc3 without the dense Psi: sparse mode 1 from the pair blocks,

@@ -205,6 +205,16 @@ int fc3_destroy(fc3_handle h)
     return FC3_OK;
 }
 
+int fc3_release_workspace(fc3_handle h)
+{
+    FC3_TRY(check_handle(h));
+    FC3_HIP(hipStreamSynchronize(h->stream));
+    if (h->ws) FC3_HIP(hipFree(h->ws));
+    h->ws = nullptr;
+    h->ws_bytes = 0;
+    return FC3_OK;
+}
+
 int fc3_synchronize(fc3_handle h)
 {
     FC3_TRY(check_handle(h));
@@ -591,6 +601,25 @@ int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int prec
     const size_t ws = mcc3_workspace_bytes(n, precision, true);
     FC3_TRY(ensure_workspace(h, ws));
     return mcc3_run_inplace(h->stream, psi, phi, n, precision, h->ws, h->ws_bytes);
+}
+
+int fc3_mcc3_unique(fc3_handle h, const float* psi, const float* phi, int n, int precision,
+                    float* k3u)
+{
+    FC3_TRY(check_handle(h));
+    if (!psi || !phi || !k3u) return set_error(FC3_ERR_ARGUMENT, "psi, phi, k3u must be non-null");
+    if (n < 1) return set_error(FC3_ERR_ARGUMENT, "n must be >= 1");
+    if (precision != FC3_PREC_FP32 && precision != FC3_PREC_BF16X3)
+        return set_error(FC3_ERR_ARGUMENT, "unknown precision");
+    if ((long long)n * n >= (1LL << 31))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: n^2 must be < 2^31");
+    // the full pipeline's U and slab outputs take n^3 floats of workspace ahead of the
+    // mcc3 scratch
+    const size_t sb = Carve::need(sizeof(float) * (size_t)n * n * n);
+    FC3_TRY(ensure_workspace(h, sb + mcc3_unique_workspace_bytes(n, precision)));
+    char* ws = static_cast<char*>(h->ws);
+    return mcc3_unique_run(h->stream, psi, phi, n, precision, reinterpret_cast<float*>(ws), k3u,
+                           ws + sb, h->ws_bytes - sb);
 }
 
 int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n, int block_size,

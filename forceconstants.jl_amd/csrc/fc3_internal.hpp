@@ -174,6 +174,13 @@ bool mcc3_variant_applies(int n, int precision, int variant);
 size_t mcc3_workspace_bytes(int n, int precision, bool inplace, const Mcc3Config* cfg = nullptr);
 int mcc3_run(hipStream_t s, const float* psi, const float* phi, int n, int precision,
              float* k3, void* ws, size_t ws_bytes, const Mcc3Config* cfg = nullptr);
+// Symmetric-unique K3 (fc3_mcc3_unique): the n(n+1)(n+2)/6 entries K3[a, b, l], a <= b <= l,
+// at k3u[l(l+1)(l+2)/6 + b(b+1)/2 + a], each bitwise the full mcc3's entry; k3s is n^3
+// floats of device scratch (the full pipeline's U and slab outputs live there).
+long mcc3_unique_count(int n);
+size_t mcc3_unique_workspace_bytes(int n, int precision);
+int mcc3_unique_run(hipStream_t s, const float* psi, const float* phi, int n, int precision,
+                    float* k3s, float* k3u, void* ws, size_t ws_bytes);
 int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int precision,
                      void* ws, size_t ws_bytes);
 // Mode products of the mcc3 adjoint on the BF16X3 kernels, out of place, FP32 in and out.

@@ -17,6 +17,7 @@
 //                split for the next one) and copied global -> LDS without registers.
 #include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include "bf16_split.hpp"
 #include "fc3_internal.hpp"
@@ -345,7 +346,15 @@ struct G3 {
     // (c_CS == 16): the tile goes out through LDS so that consecutive lanes store
     // consecutive addresses
     int epi;
+    // optional explicit tile list: the launch computes only the tlist_n tiles
+    // tlist[t] = (tm << 8) | tn, in the list's order (instead of every tile in the grouped
+    // order); a tile's result does not depend on which other tiles a launch holds
+    const int* tlist;
+    int tlist_n;
 };
+
+// row tiles walked together, column by column, in the grouped tile order
+constexpr int kGM = 3;
 
 // K blocks a sparse-K tile can list (K <= 65536)
 constexpr int kMaxKBlocks = 4096;
@@ -739,12 +748,20 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
     // 4 678, 8 686, 16 732, 32 760; the LJ workload, same GPU: GM = 3 625 ms, 8 630 ms.
     const long wg = blockIdx.x;
     const long tile = (wg % kXcd) * g.per_xcd + wg / kXcd;
-    if (tile >= (long)g.tiles_m * g.tiles_n) return;
-    constexpr int GM = 3;
-    const long grp = tile / (GM * (long)g.tiles_n), first = grp * GM;
-    const int gsz = (int)((g.tiles_m - first) < GM ? (g.tiles_m - first) : GM);
-    const long w = tile - grp * GM * (long)g.tiles_n;
-    const int tm = (int)(first + w % gsz), tn = (int)(w / gsz);
+    int tm, tn;
+    if (g.tlist) {
+        if (tile >= g.tlist_n) return;
+        const int v = g.tlist[tile];
+        tm = v >> 8;
+        tn = v & 255;
+    } else {
+        if (tile >= (long)g.tiles_m * g.tiles_n) return;
+        const long grp = tile / (kGM * (long)g.tiles_n), first = grp * kGM;
+        const int gsz = (int)((g.tiles_m - first) < kGM ? (g.tiles_m - first) : kGM);
+        const long w = tile - grp * kGM * (long)g.tiles_n;
+        tm = (int)(first + w % gsz);
+        tn = (int)(w / gsz);
+    }
     const int m0 = tm * BM, n0 = tn * 256;
     // One LDS object per stage: the compiler then sees that the fragment reads of one
     // stage do not alias the copies in flight into the others, and adds no wait.
@@ -1149,7 +1166,8 @@ int launch3(hipStream_t s, int kind, v3::G3 g)
     const int rem = g.N % 256;
     g.tiles_m = (int)((g.M + v3::BM - 1) / v3::BM);
     g.tiles_n = NARROW ? (g.N - rem) / 256 + 1 : (g.N + 255) / 256;
-    const long tiles = (long)g.tiles_m * g.tiles_n;
+    const long tiles = g.tlist ? (long)g.tlist_n : (long)g.tiles_m * g.tiles_n;
+    if (tiles == 0) return FC3_OK;
     g.per_xcd = (tiles + kXcd - 1) / kXcd;
     const long grid = g.per_xcd * kXcd;
     if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
@@ -1179,7 +1197,8 @@ int launch3_cols(hipStream_t s, int kind, const v3::G3& g)
 int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_SS, long a_KS,
           const Split3& B, const OutMap& o, bool split_out, long M, int N, int K, int kc = 0,
           long c_ZS = 0, int batch = 0, long a_ZS = 0, long b_ZS = 0,
-          const unsigned char* kmask = nullptr, int kmask_div = 1, long a_R2 = 0, long a_SS2 = 0)
+          const unsigned char* kmask = nullptr, int kmask_div = 1, long a_R2 = 0, long a_SS2 = 0,
+          const int* tlist = nullptr, int tlist_n = 0)
 {
     if (M >= (1L << 31) || a_R >= (1L << 31) || o.R >= (1L << 31))
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
@@ -1209,6 +1228,10 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
     g.a_R2 = a_R2 > 0 ? (int)a_R2 : (1 << 30);
     g.a_SS2 = a_R2 > 0 ? a_SS2 : 0;
     g.epi = o.KS == 16 ? 1 : o.CS == 16 ? 2 : 0;
+    g.tlist = tlist;
+    g.tlist_n = tlist ? tlist_n : 0;
+    if (tlist && (batch > 0 || g.nz != 1 || (N + 255) / 256 > 256))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: tile list shape");
     if (kmask) {   // sparse K: unsplit, unbatched products with split output
         if (!split_out || batch > 0 || g.nz != 1 || g.kmask_ld > v3::kMaxKBlocks)
             return set_error(FC3_ERR_UNSUPPORTED, "mcc3: sparse-K GEMM shape");
@@ -1318,8 +1341,12 @@ bool mode3_skips_zero_blocks()
 // 256-row tile), so that a tile's K blocks are those of 16 j only, and multiplies only the
 // K blocks T marks: U_l(i, j) is exactly zero wherever Psi[i, j, :] is (for every l), so
 // S is the same bit for bit.
+//
+// tl1 / tl2 (with counts): explicit tile lists of the mode-1 / mode-2 GEMMs (null: all
+// tiles).
 int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, __bf16* S,
-               float* T, const unsigned char* mask1 = nullptr)
+               float* T, const unsigned char* mask1 = nullptr, const int* tl1 = nullptr,
+               int ntl1 = 0, const int* tl2 = nullptr, int ntl2 = 0)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
@@ -1329,17 +1356,18 @@ int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, 
         // C(m, b) = S_l(j, b) at S[l 2P + (j / 16) 16 n + 16 b + j % 16] (mode-2 A operand)
         const OutMap oS1{S, 16L * cnt, 16L * n, 2 * P, 16, P};
         FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, 16, 2 * P, 16L * n, B, oS1, true,
-                      (long)cnt * n, n, Kp, 0, 0, 0, 0, 0, mask1, cnt / 16, 16L * cnt, 256));
+                      (long)cnt * n, n, Kp, 0, 0, 0, 0, 0, mask1, cnt / 16, 16L * cnt, 256,
+                      tl1, ntl1));
     } else {
         // A(l n + j, i) = U_l(i, j);  C(l n + j, b) = S_l(j, b) -> mode-2 A(l n + b, j)
         const OutMap oS{S, n, 2 * P, 16L * n, 16, P};
         FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, n, 2 * P, 16L * n, B, oS, true,
-                      (long)cnt * n, n, Kp));
+                      (long)cnt * n, n, Kp, 0, 0, 0, 0, 0, nullptr, 1, 0, 0, tl1, ntl1));
     }
     // A(l n + b, j) = S_l(j, b);  C(l n + b, c) = K3_l[b + n c]
     const OutMap oT{T, n, n2, 16, n, 0};
     return gemm3(s, FC3_KERNEL_GEMM_MODE2, S, P, n, 2 * P, 16L * n, B, oT, false, (long)cnt * n,
-                 n, Kp);
+                 n, Kp, 0, 0, 0, 0, 0, nullptr, 1, 0, 0, tl2, ntl2);
 }
 
 size_t mcc3_v3_workspace_bytes(int n, bool inplace, long tiles)
@@ -1355,8 +1383,76 @@ size_t mcc3_v3_workspace_bytes(int n, bool inplace, long tiles)
            Carve::need((size_t)(n / 16 + 1) * (padk(n) / 16)) + 4096;
 }
 
+// ---- symmetric-unique K3 (the "calculate only the unique part" TODO of src/mcc.jl:116-117)
+// The entries K3[a, b, l] with a <= b <= l, packed as out[l(l+1)(l+2)/6 + b(b+1)/2 + a]
+// (slab by slab, each slab's upper triangle column by column).  The BF16X3 path runs the
+// full pipeline's mode 3 and computes in modes 1 and 2 only the GEMM tiles that hold a
+// needed entry: mode 1 (S_l(j, b) = sum_i U_l(i, j) phi(i, b)) the column tiles with
+// some b <= l, mode 2 (K3_l(b, c)) the tiles with some b <= c <= l, about 1/2 and 1/6 of
+// their work.  Each needed entry comes out of the same tile, rows and K loop as in the full
+// mcc3, so it is the full K3's entry bit for bit.
+__global__ void k_pack_unique(const float* __restrict__ T, int n, int l0,
+                              float* __restrict__ out)
+{
+    const int b = blockIdx.x;
+    const long l = (long)l0 + blockIdx.y;
+    if (b > l) return;
+    const float* src = T + (size_t)blockIdx.y * n * n + (size_t)b * n;
+    float* dst = out + l * (l + 1) * (l + 2) / 6 + (long)b * (b + 1) / 2;
+    for (int a = threadIdx.x; a <= b; a += blockDim.x) dst[a] = src[a];
+}
+
+// Tiles (tm << 8 | tn) of a GEMM over M x N (256 x 256 tiles, the last column up to N - 1)
+// for which need(tm, row range, column range) holds, in the kernel's grouped order.
+template <class F>
+void push_tiles(std::vector<int>& v, long M, int N, F need)
+{
+    const int tiles_m = (int)((M + 255) / 256), tiles_n = (N + 255) / 256;
+    for (int first = 0; first < tiles_m; first += v3::kGM) {
+        const int gsz = std::min(v3::kGM, tiles_m - first);
+        for (int tn = 0; tn < tiles_n; ++tn)
+            for (int tm = first; tm < first + gsz; ++tm)
+                if (need(256L * tm, std::min(256L * tm + 255, M - 1), 256 * tn,
+                         std::min(256 * tn + 255, N - 1)))
+                    v.push_back((tm << 8) | tn);
+    }
+}
+
+// Mode-1 and mode-2 tile lists of the slab batch [l0, l0 + cnt) for the unique entries.
+// permuted: mode 1 takes its rows as m = jb (16 cnt) + 16 l + j % 16 (else m = l n + j);
+// mode 2 always as m = l n + b.
+void unique_tiles(int n, int l0, int cnt, bool permuted, std::vector<int>& t1,
+                  std::vector<int>& t2)
+{
+    const long M = (long)cnt * n;
+    push_tiles(t1, M, n, [&](long r0, long r1, int c0, int) {
+        long lmax;
+        if (permuted) {
+            lmax = (r0 % (16L * cnt)) / 16 + 15;
+        } else {
+            lmax = std::min(r1 / n, (long)cnt - 1);
+        }
+        return c0 <= l0 + lmax;   // some column b <= some slab l of the tile
+    });
+    push_tiles(t2, M, n, [&](long r0, long r1, int c0, int c1) {
+        for (long ll = r0 / n; ll <= std::min(r1 / n, (long)cnt - 1); ++ll) {
+            const long blo = std::max(r0 - ll * n, 0L);
+            if (std::max(blo, (long)c0) <= std::min((long)l0 + ll, (long)c1)) return true;
+        }
+        return false;   // no b <= c <= l in the tile
+    });
+}
+
+// Bytes of the tile lists of a whole unique mcc3 (upper bound).
+size_t unique_list_bytes(int n, int nb)
+{
+    const long tiles_n = (n + 255) / 256;
+    const long batches = (n + nb - 1) / nb;
+    return Carve::need(sizeof(int) * 2 * ((long)n * n / 256 + batches) * tiles_n);
+}
+
 int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3, void* ws,
-            size_t ws_bytes, long tiles)
+            size_t ws_bytes, long tiles, float* k3u = nullptr)
 {
     Carve cv(ws, ws_bytes);
     Split3 B;
@@ -1376,7 +1472,29 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
     const bool skip1 = skip && n % 16 == 0;
     unsigned char* pat = skip1 ? cv.take<unsigned char>((size_t)n * n) : nullptr;
     unsigned char* mask1 = skip1 ? cv.take<unsigned char>((size_t)(n / 16) * (Kp / 16)) : nullptr;
+    // unique entries only: every batch's mode-1 and mode-2 tile lists, uploaded once
+    const size_t list_cap = k3u ? unique_list_bytes(n, nb) / sizeof(int) : 0;
+    int* dl = k3u ? cv.take<int>(list_cap) : nullptr;
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 workspace too small");
+    std::vector<int> lists;
+    std::vector<long> off;   // per batch: mode-1 start, mode-2 start; then the end
+    if (k3u) {
+        for (int l0 = 0; l0 < n; l0 += nb) {
+            const int cnt = (n - l0) < nb ? (n - l0) : nb;
+            std::vector<int> t1, t2;
+            unique_tiles(n, l0, cnt, skip1 && cnt % 16 == 0, t1, t2);
+            off.push_back((long)lists.size());
+            lists.insert(lists.end(), t1.begin(), t1.end());
+            off.push_back((long)lists.size());
+            lists.insert(lists.end(), t2.begin(), t2.end());
+        }
+        off.push_back((long)lists.size());
+        if (lists.size() > list_cap) return set_error(FC3_ERR_OUT_OF_MEMORY, "mcc3 tile lists");
+        FC3_HIP(hipMemcpyAsync(dl, lists.data(), sizeof(int) * lists.size(),
+                               hipMemcpyHostToDevice, s));
+        // the host lists must outlive their copy (this waits for work already queued)
+        FC3_HIP(hipStreamSynchronize(s));
+    }
     FC3_TRY(zero_kpad(s, S, n, nb));
     if (pat) {
         hipLaunchKernelGGL(v3::k_fill_u8, dim3(1024), dim3(256), 0, s, pat, n2, (unsigned char)0);
@@ -1393,9 +1511,22 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
                            dim3(256), 0, s, pat, n, Kp / 16, mask1);
         FC3_HIP(hipGetLastError());
     }
-    for (int l0 = 0; l0 < n; l0 += nb) {
+    if (!k3u) {
+        for (int l0 = 0; l0 < n; l0 += nb) {
+            const int cnt = (n - l0) < nb ? (n - l0) : nb;
+            FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0, mask1));
+        }
+        return FC3_OK;
+    }
+    int bi = 0;
+    for (int l0 = 0; l0 < n; l0 += nb, bi += 2) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0, mask1));
+        const long a = off[bi], b = off[bi + 1], e = off[bi + 2];
+        FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0, mask1, dl + a,
+                           (int)(b - a), dl + b, (int)(e - b)));
+        hipLaunchKernelGGL(k_pack_unique, dim3((unsigned)n, (unsigned)cnt), dim3(256), 0, s,
+                           k3 + n2 * l0, n, l0, k3u);
+        FC3_HIP(hipGetLastError());
     }
     return FC3_OK;
 }
@@ -1450,6 +1581,33 @@ AdjScratch adj_scratch(int n)
 }
 
 }  // namespace
+
+long mcc3_unique_count(int n) { return (long)n * (n + 1) * (n + 2) / 6; }
+
+size_t mcc3_unique_workspace_bytes(int n, int precision)
+{
+    size_t b = mcc3_workspace_bytes(n, precision, false);
+    if (v3_applies(n, precision))
+        b += unique_list_bytes(n, slab_batch3(n, kScratchSlabs, kTilesPerLaunch));
+    return b;
+}
+
+int mcc3_unique_run(hipStream_t s, const float* psi, const float* phi, int n, int precision,
+                    float* k3s, float* k3u, void* ws, size_t ws_bytes)
+{
+    if (v3_applies(n, precision) && aligned16(psi) && aligned16(k3s))
+        return mcc3_v3(s, psi, phi, n, k3s, ws, ws_bytes, kTilesPerLaunch, k3u);
+    // other shapes and FP32: the full K3 into the scratch, then its unique entries
+    FC3_TRY(mcc3_run(s, psi, phi, n, precision, k3s, ws, ws_bytes));
+    for (int l0 = 0; l0 < n; l0 += 65535) {
+        const int cnt = std::min(65535, n - l0);
+        hipLaunchKernelGGL(k_pack_unique, dim3((unsigned)n, (unsigned)cnt), dim3(256), 0, s,
+                           k3s + (size_t)n * n * l0, n, l0, k3u);
+        FC3_HIP(hipGetLastError());
+    }
+    return FC3_OK;
+}
+
 
 bool mcc3_adjoint_v3_applies(int n, const void* a, const void* b, const void* c)
 {

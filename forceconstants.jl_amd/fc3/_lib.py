@@ -49,6 +49,7 @@ SIGNATURES = {
     "fc3_status_string": (ctypes.c_char_p, [_c_int]),
     "fc3_create": (_c_int, [ctypes.POINTER(_handle), _c_int]),
     "fc3_destroy": (_c_int, [_handle]),
+    "fc3_release_workspace": (_c_int, [_handle]),
     "fc3_synchronize": (_c_int, [_handle]),
     "fc3_get_device": (_c_int, [_handle, ctypes.POINTER(_c_int)]),
     "fc3_device_alloc": (_c_int, [_handle, ctypes.POINTER(_vp), _c_size]),
@@ -78,6 +79,10 @@ SIGNATURES = {
     "fc3_third_order_mcc3_lj": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double,
                                          _c_double, _c_double, _c_double, _dp, _vp, _vp]),
     "fc3_mass_weight_third_order": (_c_int, [_handle, _vp, _c_int, _c_int, _dp]),
+    "fc3_asr_satisfied": (_c_int, [_handle, _vp, _c_int, _c_int, _c_int, _c_double,
+                                   ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.POINTER(ctypes.c_longlong), _dp]),
+    "fc3_issymmetric2": (_c_int, [_handle, _vp, _c_int, ctypes.POINTER(ctypes.c_longlong)]),
     "fc3_save_third_order_txt": (_c_int, [_handle, _vp, _c_int, _c_int, ctypes.c_char_p,
                                           ctypes.c_char_p]),
     "fc3_save_second_order_txt": (_c_int, [_handle, _vp, _c_int, _c_int, ctypes.c_char_p,
@@ -85,6 +90,7 @@ SIGNATURES = {
     "fc3_format_float": (_c_int, [_c_double, ctypes.c_char_p, _c_int]),
     "fc3_mcc3": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _vp]),
     "fc3_mcc3_inplace": (_c_int, [_handle, _vp, _vp, _c_int, _c_int]),
+    "fc3_mcc3_unique": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _vp]),
     "fc3_mcc3_blocked": (_c_int, [_handle, _vp, _vp, _c_int, _c_int, _c_int, _fp]),
     "fc3_batch_third_order_mcc3_lj": (_c_int, [_handle, ctypes.POINTER(Snapshot), _c_int, _c_double,
                                                _c_double, _c_double, _c_double, _c_double, _c_int,

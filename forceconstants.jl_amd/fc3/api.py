@@ -66,6 +66,10 @@ class Context:
     def synchronize(self):
         check(self.lib.fc3_synchronize(self.h))
 
+    def release_workspace(self):
+        """Return the handle's workspace to the device (it regrows on demand)."""
+        check(self.lib.fc3_release_workspace(self.h))
+
     def timer_start(self):
         check(self.lib.fc3_timer_start(self.h))
 
@@ -195,12 +199,82 @@ def second_order_device(sys: SuperCellSystem, pot, calc, out: Optional[DeviceArr
     return out
 
 
-def second_order(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = None
-                 ) -> DenseForceConstants:
-    """second_order(sys, pot, calc) -> DenseForceConstants{2} (host values)."""
+def _as_device_f64(ifc, order: int, ctx: Context):
+    """(DeviceArray, temporary?) for a DeviceArray, DenseForceConstants or host array."""
+    if isinstance(ifc, DeviceArray):
+        n = ifc.shape[0]
+        return _check_buf(ifc, (n,) * order, np.float64, "ifc", ctx), False
+    vals = ifc.values if isinstance(ifc, DenseForceConstants) else ifc
+    vals = np.asarray(vals)
+    if vals.ndim != order or len(set(vals.shape)) != 1:
+        raise AssertionError(f"ifc must be n^{order}, got shape {vals.shape}")
+    return DeviceArray.from_host(np.asfortranarray(vals, dtype=np.float64), ctx=ctx), True
+
+
+def asr_satisfied(ifc, N_atoms: int, D: int, tol: float, verbose: bool = False,
+                  ctx: Optional[Context] = None) -> bool:
+    """asr_satisfied(ifc, N_atoms, D, tol; verbose) of asr.jl:43-74 for second- (n x n)
+    or third-order (n x n x n) force constants, on the GPU (fc3_asr_satisfied): false
+    where some sum(ifc[D(i-1)+a, b:D:end(, kk)]) > tol.  verbose prints the first failing
+    set in the reference's words and the number of failing sets."""
+    ctx = ctx or Context.default()
+    order = len(ifc.shape)
+    if order not in (2, 3):
+        raise AssertionError("asr_satisfied takes second- or third-order force constants")
+    if ifc.shape[0] != N_atoms * D:
+        raise AssertionError("Incorrect dimensions")
+    d, tmp = _as_device_f64(ifc, order, ctx)
+    cnt, first, fsum = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+    try:
+        check(ctx.lib.fc3_asr_satisfied(ctx.h, d.ptr, order, int(N_atoms), int(D), float(tol),
+                                        ctypes.byref(cnt), ctypes.byref(first),
+                                        ctypes.byref(fsum)))
+    finally:
+        if tmp:
+            d.free()
+    if verbose and cnt.value:
+        n = N_atoms * D
+        key = first.value
+        b, rs = key % D, key // D
+        row, sl = rs % n, rs // n
+        if order == 3:
+            print(f"=======\n On k = {sl // D + 1}, γ = {sl % D + 1} \n=======")
+        print(f"Set i = {row // D + 1}, α = {row % D + 1}, β = {b + 1} does not satisfy ASR. "
+              f"Sum was: {format_float(fsum.value)}")
+        print(f"({cnt.value} sets do not satisfy ASR)")
+    return cnt.value == 0
+
+
+def issymmetric(ifc2, ctx: Optional[Context] = None) -> bool:
+    """issymmetric(Phi) (ifc.jl:15) on the GPU: Phi[r, c] == Phi[c, r] for all r, c."""
+    ctx = ctx or Context.default()
+    d, tmp = _as_device_f64(ifc2, 2, ctx)
+    cnt = ctypes.c_longlong()
+    try:
+        check(ctx.lib.fc3_issymmetric2(ctx.h, d.ptr, d.shape[0], ctypes.byref(cnt)))
+    finally:
+        if tmp:
+            d.free()
+    return cnt.value == 0
+
+
+def second_order(sys: SuperCellSystem, pot, calc, n_threads: Optional[int] = None,
+                 check_asr: bool = False, check_symmetry: bool = False,
+                 ctx: Optional[Context] = None) -> DenseForceConstants:
+    """second_order(sys, pot, calc; n_threads, check_asr, check_symmetry) ->
+    DenseForceConstants{2} (host values), ifc.jl:3-18.  n_threads is accepted for the
+    reference's signature; the work runs on the GPU."""
+    ctx = ctx or Context.default()
     d = second_order_device(sys, pot, calc, ctx=ctx)
-    vals = d.to_host()
-    d.free()
+    try:
+        if check_asr and not asr_satisfied(d, sys.n_atoms, 3, calc.tol, ctx=ctx):
+            raise AssertionError("ASR not satisfied for second order force constants, "
+                                 "please report this")
+        if check_symmetry and not issymmetric(d, ctx=ctx):
+            raise AssertionError("IFC were not symmetric, please report this")
+        vals = d.to_host()
+    finally:
+        d.free()
     return DenseForceConstants(vals, _units(pot, 2), calc.tol)
 
 
@@ -257,14 +331,21 @@ def third_order_mcc3(sys: SuperCellSystem, pot, calc, phi: DeviceArray, masses=N
     return k3
 
 
-def third_order(sys: SuperCellSystem, pot, calc, ctx: Optional[Context] = None
-                ) -> DenseForceConstants:
-    """third_order(sys, pot, calc) -> DenseForceConstants{3} (host values)."""
+def third_order(sys: SuperCellSystem, pot, calc, n_threads: Optional[int] = None,
+                check_asr: bool = False, ctx: Optional[Context] = None) -> DenseForceConstants:
+    """third_order(sys, pot, calc; n_threads, check_asr) -> DenseForceConstants{3} (host
+    values), ifc.jl:20-35.  n_threads is accepted for the reference's signature."""
+    ctx = ctx or Context.default()
     n = sys.n_dof
     d = DeviceArray((n, n, n), np.float64, ctx)
-    third_order_device(sys, pot, calc, psi64=d, ctx=ctx)
-    vals = d.to_host()
-    d.free()
+    try:
+        third_order_device(sys, pot, calc, psi64=d, ctx=ctx)
+        if check_asr and not asr_satisfied(d, sys.n_atoms, 3, calc.tol, ctx=ctx):
+            raise AssertionError("ASR not satisfied for third order force constants, "
+                                 "please report this")
+        vals = d.to_host()
+    finally:
+        d.free()
     return DenseForceConstants(vals, _units(pot, 3), calc.tol)
 
 
@@ -373,6 +454,47 @@ def mcc3(psi: DeviceArray, phi: DeviceArray, block_size: Optional[int] = None,
     check(ctx.lib.fc3_mcc3_blocked(ctx.h, psi.ptr, phi.ptr, n, int(block_size), p,
                                    host.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
     return host
+
+
+def unique_count(n: int) -> int:
+    """Entries K3[a, b, l] with a <= b <= l: n (n + 1) (n + 2) / 6."""
+    return n * (n + 1) * (n + 2) // 6
+
+
+def unique_index(a, b, l):
+    """Position of K3[a, b, l] (a <= b <= l, 0-based) in the packed unique vector:
+    l (l + 1) (l + 2) / 6 + b (b + 1) / 2 + a."""
+    a, b, l = (np.asarray(x, dtype=np.int64) for x in (a, b, l))
+    return l * (l + 1) * (l + 2) // 6 + b * (b + 1) // 2 + a
+
+
+def mcc3_unique(psi: DeviceArray, phi: DeviceArray, precision="fp32",
+                out: Optional[DeviceArray] = None) -> DeviceArray:
+    """The symmetric-unique part of mcc3(Psi, phi) (the unique-part TODO of mcc.jl:116-117):
+    a device vector of n (n + 1) (n + 2) / 6 floats holding K3[a, b, l] for a <= b <= l at
+    unique_index(a, b, l), each bitwise the entry mcc3 computes (fc3_mcc3_unique)."""
+    ctx = psi.ctx
+    n = phi.shape[0]
+    _check_buf(psi, (n, n, n), np.float32, "mcc3_unique: Psi")
+    _check_buf(phi, (n, n), np.float32, "mcc3_unique: phi", ctx)
+    if out is not None:
+        _check_buf(out, (unique_count(n),), np.float32, "mcc3_unique: out", ctx)
+    out = out or DeviceArray((unique_count(n),), np.float32, ctx)
+    check(ctx.lib.fc3_mcc3_unique(ctx.h, psi.ptr, phi.ptr, n, _prec(precision), out.ptr))
+    return out
+
+
+def k3_from_unique(values: np.ndarray, n: int) -> np.ndarray:
+    """The full symmetric K3 (host, Fortran order) from the packed unique entries: every
+    permutation of (a, b, l) takes the entry of its sorted indices ("rotate to enforce
+    symmetry", mcc.jl:116-117)."""
+    v = np.asarray(values).reshape(-1)
+    if v.shape != (unique_count(n),):
+        raise AssertionError(f"values must hold {unique_count(n)} entries")
+    idx = np.indices((n, n, n), dtype=np.int64)
+    s = np.sort(idx.reshape(3, -1), axis=0)
+    full = v[unique_index(s[0], s[1], s[2])]
+    return np.asfortranarray(full.reshape(n, n, n))
 
 
 def mcc3_(psi: DeviceArray, phi: DeviceArray, precision="fp32") -> DeviceArray:

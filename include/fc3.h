@@ -59,6 +59,10 @@ const char* fc3_status_string(int status);
 int fc3_create(fc3_handle* out, int device);
 int fc3_destroy(fc3_handle h);
 int fc3_synchronize(fc3_handle h);
+/* Frees the handle's workspace (grown on demand by the calls that need scratch and
+ * otherwise kept until fc3_destroy) after the stream has finished.  Not a reference
+ * interface: returns device memory between large calls.                          */
+int fc3_release_workspace(fc3_handle h);
 int fc3_get_device(fc3_handle h, int* device);
 
 /* ---- device memory (owned by the caller) ----------------------------------- */
@@ -188,6 +192,25 @@ int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms,
 int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int dim,
                                 const double* masses);
 
+/* ---- checks of finished force constants (check_asr / check_symmetry) ----------- */
+/* asr_satisfied(ifc, N_atoms, D, tol) replaces src/asr.jl:43-74 (called by
+ * second_order and third_order with check_asr = true, src/ifc.jl:14,33).  ifc: device
+ * FP64, n^order, n = dim * n_atoms, column-major.  For every slice (order 3: the third
+ * index; order 2: the matrix), row D(i-1)+a and component b, s = sum(ifc[row, b:D:end])
+ * summed in FP64 in ascending column order; the rule fails where s > tol (signed, as in
+ * the reference).  *n_violations = the number of failing (slice, row, b); the first in
+ * the reference's loop order (slice, atom, a, b) is returned as the key
+ * (slice * n + row) * dim + b (0-based; -1 if none) with its sum.  The tensor is read
+ * once.  first_violation and first_sum may be NULL.                                 */
+int fc3_asr_satisfied(fc3_handle h, const double* ifc, int order, int n_atoms, int dim,
+                      double tol, long long* n_violations, long long* first_violation,
+                      double* first_sum);
+
+/* issymmetric(Phi) of second_order(...; check_symmetry = true), src/ifc.jl:15: the
+ * number of pairs r < c with Phi[r, c] != Phi[c, r] (exact comparison; NaN counts),
+ * Phi a device FP64 n x n matrix.                                                   */
+int fc3_issymmetric2(fc3_handle h, const double* ifc2, int n, long long* n_asymmetric);
+
 /* ---- text output (save_second_order / save_third_order, fmt = :txt) ----------- */
 /* save_third_order(Psi, N_atoms, D, outpath; fmt = :txt) replaces src/io.jl:39-62: the
  * device tensor psi (FP64, n^3, n = dim * n_atoms) written to the file `path` as the
@@ -219,6 +242,19 @@ int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precis
  * using a bounded workspace (at most about 4 GiB, O(n^2) slabs plus O(n) rows of
  * Psi) instead of the reference's two extra n^3 tensors.                         */
 int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int precision);
+
+/* Symmetric-unique K3: the part the TODO at src/mcc.jl:116-117 asks for ("can only
+ * calculate unique part and rotate to enforce symmetry").  k3u (device, n(n+1)(n+2)/6
+ * floats) receives K3[a,b,l] for a <= b <= l at k3u[l(l+1)(l+2)/6 + b(b+1)/2 + a] (slab
+ * by slab, the upper triangle column by column), each entry bit for bit the one fc3_mcc3
+ * writes at (a,b,l).  With BF16X3 (n % 4 == 0) only the GEMM tiles that hold such
+ * entries are computed in modes 1 and 2 (about 1/2 and 1/6 of their work); other
+ * shapes compute the full K3 and pack it.  For a symmetric Psi, K3 is symmetric and
+ * these entries determine it.  The handle's workspace grows by n^3 floats (the full
+ * pipeline's intermediate lives there; fc3_release_workspace returns it).  The tile
+ * lists are uploaded first, which waits for work already queued on the stream.     */
+int fc3_mcc3_unique(fc3_handle h, const float* psi, const float* phi, int n, int precision,
+                    float* k3u);
 
 /* mcc3(Psi, phi, block_size) replaces src/mcc.jl:89-120: same result as fc3_mcc3,
  * delivered into a HOST array k3_host (n^3 floats) in slabs of block_size, with the

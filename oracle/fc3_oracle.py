@@ -174,6 +174,36 @@ def mass_weight_third_order(psi: np.ndarray, masses) -> np.ndarray:
     return psi
 
 
+def asr_violations(ifc: np.ndarray, N_atoms: int, D: int, tol: float):
+    """asr_satisfied (asr.jl:43-74) as a list of failures: for each slice ifc[:, :, kk]
+    (the matrix itself for order 2), row D(i-1)+a and component b (0-based here), the
+    sum s = sum(ifc[row, b::D]) fails where s > tol (signed).  Returned in the
+    reference's loop order (slice, atom, a, b) as ((slice * n + row) * D + b, s); the
+    rule is satisfied when the list is empty.  Pure-Python loop over the sums: small
+    tensors only."""
+    v = np.asarray(ifc, dtype=np.float64)
+    n = v.shape[0]
+    assert n == N_atoms * D and all(x == n for x in v.shape)
+    slices = [v] if v.ndim == 2 else [v[:, :, kk] for kk in range(n)]
+    out = []
+    for sl, m in enumerate(slices):
+        for row in range(n):
+            for b in range(D):
+                acc = 0.0
+                for x in m[row, b::D]:
+                    acc += float(x)
+                if acc > tol:
+                    out.append(((sl * n + row) * D + b, acc))
+    return out
+
+
+def issymmetric_count(ifc2: np.ndarray) -> int:
+    """Pairs r < c with ifc2[r, c] != ifc2[c, r] (issymmetric, ifc.jl:15; NaN counts)."""
+    a = np.asarray(ifc2)
+    iu = np.triu_indices(a.shape[0], 1)
+    return int(np.count_nonzero(~(a[iu] == a.T[iu])))
+
+
 def sw_blocks3(r9, params, c0_fd=SW_C0_FD, c0_iik=SW_C0_IIK) -> np.ndarray:
     """7 x (3,3,3) blocks iij, ijj, iik, ikk, jjk, jkk, ijk at r_arr = [r_i; r_j; r_k]."""
     r9 = np.ascontiguousarray(r9, dtype=np.float64)

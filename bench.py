@@ -308,6 +308,43 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
             num += float((d * d).sum())
             den += float((ref * ref).sum())
         errs.append({"n": n, "seed": seed, "sigma_u": su, "rel_fro_err": (num / den) ** 0.5})
+    # F2: the same batch with every phi from the device eigensolver inside the call
+    # (phi buffers are outputs now); wall time of the whole call, second of two
+    dev = {}
+    for rep in range(2):
+        tw = time.perf_counter()
+        freqs = fc3.batch_third_order_mcc3(snaps, S.ARGON_LJ, calc, precision="bf16x3",
+                                           device_modes=True, ctx=ctx)
+        ctx.synchronize()
+        dev["ms_per_batch_with_phi"] = (time.perf_counter() - tw) * 1e3
+    dev["phi_setup_s"] = max(dev["ms_per_batch_with_phi"] - t, 0.0) * 1e-3
+    dev_errs = []
+    for i in (nsnap - 1, nsnap - 1 - per_size, nsnap - 1 - 2 * per_size):
+        sysd, dph, seed, su, w2 = meta[i]
+        n = sysd.n_dof
+        w_ref = w2
+        phi32 = dph.to_host()
+        coo = O.lj_third_order_coo(sysd, S.ARGON_LJ, calc)
+        v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
+        k3 = snaps[i][2]
+        num = den = 0.0
+        buf = np.empty((n, n), dtype=np.float32, order="F")
+        for l in spread_slabs(w_ref, n, 4):
+            ref = O.mcc3_fp64_slab((coo[0], coo[1], coo[2], v32), phi32.astype(np.float64), l)
+            check(ctx.lib.fc3_memcpy_d2h(ctx.h, buf.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_void_p(k3.ptr.value + 4 * n * n * l), 4 * n * n))
+            d = buf.astype(np.float64) - ref
+            num += float((d * d).sum())
+            den += float((ref * ref).sum())
+        dev_errs.append({"n": n, "seed": seed,
+                         "eig_max_rel_diff_vs_host": float(np.abs(freqs[i] - w_ref).max() /
+                                                           np.abs(w_ref).max()),
+                         "rel_fro_err": (num / den) ** 0.5})
+    dev["errors_sampled"] = dev_errs
+    dev["note"] = ("phi of all snapshots by IFC2 + mass weighting + rocSOLVER dsyevd "
+                   "(strided-batched over same-size snapshots) inside the batch call; "
+                   "K3 checked against the FP64 contraction with that phi")
+    log(f"batched device modes: {dev}")
     for bufs in out_bufs.values():
         for b in bufs:
             b.free()
@@ -317,7 +354,7 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
            "snapshots_per_s": nsnap / (t * 1e-3), "K3_TFLOP_per_s": flop / (t * 1e-3) / 1e12,
            "graph_nodes": nodes, "replays_timed": replays, "autotuned": tune,
            "errors_sampled": errs, "modes_setup_per_snapshot": modes,
-           "phi_setup_s": phi_s,
+           "phi_setup_s": phi_s, "device_modes": dev,
            "note": "assembly (FP64 blocks, mass-weighted FP32 Psi) + BF16x3 mcc3 per snapshot, "
                    "one graph launch per batch"}
     log(f"batched: {res}")

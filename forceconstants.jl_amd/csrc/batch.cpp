@@ -18,7 +18,9 @@
 // The graph holds no allocation, memset or host synchronisation, so replays are valid
 // as long as the plan (sizes, pointers, configurations) is unchanged; a call whose plan
 // differs captures a new graph.
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -150,6 +152,95 @@ int snapshot_work(hipStream_t s, char* base, const SnapPlan& P, const fc3_snapsh
     return mcc3_run(s, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &P.cfg);
 }
 
+// Device modes of every snapshot (FC3_BATCH_DEVICE_MODES): get_modes(dynamical_matrix(sys,
+// LJ, calc)) of src/dynmat.jl:5-33,123-131 on the device, the eigenvectors written to the
+// snapshot's phi in FP32 and the eigenvalues to its freqs_sq (host, if given) with the 3
+// of smallest magnitude set to zero.  IFC2 is built from the snapshot's neighbour list and
+// pair blocks in the batch workspace; the snapshots of one size share rocSOLVER
+// dsyevd_strided_batched calls (chunks of at most 4 GiB of matrices), which take the place
+// of one latency-bound eigensolver call per snapshot.  Runs before the graph, outside it.
+constexpr size_t kModesChunkBytes = size_t(4) << 30;
+constexpr int kRigidModes = 3;   // get_modes' default num_rigid_translation
+
+int batch_device_modes(fc3_context* h, char* base, const std::vector<SnapPlan>& P,
+                       const fc3_snapshot* snaps, int count, const Params& prm)
+{
+    hipStream_t s = h->stream;
+    std::map<long, std::vector<int>> by_n;
+    for (int i = 0; i < count; ++i) by_n[P[i].n].push_back(i);
+    for (const auto& kv : by_n) {
+        const int n = (int)kv.first, Nat = n / 3;
+        const std::vector<int>& idx = kv.second;
+        const size_t per = sizeof(double) * ((size_t)n * n + 2 * (size_t)n) + 64;
+        const size_t G = std::max<size_t>(1, std::min<size_t>(idx.size(), kModesChunkBytes / per));
+        for (size_t c0 = 0; c0 < idx.size(); c0 += G) {
+            const int g = (int)std::min<size_t>(G, idx.size() - c0);
+            size_t sv = 0;
+            FC3_TRY(dsyevd_batched_workspace_bytes(h, n, g, &sv));
+            const size_t need = Carve::need(sizeof(double) * (size_t)n * n * g) +
+                                2 * Carve::need(sizeof(double) * (size_t)n * g) +
+                                Carve::need(sizeof(int) * g) +
+                                Carve::need(sizeof(double) * 9 * (size_t)Nat) + Carve::need(sv) +
+                                4096;
+            FC3_TRY(ensure_workspace(h, need));
+            Carve cv(h->ws, h->ws_bytes);
+            double* A = cv.take<double>((size_t)n * n * g);
+            double* W = cv.take<double>((size_t)n * g);
+            double* E = cv.take<double>((size_t)n * g);
+            int* info = cv.take<int>(g);
+            double* S2 = cv.take<double>(9 * (size_t)Nat);
+            char* svw = cv.take<char>(sv);
+            if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "batch modes workspace");
+            for (int t = 0; t < g; ++t) {
+                const int i = idx[c0 + t];
+                const SnapPlan& p = P[i];
+                NeighborList nl;
+                nl.n_atoms = p.N;
+                nl.cap = p.cap;
+                nl.count = reinterpret_cast<int*>(base + p.off_count);
+                nl.nbr = reinterpret_cast<int*>(base + p.off_nbr);
+                nl.rev = reinterpret_cast<int*>(base + p.off_rev);
+                const double* pos = reinterpret_cast<const double*>(base + p.off_pos);
+                const double* m = reinterpret_cast<const double*>(base + p.off_m);
+                double* E3 = reinterpret_cast<double*>(base + p.off_E3);
+                double* E2 = reinterpret_cast<double*>(base + p.off_E2);
+                double* Ai = A + (size_t)t * n * n;
+                FC3_TRY(neighbor_fill(s, pos, p.N, snaps[i].box, prm.r_cut, nl));
+                FC3_TRY(lj_pair_blocks(s, pos, snaps[i].box, prm.sigma, prm.epsilon, nl, E3,
+                                       nullptr, E2, S2));
+                FC3_TRY(lj_write_ifc2(s, nl, E2, S2, prm.tol, Ai));
+                FC3_TRY(dynmat_scale(s, Ai, n, m));
+            }
+            FC3_TRY(eigen_strided_batched(h, A, n, g, W, E, info, svw, sv));
+            for (int t = 0; t < g; ++t)
+                FC3_TRY(to_f32(s, A + (size_t)t * n * n, (long)n * n, snaps[idx[c0 + t]].phi));
+            std::vector<double> hw((size_t)n * g);
+            std::vector<int> hinfo(g);
+            FC3_HIP(hipMemcpyAsync(hw.data(), W, sizeof(double) * hw.size(), hipMemcpyDeviceToHost, s));
+            FC3_HIP(hipMemcpyAsync(hinfo.data(), info, sizeof(int) * g, hipMemcpyDeviceToHost, s));
+            FC3_HIP(hipStreamSynchronize(s));
+            for (int t = 0; t < g; ++t) {
+                const int i = idx[c0 + t];
+                if (hinfo[t] != 0)
+                    return set_error(FC3_ERR_HIP, "snapshot " + std::to_string(i) +
+                                                      ": rocSOLVER dsyevd did not converge (info " +
+                                                      std::to_string(hinfo[t]) + ")");
+                if (!snaps[i].freqs_sq) continue;
+                double* w = snaps[i].freqs_sq;
+                std::copy(hw.begin() + (size_t)t * n, hw.begin() + (size_t)(t + 1) * n, w);
+                // get_modes: the eigenvalues of smallest magnitude (stable order, as
+                // sortperm) are set to zero; the vectors are untouched
+                std::vector<int> ord(n);
+                for (int a = 0; a < n; ++a) ord[a] = a;
+                std::stable_sort(ord.begin(), ord.end(),
+                                 [&](int a, int b) { return std::fabs(w[a]) < std::fabs(w[b]); });
+                for (int r = 0; r < std::min(kRigidModes, n); ++r) w[ord[r]] = 0.0;
+            }
+        }
+    }
+    return FC3_OK;
+}
+
 }  // namespace
 
 static int tune_size(fc3_context* h, BatchState* B, char* base, const SnapPlan& P,
@@ -219,6 +310,9 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
         if (s.masses)
             for (int a = 0; a < s.n_atoms; ++a)
                 if (!(s.masses[a] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+        if ((flags & FC3_BATCH_DEVICE_MODES) && !s.masses)
+            return set_error(FC3_ERR_ARGUMENT, "snapshot " + std::to_string(i) +
+                                                   ": device modes need the masses");
     }
     BatchState* B = state(h);
     hipStream_t st = h->stream;
@@ -326,6 +420,15 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
     }
     float* psi32 = reinterpret_cast<float*>(base + off_psi);
     void* mws = base + off_mws;
+
+    // ---- phi on the device (before tuning, which runs the snapshots) ----
+    if (flags & FC3_BATCH_DEVICE_MODES) {
+        const bool prof_on = h->prof.enabled;
+        h->prof.enabled = false;
+        const int rc = batch_device_modes(h, base, P, snaps, count, prm);
+        h->prof.enabled = prof_on;
+        FC3_TRY(rc);
+    }
 
     // ---- tune sizes not seen before ----
     const bool prof_was = h->prof.enabled;

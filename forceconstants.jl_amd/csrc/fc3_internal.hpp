@@ -236,6 +236,13 @@ int dsyevd_workspace_bytes(fc3_context* h, int n, size_t* bytes);
 int dynmat_eigen(fc3_context* h, double* A, int n, const double* d_masses, double* W, double* E,
                  int* info, double* h_w, void* solver_ws, size_t solver_ws_bytes);
 int to_f32(hipStream_t s, const double* a, long cnt, float* b);
+// D = A / sqrt(m_I m_J) in place (A: n x n, masses per atom on the device)
+int dynmat_scale(hipStream_t s, double* A, int n, const double* d_masses);
+// G same-size problems A + g n^2 at once (rocSOLVER dsyevd_strided_batched): W + g n
+// eigenvalues ascending, eigenvectors in place, info[g]; no host synchronisation.
+int dsyevd_batched_workspace_bytes(fc3_context* h, int n, int G, size_t* bytes);
+int eigen_strided_batched(fc3_context* h, double* A, int n, int G, double* W, double* E,
+                          int* info, void* solver_ws, size_t solver_ws_bytes);
 
 // ---- adjoints (adjoint.hip) ---------------------------------------------------------
 void release_blas(fc3_context* h);

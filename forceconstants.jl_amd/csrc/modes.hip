@@ -81,6 +81,56 @@ int dynmat_eigen(fc3_context* h, double* A, int n, const double* d_masses, doubl
     return FC3_OK;
 }
 
+int dynmat_scale(hipStream_t s, double* A, int n, const double* d_masses)
+{
+    const long nn = (long)n * n;
+    hipLaunchKernelGGL(k_dynmat, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, A, n, d_masses);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+int dsyevd_batched_workspace_bytes(fc3_context* h, int n, int G, size_t* bytes)
+{
+    void* hb = nullptr;
+    FC3_TRY(blas_handle(h, &hb));
+    rocblas_handle b = static_cast<rocblas_handle>(hb);
+    size_t sz = 0;
+    if (rocblas_start_device_memory_size_query(b) != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS device memory size query could not start");
+    (void)rocsolver_dsyevd_strided_batched(b, rocblas_evect_original, rocblas_fill_upper, n,
+                                           nullptr, n, (rocblas_stride)n * n, nullptr, n, nullptr,
+                                           n, nullptr, G);
+    if (rocblas_stop_device_memory_size_query(b, &sz) != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS device memory size query failed");
+    *bytes = sz;
+    return FC3_OK;
+}
+
+// G matrices A + g n^2 (upper triangles) -> eigenvalues W + g n (ascending) and
+// eigenvectors in place, one rocSOLVER call on the library's stream; info: G ints.
+// No host synchronisation: the caller reads W and info after the stream.
+int eigen_strided_batched(fc3_context* h, double* A, int n, int G, double* W, double* E,
+                          int* info, void* solver_ws, size_t solver_ws_bytes)
+{
+    void* hb = nullptr;
+    FC3_TRY(blas_handle(h, &hb));
+    rocblas_handle b = static_cast<rocblas_handle>(hb);
+    if (solver_ws_bytes &&
+        rocblas_set_workspace(b, solver_ws, solver_ws_bytes) != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS: setting the eigensolver workspace failed");
+    const rocblas_status st = rocsolver_dsyevd_strided_batched(
+        b, rocblas_evect_original, rocblas_fill_upper, n, A, n, (rocblas_stride)n * n, W, n, E, n,
+        info, G);
+    const rocblas_status st2 = solver_ws_bytes ? rocblas_set_workspace(b, nullptr, 0)
+                                               : rocblas_status_success;
+    if (st != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, std::string("rocSOLVER dsyevd (batched): ") +
+                                          rocblas_status_to_string(st));
+    if (st2 != rocblas_status_success)
+        return set_error(FC3_ERR_HIP, "rocBLAS: releasing the eigensolver workspace failed");
+    return FC3_OK;
+}
+
 int to_f32(hipStream_t s, const double* a, long cnt, float* b)
 {
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, a, cnt, b);

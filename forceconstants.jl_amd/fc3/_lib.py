@@ -37,11 +37,13 @@ class Snapshot(ctypes.Structure):
     """fc3_snapshot of fc3.h"""
     _fields_ = [("positions", ctypes.POINTER(ctypes.c_double)), ("n_atoms", ctypes.c_int),
                 ("box", ctypes.c_double * 3), ("masses", ctypes.POINTER(ctypes.c_double)),
-                ("phi", ctypes.c_void_p), ("k3", ctypes.c_void_p)]
+                ("phi", ctypes.c_void_p), ("k3", ctypes.c_void_p),
+                ("freqs_sq", ctypes.POINTER(ctypes.c_double))]
 
 
 FC3_BATCH_AUTOTUNE = 1
 FC3_BATCH_NO_GRAPH = 2
+FC3_BATCH_DEVICE_MODES = 4
 
 SIGNATURES = {
     "fc3_version": (_c_int, []),

@@ -632,10 +632,15 @@ def third_order_param_backward(sys: SuperCellSystem, pot, calc, dpsi: DeviceArra
 # ---- batched snapshots ------------------------------------------------------------------
 
 def batch_third_order_mcc3(snapshots, pot, calc, precision="bf16x3", autotune: bool = True,
-                           graph: bool = True, ctx: Optional[Context] = None):
+                           graph: bool = True, device_modes: bool = False,
+                           ctx: Optional[Context] = None):
     """For each (sys, phi, k3) in `snapshots`: Psi = third_order(sys, pot, calc) mass
     weighted to FP32 with sys.masses, then k3 <- mcc3(Psi, phi), all in one graph launch.
-    phi, k3: device FP32 arrays (n x n, n x n x n).  Returns once the work is enqueued."""
+    phi, k3: device FP32 arrays (n x n, n x n x n).  Returns once the work is enqueued.
+
+    device_modes: phi is an output: get_modes(dynamical_matrix(sys, pot, calc)) of each
+    snapshot is computed on the device first (rocSOLVER, batched over same-size
+    snapshots); returns the list of eigenvalue arrays (3 rigid modes zeroed)."""
     if not isinstance(pot, LJ):
         raise NotImplementedError("batched assembly implemented for LJ")
     snaps = list(snapshots)
@@ -643,7 +648,7 @@ def batch_third_order_mcc3(snapshots, pot, calc, precision="bf16x3", autotune: b
         raise AssertionError("empty batch")
     ctx = ctx or snaps[0][1].ctx
     arr = (_lib.Snapshot * len(snaps))()
-    keep = []
+    keep, freqs = [], []
     for i, (sys, phi, k3) in enumerate(snaps):
         n = sys.n_dof
         _check_buf(phi, (n, n), np.float32, f"snapshot {i}: phi", ctx)
@@ -658,11 +663,18 @@ def batch_third_order_mcc3(snapshots, pot, calc, precision="bf16x3", autotune: b
         arr[i].masses = _ptr(m)
         arr[i].phi = phi.ptr
         arr[i].k3 = k3.ptr
-    flags = (_lib.FC3_BATCH_AUTOTUNE if autotune else 0) | (0 if graph else _lib.FC3_BATCH_NO_GRAPH)
+        if device_modes:
+            w = np.zeros(n)
+            freqs.append(w)
+            arr[i].freqs_sq = _ptr(w)
+    flags = ((_lib.FC3_BATCH_AUTOTUNE if autotune else 0) |
+             (0 if graph else _lib.FC3_BATCH_NO_GRAPH) |
+             (_lib.FC3_BATCH_DEVICE_MODES if device_modes else 0))
     check(ctx.lib.fc3_batch_third_order_mcc3_lj(ctx.h, arr, len(snaps), pot.sigma, pot.epsilon,
                                                 pot.r_cut, calc.r_cut, calc.tol, _prec(precision),
                                                 flags))
     del keep
+    return freqs if device_modes else None
 
 
 def batch_graph_nodes(ctx: Optional[Context] = None) -> int:

@@ -270,13 +270,21 @@ typedef struct {
     int n_atoms;
     double box[3];             /* cubic/orthorhombic box, Angstrom */
     const double* masses;      /* host, n_atoms (mass-weighted Psi), or NULL */
-    const float* phi;          /* device, n x n FP32 modes (n = 3 n_atoms) */
+    float* phi;                /* device, n x n FP32 modes (n = 3 n_atoms): input, or
+                                  output with FC3_BATCH_DEVICE_MODES */
     float* k3;                 /* device, n^3 FP32 output */
+    double* freqs_sq;          /* host, n: with FC3_BATCH_DEVICE_MODES the eigenvalues of
+                                  get_modes (3 rigid modes zeroed), or NULL */
 } fc3_snapshot;
 
 enum {
     FC3_BATCH_AUTOTUNE = 1,    /* time the configurations of sizes not seen before */
-    FC3_BATCH_NO_GRAPH = 2     /* issue the kernels directly instead of one graph launch */
+    FC3_BATCH_NO_GRAPH = 2,    /* issue the kernels directly instead of one graph launch */
+    FC3_BATCH_DEVICE_MODES = 4 /* phi = get_modes(dynamical_matrix(sys, LJ, calc)) of every
+                                  snapshot (src/dynmat.jl:5-33,123-131, num_rigid_translation
+                                  = 3) on the device before the contraction: IFC2, mass
+                                  weighting and rocSOLVER dsyevd, batched over same-size
+                                  snapshots; needs the masses.  Synchronises the stream. */
 };
 
 /* For every snapshot: Psi = third_order(LJ, AnalyticalCalculator) mass weighted to FP32

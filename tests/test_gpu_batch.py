@@ -148,3 +148,44 @@ def test_batch_forced_variant_timed_sizes_match_fp64(ctx, tmp_path, variant):
             k3.free()
     finally:
         other.close()
+
+
+def test_batch_device_modes(ctx):
+    """FC3_BATCH_DEVICE_MODES: every snapshot's phi from the device eigensolver (batched
+    over same-size snapshots) inside the batch call, after a single-snapshot fc3_modes_lj
+    call in the same process.  Eigenvalues against the host FP64 get_modes (src/dynmat.jl:
+    123-131) of the oracle's IFC2; phi orthonormal and an eigenbasis of D (FP32 rounding);
+    K3 against the FP64 contraction of the oracle's Psi with that phi."""
+    spec = [((3, 3, 3), 11), ((2, 2, 2), 12), ((3, 3, 3), 13), ((4, 4, 4), 14), ((3, 3, 3), 15)]
+    first = S.fcc(S.ARGON_A, (3, 3, 3), S.ARGON_MASS, 0.05, seed=10)
+    w_single, phi_single = fc3.modes_device(first, LJ, CALC, ctx=ctx)
+    print("modes_device (single snapshot) done; batch with device modes next", flush=True)
+    snaps = []
+    for cells, seed in spec:
+        su = float(np.random.default_rng(seed).uniform(0.02, 0.10))
+        sysd = S.fcc(S.ARGON_A, cells, S.ARGON_MASS, su, seed=seed)
+        n = sysd.n_dof
+        snaps.append((sysd, fc3.DeviceArray((n, n), np.float32, ctx, zero=True),
+                      fc3.DeviceArray((n, n, n), np.float32, ctx)))
+    freqs = fc3.batch_third_order_mcc3(snaps, LJ, CALC, precision="bf16x3",
+                                       device_modes=True, ctx=ctx)
+    ctx.synchronize()
+    for (sysd, dphi, k3), w, (cells, seed) in zip(snaps, freqs, spec):
+        n = sysd.n_dof
+        dyn = O.dynamical_matrix(O.lj_second_order(sysd, LJ, CALC), sysd.masses)
+        w_ref, _ = O.get_modes(dyn)
+        scale = np.abs(w_ref).max()
+        assert np.abs(w - w_ref).max() <= 1e-11 * scale
+        assert set(np.flatnonzero(w == 0.0)) == set(np.flatnonzero(w_ref == 0.0))
+        phi = dphi.to_host().astype(np.float64)
+        assert np.abs(phi.T @ phi - np.eye(n)).max() <= 2e-6
+        lam = np.linalg.eigvalsh(dyn)                 # unzeroed eigenvalues, ascending
+        res = np.linalg.norm(dyn @ phi - phi * lam) / np.linalg.norm(dyn)
+        assert res <= 2e-6
+        err = _rel(k3.to_host(), _ref(sysd, phi.astype(np.float32)))
+        print(f"device modes {cells} seed {seed}: eig ok, K3 rel fro err {err:.3e}", flush=True)
+        assert err <= TOL["bf16x3"]
+        dphi.free()
+        k3.free()
+    phi_single.free()
+    assert w_single.shape == (81 * 4,)

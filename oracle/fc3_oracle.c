@@ -628,6 +628,23 @@ static void set_abc(double* p, long n, int a, int b, int c, const double* B)
     add_perm(p, n, c, b, a, B, 3, 2, 1);
 }
 
+/* Block providers of the SW assembly: the blocks themselves, or (g_sw_dparam >= 0, set
+ * by fco_sw_ifc3_dparam below) their derivatives with respect to one parameter. */
+static int g_sw_dparam = -1;
+static void sw_block3_dp(const double* r9, const swp_t* P, double c0, int g0, int g1, int g2,
+                         double* out27, int which);
+static void sw_pair_block3_dp(const double r[3], const swp_t* P, double* h3, int which);
+static void sw_b3(const double* r9, const swp_t* P, double c0, int g0, int g1, int g2, double* out)
+{
+    if (g_sw_dparam >= 0) sw_block3_dp(r9, P, c0, g0, g1, g2, out, g_sw_dparam);
+    else sw_block3(r9, P, c0, g0, g1, g2, out);
+}
+static void sw_pb3(const double r[3], const swp_t* P, double* h3)
+{
+    if (g_sw_dparam >= 0) sw_pair_block3_dp(r, P, h3, g_sw_dparam);
+    else sw_pair_blocks(r, P, NULL, h3);
+}
+
 /* SW third_order! with the AutoDiffCalculator (third_order_AD.jl:71-170) + ASR! +
  * apply_tols!.  c0_fd / c0_iik: cos(theta0) as baked into the FastDifferentiation
  * blocks and as used by the runtime-built iik block.                            */
@@ -648,7 +665,7 @@ int fco_sw_ifc3(const double* pos, int N, const double* L, const double* swv, do
             mirror(rij, L);
             if (!(sumsq(rij) < rc2)) continue;
             if (j > i) {
-                sw_pair_blocks(rij, &P, NULL, B);
+                sw_pb3(rij, &P, B);
                 add_plain(out, n, i, i, j, B, -1.0);
                 add_plain(out, n, i, j, i, B, -1.0);
                 add_plain(out, n, j, i, i, B, -1.0);
@@ -667,13 +684,13 @@ int fco_sw_ifc3(const double* pos, int N, const double* L, const double* swv, do
                     r9[3 + c] = pos[3 * i + c] - rij[c];
                     r9[6 + c] = pos[3 * i + c] - rik[c];
                 }
-                sw_block3(r9, &P, c0_fd, 0, 0, 1, B);  set_aab(out, n, i, j, B);     /* iij */
-                sw_block3(r9, &P, c0_fd, 0, 1, 1, B);  set_abb(out, n, j, i, B);     /* ijj */
-                sw_block3(r9, &P, c0_iik, 0, 0, 2, B); set_aab(out, n, i, k, B);     /* iik */
-                sw_block3(r9, &P, c0_fd, 0, 2, 2, B);  set_abb(out, n, k, i, B);     /* ikk */
-                sw_block3(r9, &P, c0_fd, 1, 1, 2, B);  set_aab(out, n, j, k, B);     /* jjk */
-                sw_block3(r9, &P, c0_fd, 1, 2, 2, B);  set_abb(out, n, k, j, B);     /* jkk */
-                sw_block3(r9, &P, c0_fd, 0, 1, 2, B);  set_abc(out, n, i, j, k, B);  /* ijk */
+                sw_b3(r9, &P, c0_fd, 0, 0, 1, B);  set_aab(out, n, i, j, B);     /* iij */
+                sw_b3(r9, &P, c0_fd, 0, 1, 1, B);  set_abb(out, n, j, i, B);     /* ijj */
+                sw_b3(r9, &P, c0_iik, 0, 0, 2, B); set_aab(out, n, i, k, B);     /* iik */
+                sw_b3(r9, &P, c0_fd, 0, 2, 2, B);  set_abb(out, n, k, i, B);     /* ikk */
+                sw_b3(r9, &P, c0_fd, 1, 1, 2, B);  set_aab(out, n, j, k, B);     /* jjk */
+                sw_b3(r9, &P, c0_fd, 1, 2, 2, B);  set_abb(out, n, k, j, B);     /* jkk */
+                sw_b3(r9, &P, c0_fd, 0, 1, 2, B);  set_abc(out, n, i, j, k, B);  /* ijk */
             }
         }
     asr3(out, N);
@@ -840,4 +857,234 @@ long fco_lj_ifc3_coo(const double* pos, int N, const double* L, double sigma, do
 #undef EMIT
     free(self);
     return m;
+}
+
+/* ========================================================================= */
+/* Forward-mode parameter derivatives of the third-order tensors: the same loops
+ * as fco_lj_ifc3 / fco_sw_ifc3, instantiated on dual numbers in one potential
+ * parameter, so that every entry carries d Psi / d p beside its value.  ASR! is
+ * linear and is applied to the derivatives as to the values; apply_tols! is not
+ * (an entry it zeroes has derivative zero: the caller masks with the values).
+ * The reference has no parameter derivatives (no ForwardDiff in Project.toml);
+ * these are the checker of the GPU parameter pullbacks.                        */
+
+/* ---- LJ: dual numbers a + b eps, eps^2 = 0 ---- */
+typedef struct { double v, d; } du;
+static du du_c(double x) { du r = {x, 0.0}; return r; }
+static du du_add(du a, du b) { du r = {a.v + b.v, a.d + b.d}; return r; }
+static du du_sub(du a, du b) { du r = {a.v - b.v, a.d - b.d}; return r; }
+static du du_mul(du a, du b) { du r = {a.v * b.v, a.v * b.d + a.d * b.v}; return r; }
+static du du_scale(du a, double s) { du r = {a.v * s, a.d * s}; return r; }
+static du du_divd(du a, double s) { du r = {a.v / s, a.d / s}; return r; }
+static du du_pow(du x, int n)
+{
+    du r = du_c(1.0);
+    for (int t = 0; t < n; ++t) r = du_mul(r, x);
+    return r;
+}
+/* LennardJones.jl:38-52 with sigma, epsilon dual */
+static du lj_d1_du(du s, du e, double r)
+{
+    du s12 = du_pow(s, 12), s6 = du_pow(s, 6);
+    return du_mul(du_scale(e, -1.0), du_sub(du_scale(du_divd(s12, ipow(r, 13)), 48.0),
+                                            du_scale(du_divd(s6, ipow(r, 7)), 24.0)));
+}
+static du lj_d2_du(du s, du e, double r)
+{
+    du s12 = du_pow(s, 12), s6 = du_pow(s, 6);
+    return du_mul(du_scale(e, 4.0), du_sub(du_scale(du_divd(s12, ipow(r, 14)), 156.0),
+                                           du_scale(du_divd(s6, ipow(r, 8)), 42.0)));
+}
+static du lj_d3_du(du s, du e, double r)
+{
+    du s12 = du_pow(s, 12), s6 = du_pow(s, 6);
+    return du_mul(du_scale(e, -4.0), du_sub(du_scale(du_divd(s12, ipow(r, 15)), 156.0 * 14.0),
+                                            du_scale(du_divd(s6, ipow(r, 9)), 42.0 * 8.0)));
+}
+/* third_order_analytical.jl:216-224 on duals */
+static du lj_phi3_du(du d1, du d2, du d3, double dist, const double r[3], int a, int b, int c)
+{
+    const double g = ((r[a] * r[b]) * r[c]) / ((dist * dist) * dist);
+    du t1 = du_scale(du_add(du_sub(d3, du_divd(du_scale(d2, 3.0), dist)),
+                            du_divd(du_scale(d1, 3.0), dist * dist)), g);
+    const double dl = ((b == c ? r[a] : 0.0) + (c == a ? r[b] : 0.0)) + (a == b ? r[c] : 0.0);
+    du t2 = du_scale(du_sub(d2, du_divd(d1, dist)), dl / (dist * dist));
+    return du_add(t1, t2);
+}
+
+/* d Psi / d sigma (which = 0) or d Psi / d epsilon (which = 1) of fco_lj_ifc3, before
+ * the tolerance pass.  Returns 0, 2 on the cutoff check, 1 on a bad `which`.    */
+int fco_lj_ifc3_dparam(const double* pos, int N, const double* L, double sigma, double eps,
+                       double pot_rcut, double rcut, int which, double* out)
+{
+    for (int c = 0; c < 3; ++c)
+        if (!(pot_rcut < L[c])) return 2;
+    if (which != 0 && which != 1) return 1;
+    const long n = 3L * N, n2 = n * n;
+    memset(out, 0, sizeof(double) * n2 * n);
+    const double rc2 = rcut * rcut;
+    const du s = {sigma, which == 0 ? 1.0 : 0.0}, e = {eps, which == 1 ? 1.0 : 0.0};
+#define P3(x, y, z) out[(x) + n * (y) + n2 * (z)]
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j)
+            for (int fam = 0; fam < 2; ++fam) {
+                double r[3];
+                if (fam == 0)
+                    for (int c = 0; c < 3; ++c) r[c] = pos[3 * i + c] - pos[3 * j + c];
+                else
+                    for (int c = 0; c < 3; ++c) r[c] = pos[3 * j + c] - pos[3 * i + c];
+                mirror(r, L);
+                double d2s = sumsq(r);
+                if (!(d2s < rc2)) continue;
+                double dist = sqrt(d2s);
+                du d1 = lj_d1_du(s, e, dist), d2 = lj_d2_du(s, e, dist), d3 = lj_d3_du(s, e, dist);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        for (int c = 0; c < 3; ++c) {
+                            double v = -lj_phi3_du(d1, d2, d3, dist, r, a, b, c).d;
+                            long ii, jj, kk;
+                            if (fam == 0) { ii = 3L * i + a; jj = 3L * i + b; kk = 3L * j + c; }
+                            else          { ii = 3L * i + a; jj = 3L * j + b; kk = 3L * j + c; }
+                            P3(ii, jj, kk) = v; P3(ii, kk, jj) = v;
+                            P3(jj, kk, ii) = v; P3(jj, ii, kk) = v;
+                            P3(kk, ii, jj) = v; P3(kk, jj, ii) = v;
+                        }
+            }
+#undef P3
+    asr3(out, N);
+    return 0;
+}
+
+/* ---- SW: order-3 hyper-dual numbers in the coordinates (e1, e2, e3) times a dual
+ * direction e4 in one parameter: 16 coefficients, basis index = bitmask of e1..e4,
+ * e_i^2 = 0.  The e1 e2 e3 e4 coefficient of a block entry is d/dp of the third
+ * derivative.                                                                   */
+typedef struct { double v[16]; } hq;
+static hq hq_c(double x) { hq r; memset(&r, 0, sizeof r); r.v[0] = x; return r; }
+static hq hq_add(hq a, hq b) { for (int i = 0; i < 16; ++i) a.v[i] += b.v[i]; return a; }
+static hq hq_sub(hq a, hq b) { for (int i = 0; i < 16; ++i) a.v[i] -= b.v[i]; return a; }
+static hq hq_scale(hq a, double s) { for (int i = 0; i < 16; ++i) a.v[i] *= s; return a; }
+static hq hq_mul(hq a, hq b)
+{
+    hq r; memset(&r, 0, sizeof r);
+    for (int i = 0; i < 16; ++i) {
+        if (a.v[i] == 0.0) continue;
+        for (int j = 0; j < 16; ++j) {
+            if (i & j) continue;
+            r.v[i | j] += a.v[i] * b.v[j];
+        }
+    }
+    return r;
+}
+/* f(a) from f and its first four derivatives at a.v[0] (the nilpotent part has at
+ * most four distinct factors, so the Taylor series ends at the fourth power)   */
+static hq hq_apply(hq a, double f0, double f1, double f2, double f3, double f4)
+{
+    hq d = a; d.v[0] = 0.0;
+    hq d2 = hq_mul(d, d), d3 = hq_mul(d2, d), d4 = hq_mul(d3, d);
+    hq r = hq_c(f0);
+    r = hq_add(r, hq_scale(d, f1));
+    r = hq_add(r, hq_scale(d2, f2 / 2.0));
+    r = hq_add(r, hq_scale(d3, f3 / 6.0));
+    r = hq_add(r, hq_scale(d4, f4 / 24.0));
+    return r;
+}
+static hq hq_sqrt(hq a)
+{
+    double x = a.v[0], s = sqrt(x);
+    return hq_apply(a, s, 0.5 / s, -0.25 / (s * x), 0.375 / (s * x * x),
+                    -0.9375 / (s * x * x * x));
+}
+static hq hq_inv(hq a)
+{
+    double x = a.v[0], i1 = 1.0 / x, i2 = i1 * i1;
+    return hq_apply(a, i1, -i2, 2.0 * i2 * i1, -6.0 * i2 * i2, 24.0 * i2 * i2 * i1);
+}
+static hq hq_exp(hq a)
+{
+    double e = exp(a.v[0]);
+    return hq_apply(a, e, e, e, e, e);
+}
+static hq hq_div(hq a, hq b) { return hq_mul(a, hq_inv(b)); }
+
+/* parameters with parameter `which` (index in the swp_t vector layout) seeded on e4 */
+typedef struct { hq eps, sigma, a, lambda, gamma, A, B; double p, q; } swq_t;
+static swq_t swq_seed(const swp_t* P, int which)
+{
+    swq_t q;
+    q.eps = hq_c(P->eps); q.sigma = hq_c(P->sigma); q.a = hq_c(P->a);
+    q.lambda = hq_c(P->lambda); q.gamma = hq_c(P->gamma); q.A = hq_c(P->A); q.B = hq_c(P->B);
+    q.p = P->p; q.q = P->q;
+    hq* slot[8] = {&q.eps, &q.sigma, &q.a, &q.lambda, &q.gamma, NULL, &q.A, &q.B};
+    if (which >= 0 && which < 8 && slot[which]) slot[which]->v[8] = 1.0;
+    return q;
+}
+/* sw_h and sw_pair above, on hq with hq parameters */
+static hq sw_h_q(const hq x[9], const swq_t* P, double c0)
+{
+    hq rij[3], rik[3];
+    for (int c = 0; c < 3; ++c) { rij[c] = hq_sub(x[c], x[3 + c]); rik[c] = hq_sub(x[c], x[6 + c]); }
+    hq sij = hq_add(hq_add(hq_mul(rij[0], rij[0]), hq_mul(rij[1], rij[1])), hq_mul(rij[2], rij[2]));
+    hq sik = hq_add(hq_add(hq_mul(rik[0], rik[0]), hq_mul(rik[1], rik[1])), hq_mul(rik[2], rik[2]));
+    hq dij = hq_sqrt(sij), dik = hq_sqrt(sik);
+    hq dot = hq_add(hq_add(hq_mul(rij[0], rik[0]), hq_mul(rij[1], rik[1])), hq_mul(rij[2], rik[2]));
+    hq cs = hq_div(dot, hq_mul(dij, dik));
+    hq dc = hq_sub(cs, hq_c(c0));
+    hq le = hq_mul(P->eps, P->lambda), gs = hq_mul(P->sigma, P->gamma), rc = hq_mul(P->a, P->sigma);
+    hq e1 = hq_exp(hq_mul(hq_inv(hq_sub(dij, rc)), gs));
+    hq e2 = hq_exp(hq_mul(hq_inv(hq_sub(dik, rc)), gs));
+    return hq_mul(hq_mul(hq_mul(hq_mul(dc, dc), le), e1), e2);
+}
+static hq sw_pair_q(const hq r[3], const swq_t* P)
+{
+    hq s = hq_add(hq_add(hq_mul(r[0], r[0]), hq_mul(r[1], r[1])), hq_mul(r[2], r[2]));
+    hq d = hq_sqrt(s);
+    hq sr = hq_mul(hq_inv(d), P->sigma);
+    hq srp = hq_c(1.0);
+    for (int t = 0; t < (int)P->p; ++t) srp = hq_mul(srp, sr);
+    hq srq = hq_c(1.0);
+    for (int t = 0; t < (int)P->q; ++t) srq = hq_mul(srq, sr);
+    hq poly = hq_sub(hq_mul(srp, P->B), srq);
+    hq ex = hq_exp(hq_mul(hq_inv(hq_sub(d, hq_mul(P->a, P->sigma))), P->sigma));
+    return hq_mul(hq_mul(poly, ex), hq_mul(P->A, P->eps));
+}
+static void sw_block3_dp(const double* r9, const swp_t* P, double c0, int g0, int g1, int g2,
+                         double* out27, int which)
+{
+    swq_t Q = swq_seed(P, which);
+    hq x[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) {
+                for (int t = 0; t < 9; ++t) x[t] = hq_c(r9[t]);
+                x[3 * g0 + a].v[1] += 1.0;
+                x[3 * g1 + b].v[2] += 1.0;
+                x[3 * g2 + c].v[4] += 1.0;
+                out27[a + 3 * b + 9 * c] = sw_h_q(x, &Q, c0).v[15];
+            }
+}
+static void sw_pair_block3_dp(const double r[3], const swp_t* P, double* h3, int which)
+{
+    swq_t Q = swq_seed(P, which);
+    hq x[3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) {
+                for (int t = 0; t < 3; ++t) x[t] = hq_c(r[t]);
+                x[a].v[1] += 1.0; x[b].v[2] += 1.0; x[c].v[4] += 1.0;
+                h3[a + 3 * b + 9 * c] = sw_pair_q(x, &Q).v[15];
+            }
+}
+
+/* d Psi / d p of fco_sw_ifc3 before the tolerance pass, p = swv[which] (0 epsilon,
+ * 1 sigma, 2 a, 3 lambda, 4 gamma, 6 A, 7 B).  Returns 0, 2 on the cutoff check, 1 on
+ * a bad `which`.  Not reentrant (the block provider is switched for the call).    */
+int fco_sw_ifc3_dparam(const double* pos, int N, const double* L, const double* swv,
+                       double rcut, double c0_fd, double c0_iik, int which, double* out)
+{
+    if (which < 0 || which > 7 || which == 5) return 1;
+    g_sw_dparam = which;
+    const int rc = fco_sw_ifc3(pos, N, L, swv, rcut, 0.0, c0_fd, c0_iik, out);
+    g_sw_dparam = -1;
+    return rc;
 }

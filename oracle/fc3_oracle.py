@@ -75,6 +75,11 @@ def lib():
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_long, _i32p, _i32p, _i32p, _dp]
         L.fco_lj_ifc3_coo.restype = ctypes.c_long
+        L.fco_lj_ifc3_dparam.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, _dp]
+        L.fco_sw_ifc3_dparam.argtypes = [_dp, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
         L.fco_ipow.argtypes = [ctypes.c_double, ctypes.c_int]
         L.fco_ipow.restype = ctypes.c_double
         L.fco_num_threads.argtypes = []
@@ -240,6 +245,46 @@ def sw_third_order(sys, pot, calc, c0_fd=SW_C0_FD, c0_iik=SW_C0_IIK) -> np.ndarr
                            calc.r_cut, calc.tol, c0_fd, c0_iik, _p(out))
     if rc:
         raise AssertionError("For AutoDiff SW silicon force constant cutoff must be less than potential cutoff")
+    return out
+
+
+LJ_PARAMS = {"sigma": 0, "epsilon": 1}
+SW_PARAMS = {"epsilon": 0, "sigma": 1, "a": 2, "lambda": 3, "gamma": 4, "A": 6, "B": 7}
+
+
+def lj_third_order_dparam(sys, pot, calc, name: str, masses=None) -> np.ndarray:
+    """d Psi / d p (p = "sigma" or "epsilon") of third_order(sys, LJ, calc) by forward-mode
+    dual numbers through the same loops (fco_lj_ifc3_dparam), entries that the tolerance
+    pass zeroes set to zero; mass weighted as mass_weight_third_order! when masses given."""
+    pos, box = _sysargs(sys)
+    n = 3 * sys.n_atoms
+    out = np.zeros((n, n, n), dtype=np.float64, order="F")
+    rc = lib().fco_lj_ifc3_dparam(_p(pos), sys.n_atoms, _p(box), pot.sigma, pot.epsilon,
+                                  pot.r_cut, calc.r_cut, LJ_PARAMS[name], _p(out))
+    if rc:
+        raise AssertionError("Cutoff larger than L/2")
+    out[lj_third_order(sys, pot, calc) == 0.0] = 0.0
+    if masses is not None:
+        mass_weight_third_order(out, masses)
+    return out
+
+
+def sw_third_order_dparam(sys, pot, calc, name: str, masses=None, c0_fd=SW_C0_FD,
+                          c0_iik=SW_C0_IIK) -> np.ndarray:
+    """d Psi / d p of third_order(sys, StillingerWeberSilicon, AutoDiffCalculator) for a
+    parameter name of SW_PARAMS, by 4-direction hyper-dual numbers (the coordinates'
+    three directions and the parameter's) through the same loops (fco_sw_ifc3_dparam);
+    tolerance mask and mass weighting as lj_third_order_dparam."""
+    pos, box = _sysargs(sys)
+    n = 3 * sys.n_atoms
+    out = np.zeros((n, n, n), dtype=np.float64, order="F")
+    rc = lib().fco_sw_ifc3_dparam(_p(pos), sys.n_atoms, _p(box), _p(pot.params.as_array()),
+                                  calc.r_cut, c0_fd, c0_iik, SW_PARAMS[name], _p(out))
+    if rc:
+        raise AssertionError("For AutoDiff SW silicon force constant cutoff must be less than potential cutoff")
+    out[sw_third_order(sys, pot, calc, c0_fd, c0_iik) == 0.0] = 0.0
+    if masses is not None:
+        mass_weight_third_order(out, masses)
     return out
 
 

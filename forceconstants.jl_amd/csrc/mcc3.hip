@@ -351,6 +351,9 @@ struct G3 {
     // order); a tile's result does not depend on which other tiles a launch holds
     const int* tlist;
     int tlist_n;
+    // sparse K: a row tile without K blocks writes nothing (its output is all zero, and
+    // the caller guarantees that no consumer reads it)
+    int skip_empty;
 };
 
 // row tiles walked together, column by column, in the grouped tile order
@@ -788,6 +791,7 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
         }
         __syncthreads();
         const int nkl = kcount;
+        if (nkl == 0 && g.skip_empty) return;   // uniform: every thread read kcount
         if constexpr (NARROW != 0) {
             if (tn == g.tiles_n - 1) {
                 gemm3_tile<SPLIT_OUT, NARROW, true>(g, m0, n0, s0, s1, s2, s3, klist, nkl);
@@ -890,6 +894,62 @@ __global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restri
         if (!any) return;
     }
     if (!inb) return;
+    const long off = (long)kb * 16 * rows + 16 * m;
+    bf16x8* dh = reinterpret_cast<bf16x8*>(X + off);
+    bf16x8* dl = reinterpret_cast<bf16x8*>(X + rows * Kp + off);
+    dh[0] = h0;
+    dh[1] = h1;
+    dl[0] = l0;
+    dl[1] = l1;
+}
+
+// k_split_transpose_u over two neighbouring i blocks per workgroup (512 threads: 32 i x
+// 16 j), so that each load of a k covers whole 128-byte lines of Psi (32 consecutive i)
+// instead of half lines that another workgroup fetches again.  Same X, kmask and pat as
+// k_split_transpose_u.  Block (K block blockIdx.x, (i-block pair, 16-j chunk) blockIdx.y);
+// u % 16 == 0.
+__global__ void __launch_bounds__(512) k_split_transpose_u2(const float* __restrict__ psi, long n2,
+                                                             int n, int Kp, long r0, int u,
+                                                             __bf16* __restrict__ X,
+                                                             unsigned char* __restrict__ kmask,
+                                                             int nkb, unsigned char* __restrict__ pat)
+{
+    __shared__ int any2[2];
+    const int kb = blockIdx.x;
+    const long rows = (long)u * Kp;
+    const int chunks = u / 16;
+    const int ib2 = blockIdx.y / chunks, jc = blockIdx.y - ib2 * chunks;
+    const int il = threadIdx.x & 31, jl = threadIdx.x >> 5, half = il >> 4;
+    const int i = 32 * ib2 + il, jj = 16 * jc + jl;
+    const long ib = 2L * ib2 + half;
+    const long m = ib * 16L * u + 16L * jj + (il & 15);
+    const bool inb = m < rows, valid = inb && i < n;
+    if (threadIdx.x < 2) any2[threadIdx.x] = 0;
+    __syncthreads();
+    const int k0 = 16 * kb;
+    bf16x8 h0, h1, l0, l1;
+    bool nz = false;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int k = k0 + t;
+        const float x = (valid && k < n) ? psi[r0 + i + (long)n * jj + n2 * k] : 0.0f;
+        nz |= x != 0.0f;
+        __bf16 hh, ll;
+        split_bf16(x, hh, ll);
+        if (t < 8) { h0[t] = hh; l0[t] = ll; } else { h1[t - 8] = hh; l1[t - 8] = ll; }
+    }
+    if (pat && nz) pat[(long)i + (long)n * (r0 / n + jj)] = 1;   // same value from every writer
+    bool keep = true;
+    if (kmask) {   // every thread of the block reaches these barriers
+        if (nz) atomicOr(&any2[half], 1);
+        __syncthreads();
+        const long tile = ib * (u / 16) + jc;   // the 256-row tile of rows (ib, jc)
+        if (threadIdx.x == 0 || threadIdx.x == 16) {
+            if (ib * 16 < Kp) kmask[tile * nkb + kb] = any2[half] ? 1 : 0;
+        }
+        keep = any2[half] != 0;
+    }
+    if (!keep || !inb) return;
     const long off = (long)kb * 16 * rows + 16 * m;
     bf16x8* dh = reinterpret_cast<bf16x8*>(X + off);
     bf16x8* dl = reinterpret_cast<bf16x8*>(X + rows * Kp + off);
@@ -1198,7 +1258,7 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
           const Split3& B, const OutMap& o, bool split_out, long M, int N, int K, int kc = 0,
           long c_ZS = 0, int batch = 0, long a_ZS = 0, long b_ZS = 0,
           const unsigned char* kmask = nullptr, int kmask_div = 1, long a_R2 = 0, long a_SS2 = 0,
-          const int* tlist = nullptr, int tlist_n = 0)
+          const int* tlist = nullptr, int tlist_n = 0, bool skip_empty = false)
 {
     if (M >= (1L << 31) || a_R >= (1L << 31) || o.R >= (1L << 31))
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
@@ -1230,6 +1290,7 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
     g.epi = o.KS == 16 ? 1 : o.CS == 16 ? 2 : 0;
     g.tlist = tlist;
     g.tlist_n = tlist ? tlist_n : 0;
+    g.skip_empty = (kmask && skip_empty) ? 1 : 0;
     if (tlist && (batch > 0 || g.nz != 1 || (N + 255) / 256 > 256))
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: tile list shape");
     if (kmask) {   // sparse K: unsplit, unbatched products with split output
@@ -1271,7 +1332,11 @@ long mode3_rows(int n, size_t cap_bytes, long tiles_per_launch)
     const long cap = (long)(cap_bytes / (4 * (size_t)padk(n))) / n;
     u = std::max(1L, std::min({u, cap, (long)n}));
     const long blocks = (n + u - 1) / u;
-    return (n + blocks - 1) / blocks * (long)n;
+    u = (n + blocks - 1) / blocks;
+    // whole groups of 16 columns when n % 16 == 0: every 256-row tile of the mode-3 GEMM
+    // is then one 16 i x 16 j block of U, the granularity of the mode-1 block mask
+    if (n % 16 == 0 && u >= 16) u = u / 16 * 16;
+    return u * (long)n;
 }
 
 // Psi rows [r0, r0 + rows) -> X (split, K-blocked, K padded) -> C(m, l) for those rows.
@@ -1303,23 +1368,35 @@ int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, cons
 // mask_bytes(u Kp, n) bytes or null (then every block is multiplied).
 long x_rows(long R, int n) { return R / n * padk(n); }
 
+// skip_empty: tiles of all-zero Psi rows leave U unwritten (for a consumer that reads only
+// the mode-1 blocks marked by Psi's pattern; needs u % 16 == 0 and j0 % 16 == 0, so that a
+// tile is one 16 i x 16 j block).
 int mode3_block_u(hipStream_t s, const float* psi, int n, long j0, int u, const Split3& B,
                   __bf16* X, __bf16* U, long P, int ncols, unsigned char* kmask,
-                  unsigned char* pat = nullptr)
+                  unsigned char* pat = nullptr, bool skip_empty = false)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
     if (kmask && Kp / 16 > v3::kMaxKBlocks) kmask = nullptr;
     const long rows = (long)u * Kp;
-    dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
-    hipLaunchKernelGGL(v3::k_split_transpose_u, gt, dim3(256), 0, s, psi, n2, n, Kp, n * j0, u, X,
-                       kmask, Kp / 16, pat);
+    if (u % 16 == 0) {   // whole 128-byte lines of Psi per load
+        const long pairs = (Kp / 16 + 1) / 2;
+        dim3 gt((unsigned)(Kp / 16), (unsigned)(pairs * (u / 16)));
+        hipLaunchKernelGGL(v3::k_split_transpose_u2, gt, dim3(512), 0, s, psi, n2, n, Kp, n * j0,
+                           u, X, kmask, Kp / 16, pat);
+    } else {
+        dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
+        hipLaunchKernelGGL(v3::k_split_transpose_u, gt, dim3(256), 0, s, psi, n2, n, Kp, n * j0,
+                           u, X, kmask, Kp / 16, pat);
+    }
     FC3_HIP(hipGetLastError());
     // C(m, c), m = (i / 16) 16 u + 16 (j - j0) + i % 16 -> U_c(i, j) = U[c 2P + (i / 16) 16 n +
     // 16 j + i % 16]
     const OutMap oU{U + 16 * j0, 16L * u, 16L * n, 16, 2 * P, P};
+    if (skip_empty && (u % 16 != 0 || j0 % 16 != 0))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: unaligned mode-3 block");
     return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, oU, true, rows,
-                 ncols, Kp, 0, 0, 0, 0, 0, kmask);
+                 ncols, Kp, 0, 0, 0, 0, 0, kmask, 1, 0, 0, nullptr, 0, skip_empty);
 }
 
 // Modes 3 and 1 skip the all-zero K blocks of their operands unless FC3_MCC3_DENSE is set
@@ -1501,10 +1578,13 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
         FC3_HIP(hipGetLastError());
     }
     // mode 3 by blocks of whole columns j (R is a multiple of n); it also writes U's
-    // K padding (zeros)
+    // K padding (zeros).  When every mode-1 launch takes the block mask, U's all-zero
+    // 16 i x 16 j blocks are never read, and mode 3 does not write them.
+    const bool skip_empty = mask1 && (R / n) % 16 == 0 && nb % 16 == 0;
     for (long r0 = 0; r0 < n2; r0 += R) {
         const long rows = (n2 - r0) < R ? (n2 - r0) : R;
-        FC3_TRY(mode3_block_u(s, psi, n, r0 / n, (int)(rows / n), B, X, U, P, n, mask, pat));
+        FC3_TRY(mode3_block_u(s, psi, n, r0 / n, (int)(rows / n), B, X, U, P, n, mask, pat,
+                              skip_empty));
     }
     if (mask1) {
         hipLaunchKernelGGL(v3::k_pattern_blocks, dim3((unsigned)(Kp / 16), (unsigned)(n / 16)),

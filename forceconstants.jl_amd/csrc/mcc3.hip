@@ -778,7 +778,9 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
         __shared__ int kcount;
         if (threadIdx.x < 64) {
             const int lane = threadIdx.x;
-            const unsigned char* mk = g.kmask + (long)(tm / g.kmask_div) * g.kmask_ld;
+            // unsplit products: the row tile's mask row; a batch (zk = 0): product z's row
+            const long mrow = g.zk ? (long)(tm / g.kmask_div) : (long)blockIdx.y;
+            const unsigned char* mk = g.kmask + mrow * g.kmask_ld;
             int c = 0;
             for (int b0 = 0; b0 < g.kmask_ld; b0 += 64) {
                 const int kb = b0 + lane;
@@ -861,13 +863,11 @@ __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict
 // k_split_transpose, with rows = u Kp.  A 256-row GEMM tile is then 16 i x 16 j (for
 // u >= 16): its output column is one contiguous run of U.  Block (K block blockIdx.x, row
 // tile blockIdx.y); kmask and the skipped all-zero blocks as k_split_transpose.
-// With pat, rows (i, j) that hold a nonzero set pat[i + n j] = 1 (Psi's (i, j) pattern,
-// which the mode-1 operand U_l(i, j) = sum_k Psi[i, j, k] phi[k, l] shares for every l).
 __global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restrict__ psi, long n2,
                                                             int n, int Kp, long r0, int u,
                                                             __bf16* __restrict__ X,
                                                             unsigned char* __restrict__ kmask,
-                                                            int nkb, unsigned char* __restrict__ pat)
+                                                            int nkb)
 {
     const int kb = blockIdx.x;
     const long rows = (long)u * Kp;
@@ -887,7 +887,6 @@ __global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restri
         split_bf16(x, hh, ll);
         if (t < 8) { h0[t] = hh; l0[t] = ll; } else { h1[t - 8] = hh; l1[t - 8] = ll; }
     }
-    if (pat && nz) pat[(long)i + (long)n * (r0 / n + jj)] = 1;   // same value from every writer
     if (kmask) {   // every thread of the block reaches this barrier
         const int any = __syncthreads_or(nz ? 1 : 0);
         if (threadIdx.x == 0) kmask[(long)blockIdx.y * nkb + kb] = any ? 1 : 0;
@@ -905,14 +904,14 @@ __global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restri
 
 // k_split_transpose_u over two neighbouring i blocks per workgroup (512 threads: 32 i x
 // 16 j), so that each load of a k covers whole 128-byte lines of Psi (32 consecutive i)
-// instead of half lines that another workgroup fetches again.  Same X, kmask and pat as
+// instead of half lines that another workgroup fetches again.  Same X and kmask as
 // k_split_transpose_u.  Block (K block blockIdx.x, (i-block pair, 16-j chunk) blockIdx.y);
 // u % 16 == 0.
 __global__ void __launch_bounds__(512) k_split_transpose_u2(const float* __restrict__ psi, long n2,
                                                              int n, int Kp, long r0, int u,
                                                              __bf16* __restrict__ X,
                                                              unsigned char* __restrict__ kmask,
-                                                             int nkb, unsigned char* __restrict__ pat)
+                                                             int nkb)
 {
     __shared__ int any2[2];
     const int kb = blockIdx.x;
@@ -938,7 +937,6 @@ __global__ void __launch_bounds__(512) k_split_transpose_u2(const float* __restr
         split_bf16(x, hh, ll);
         if (t < 8) { h0[t] = hh; l0[t] = ll; } else { h1[t - 8] = hh; l1[t - 8] = ll; }
     }
-    if (pat && nz) pat[(long)i + (long)n * (r0 / n + jj)] = 1;   // same value from every writer
     bool keep = true;
     if (kmask) {   // every thread of the block reaches these barriers
         if (nz) atomicOr(&any2[half], 1);
@@ -966,16 +964,145 @@ __global__ void k_fill_u8(unsigned char* __restrict__ p, long cnt, unsigned char
         p[e] = v;
 }
 
-// Mode-1 block mask from Psi's (i, j) pattern: T[jb nkb + kb] = 1 when some pat[i + n j]
-// with i in K block kb (i < n) and j in [16 jb, 16 jb + 16) is set.  Block (kb, jb).
-__global__ void __launch_bounds__(256) k_pattern_blocks(const unsigned char* __restrict__ pat, int n,
-                                                         int nkb, unsigned char* __restrict__ T)
+// ---- gathered mode 1 (n % 16 == 0) ---------------------------------------------------
+// Psi's (i, j) pattern, Psi viewed as an n^2 x n matrix (rows r = i + n j contiguous):
+// pat[r] = 1 when some Psi[r + n^2 k] != 0.  Thread per 4 consecutive rows, k looped: every
+// load of a wave is 1 KB contiguous, and Psi is read once.  n^2 % 4 == 0.
+__global__ void __launch_bounds__(256) k_psi_pattern(const float* __restrict__ psi, long n2, int n,
+                                                     unsigned char* __restrict__ pat)
 {
-    const int kb = blockIdx.x, jb = blockIdx.y;
-    const int i = 16 * kb + (threadIdx.x & 15), j = 16 * jb + (threadIdx.x >> 4);
-    const bool f = i < n && j < n && pat[(long)i + (long)n * j] != 0;
-    const int any = __syncthreads_or(f ? 1 : 0);
-    if (threadIdx.x == 0) T[(long)jb * nkb + kb] = any ? 1 : 0;
+    const long r = 4 * ((long)blockIdx.x * 256 + threadIdx.x);
+    if (r >= n2) return;
+    const float4* p = reinterpret_cast<const float4*>(psi + r);
+    const long step = n2 / 4;
+    bool a = false, b = false, c = false, d = false;
+    for (int k = 0; k < n; ++k) {
+        const float4 v = p[step * k];
+        a |= v.x != 0.0f;
+        b |= v.y != 0.0f;
+        c |= v.z != 0.0f;
+        d |= v.w != 0.0f;
+    }
+    uchar4 o;
+    o.x = a; o.y = b; o.z = c; o.w = d;
+    *reinterpret_cast<uchar4*>(pat + r) = o;
+}
+
+// For each group jb of 16 columns j: the rows i (ascending) with pat[i + n j] set for some
+// j of the group, list[jb n + t], their number cnt[jb], and the mode-1 mask of the group,
+// T[jb nkb + kb] = 1 for the K blocks kb < ceil(cnt / 16) of the gathered rows.  Block per
+// group; the 4 waves scan i in chunks of 256 and compact with ballots.
+__global__ void __launch_bounds__(256) k_group_lists(const unsigned char* __restrict__ pat, int n,
+                                                     int* __restrict__ list, int* __restrict__ cnt,
+                                                     int nkb, unsigned char* __restrict__ T)
+{
+    __shared__ int wcount[4];
+    __shared__ int base;
+    const int jb = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (int i0 = 0; i0 < n; i0 += 256) {
+        const int i = i0 + threadIdx.x;
+        bool f = false;
+        if (i < n) {
+            const unsigned char* q = pat + i + (long)n * 16 * jb;
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) f |= q[(long)n * jj] != 0;
+        }
+        const unsigned long long m = __ballot(f);
+        if (lane == 0) wcount[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wcount[w];
+        if (f) list[(long)jb * n + off + __popcll(m & ((1ull << lane) - 1ull))] = i;
+        __syncthreads();
+        if (threadIdx.x == 0) base += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        __syncthreads();
+    }
+    const int c = base;
+    if (threadIdx.x == 0) cnt[jb] = c;
+    for (int kb = threadIdx.x; kb < nkb; kb += 256) T[(long)jb * nkb + kb] = kb * 16 < c ? 1 : 0;
+}
+
+// The mode-1 B operand of each group: B_jb(t, c) = phi[list[jb n + t], c] for t < cnt[jb],
+// zero for the rest of the last K block and for columns c >= n; split, K-blocked, hi at
+// Bg[jb 2 n Npad + (t / 16) 16 Npad + 16 c + t % 16], lo at + n Npad.  Thread (column c,
+// 16-row block tb, group jb); blocks past the gathered rows are not written (never read).
+__global__ void __launch_bounds__(256) k_gather_phi_list(const float* __restrict__ phi, int n,
+                                                         const int* __restrict__ list,
+                                                         const int* __restrict__ cnt, int Npad,
+                                                         __bf16* __restrict__ Bg)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tb = blockIdx.y, jb = blockIdx.z;
+    const int L = cnt[jb];
+    if (c >= Npad || 16 * tb >= L) return;
+    bf16x8 h0, h1, l0, l1;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int t = 16 * tb + u;
+        const float x = (t < L && c < n) ? phi[list[(long)jb * n + t] + (long)n * c] : 0.0f;
+        __bf16 hh, ll;
+        split_bf16(x, hh, ll);
+        if (u < 8) { h0[u] = hh; l0[u] = ll; } else { h1[u - 8] = hh; l1[u - 8] = ll; }
+    }
+    const long off = (long)jb * 2 * n * Npad + (long)tb * 16 * Npad + 16L * c;
+    bf16x8* dh = reinterpret_cast<bf16x8*>(Bg + off);
+    bf16x8* dl = reinterpret_cast<bf16x8*>(Bg + off + (long)n * Npad);
+    dh[0] = h0;
+    dh[1] = h1;
+    dl[0] = l0;
+    dl[1] = l1;
+}
+
+// Mode-3 operand of the gathered layout for the groups [jb0, jb0 + G): X rows
+// m = (jb - jb0) 16 n + 256 tb + 16 jj + t % 16 hold Psi row (i, j) with i = list[jb n + t]
+// (t = 16 tb + t % 16; zero when t >= cnt[jb]) and j = 16 jb + jj, split and K-blocked over
+// k like k_split_transpose_u (rows = 16 G n).  A 256-row tile is one (jb, tb): a tile past
+// the group's rows only records an empty mask (its GEMM tile then writes nothing, and mode 1
+// never reads those rows).  kmask[tile nkb + kb] = any nonzero in the tile's K block kb.
+// Block (K block blockIdx.x, tile blockIdx.y); thread (jj, t % 16).
+__global__ void __launch_bounds__(256) k_split_gather(const float* __restrict__ psi, long n2, int n,
+                                                      int jb0, const int* __restrict__ list,
+                                                      const int* __restrict__ cnt,
+                                                      __bf16* __restrict__ X, long rows,
+                                                      unsigned char* __restrict__ kmask, int nkb)
+{
+    const int kb = blockIdx.x;
+    const long tile = blockIdx.y;
+    const int tpg = n / 16;                        // tiles per group
+    const int jb = jb0 + (int)(tile / tpg), tb = (int)(tile % tpg);
+    const int L = cnt[jb];
+    if (16 * tb >= L) {                            // uniform over the block
+        if (threadIdx.x == 0) kmask[tile * nkb + kb] = 0;
+        return;
+    }
+    const int tl = threadIdx.x & 15, jj = threadIdx.x >> 4;
+    const int t = 16 * tb + tl, j = 16 * jb + jj;
+    const int i = t < L ? list[(long)jb * n + t] : -1;
+    const int k0 = 16 * kb;
+    bf16x8 h0, h1, l0, l1;
+    bool nz = false;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int k = k0 + u;
+        const float x = (i >= 0 && k < n) ? psi[i + (long)n * j + n2 * k] : 0.0f;
+        nz |= x != 0.0f;
+        __bf16 hh, ll;
+        split_bf16(x, hh, ll);
+        if (u < 8) { h0[u] = hh; l0[u] = ll; } else { h1[u - 8] = hh; l1[u - 8] = ll; }
+    }
+    const int any = __syncthreads_or(nz ? 1 : 0);
+    if (threadIdx.x == 0) kmask[tile * nkb + kb] = any ? 1 : 0;
+    if (!any) return;
+    const long m = tile * 256 + threadIdx.x;       // 16 jj + t % 16
+    const long off = (long)kb * 16 * rows + 16 * m;
+    bf16x8* dh = reinterpret_cast<bf16x8*>(X + off);
+    bf16x8* dl = reinterpret_cast<bf16x8*>(X + rows * 16 * nkb + off);
+    dh[0] = h0;
+    dh[1] = h1;
+    dl[0] = l0;
+    dl[1] = l1;
 }
 
 // cnt slabs T_l (n x n, T_l[i + n j] at src[l n^2 + i + n j]) -> the K-blocked A operand
@@ -1291,10 +1418,10 @@ int gemm3(hipStream_t s, int kind, const __bf16* A, long a_lo, long a_R, long a_
     g.tlist = tlist;
     g.tlist_n = tlist ? tlist_n : 0;
     g.skip_empty = (kmask && skip_empty) ? 1 : 0;
-    if (tlist && (batch > 0 || g.nz != 1 || (N + 255) / 256 > 256))
+    if (tlist && ((batch == 0 && g.nz != 1) || (N + 255) / 256 > 256))
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: tile list shape");
-    if (kmask) {   // sparse K: unsplit, unbatched products with split output
-        if (!split_out || batch > 0 || g.nz != 1 || g.kmask_ld > v3::kMaxKBlocks)
+    if (kmask) {   // sparse K: split output; unsplit, or a batch with one mask row per product
+        if (!split_out || (batch == 0 && g.nz != 1) || g.kmask_ld > v3::kMaxKBlocks)
             return set_error(FC3_ERR_UNSUPPORTED, "mcc3: sparse-K GEMM shape");
         return launch3_cols<true, true>(s, kind, g);
     }
@@ -1368,12 +1495,8 @@ int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, cons
 // mask_bytes(u Kp, n) bytes or null (then every block is multiplied).
 long x_rows(long R, int n) { return R / n * padk(n); }
 
-// skip_empty: tiles of all-zero Psi rows leave U unwritten (for a consumer that reads only
-// the mode-1 blocks marked by Psi's pattern; needs u % 16 == 0 and j0 % 16 == 0, so that a
-// tile is one 16 i x 16 j block).
 int mode3_block_u(hipStream_t s, const float* psi, int n, long j0, int u, const Split3& B,
-                  __bf16* X, __bf16* U, long P, int ncols, unsigned char* kmask,
-                  unsigned char* pat = nullptr, bool skip_empty = false)
+                  __bf16* X, __bf16* U, long P, int ncols, unsigned char* kmask)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
@@ -1383,20 +1506,18 @@ int mode3_block_u(hipStream_t s, const float* psi, int n, long j0, int u, const 
         const long pairs = (Kp / 16 + 1) / 2;
         dim3 gt((unsigned)(Kp / 16), (unsigned)(pairs * (u / 16)));
         hipLaunchKernelGGL(v3::k_split_transpose_u2, gt, dim3(512), 0, s, psi, n2, n, Kp, n * j0,
-                           u, X, kmask, Kp / 16, pat);
+                           u, X, kmask, Kp / 16);
     } else {
         dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
         hipLaunchKernelGGL(v3::k_split_transpose_u, gt, dim3(256), 0, s, psi, n2, n, Kp, n * j0,
-                           u, X, kmask, Kp / 16, pat);
+                           u, X, kmask, Kp / 16);
     }
     FC3_HIP(hipGetLastError());
     // C(m, c), m = (i / 16) 16 u + 16 (j - j0) + i % 16 -> U_c(i, j) = U[c 2P + (i / 16) 16 n +
     // 16 j + i % 16]
     const OutMap oU{U + 16 * j0, 16L * u, 16L * n, 16, 2 * P, P};
-    if (skip_empty && (u % 16 != 0 || j0 % 16 != 0))
-        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: unaligned mode-3 block");
     return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, oU, true, rows,
-                 ncols, Kp, 0, 0, 0, 0, 0, kmask, 1, 0, 0, nullptr, 0, skip_empty);
+                 ncols, Kp, 0, 0, 0, 0, 0, kmask);
 }
 
 // Modes 3 and 1 skip the all-zero K blocks of their operands unless FC3_MCC3_DENSE is set
@@ -1410,41 +1531,34 @@ bool mode3_skips_zero_blocks()
 
 // Modes 1 and 2 on slabs [l0, l0 + cnt): U holds them as the mode-1 A operand (slab
 // stride 2P bf16 with P = n Kp, lo at +P, K-blocked); S is scratch of the same layout; T
-// receives the FP32 slabs K3_l[b + n c].
-//
-// mask1 (n % 16 == 0 and cnt % 16 == 0; else ignored): the mode-1 block mask of
-// k_pattern_blocks, T[jb kbs + kb] for 16-row groups jb of j and K blocks kb of i.  Mode 1
-// then takes its rows in the order m = jb (16 cnt) + 16 l + j % 16 (16 j x 16 slabs per
-// 256-row tile), so that a tile's K blocks are those of 16 j only, and multiplies only the
-// K blocks T marks: U_l(i, j) is exactly zero wherever Psi[i, j, :] is (for every l), so
-// S is the same bit for bit.
-//
-// tl1 / tl2 (with counts): explicit tile lists of the mode-1 / mode-2 GEMMs (null: all
-// tiles).
+// receives the FP32 slabs K3_l[b + n c].  tl1 / tl2 (with counts): explicit tile lists of
+// the mode-1 / mode-2 GEMMs (null: all tiles).
 int modes12_v3(hipStream_t s, const __bf16* U, int n, int cnt, const Split3& B, __bf16* S,
-               float* T, const unsigned char* mask1 = nullptr, const int* tl1 = nullptr,
-               int ntl1 = 0, const int* tl2 = nullptr, int ntl2 = 0)
+               float* T, const int* tl1 = nullptr, int ntl1 = 0, const int* tl2 = nullptr,
+               int ntl2 = 0)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
     const long P = (long)n * Kp;
-    if (mask1 && n % 16 == 0 && cnt % 16 == 0) {
-        // A(m, i) = U_l(i, j) = U[l 2P + (i / 16) 16 n + 16 j + i % 16], m as above;
-        // C(m, b) = S_l(j, b) at S[l 2P + (j / 16) 16 n + 16 b + j % 16] (mode-2 A operand)
-        const OutMap oS1{S, 16L * cnt, 16L * n, 2 * P, 16, P};
-        FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, 16, 2 * P, 16L * n, B, oS1, true,
-                      (long)cnt * n, n, Kp, 0, 0, 0, 0, 0, mask1, cnt / 16, 16L * cnt, 256,
-                      tl1, ntl1));
-    } else {
-        // A(l n + j, i) = U_l(i, j);  C(l n + j, b) = S_l(j, b) -> mode-2 A(l n + b, j)
-        const OutMap oS{S, n, 2 * P, 16L * n, 16, P};
-        FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, n, 2 * P, 16L * n, B, oS, true,
-                      (long)cnt * n, n, Kp, 0, 0, 0, 0, 0, nullptr, 1, 0, 0, tl1, ntl1));
-    }
+    // A(l n + j, i) = U_l(i, j);  C(l n + j, b) = S_l(j, b) -> mode-2 A(l n + b, j)
+    const OutMap oS{S, n, 2 * P, 16L * n, 16, P};
+    FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, P, n, 2 * P, 16L * n, B, oS, true, (long)cnt * n,
+                  n, Kp, 0, 0, 0, 0, 0, nullptr, 1, 0, 0, tl1, ntl1));
     // A(l n + b, j) = S_l(j, b);  C(l n + b, c) = K3_l[b + n c]
     const OutMap oT{T, n, n2, 16, n, 0};
     return gemm3(s, FC3_KERNEL_GEMM_MODE2, S, P, n, 2 * P, 16L * n, B, oT, false, (long)cnt * n,
                  n, Kp, 0, 0, 0, 0, 0, nullptr, 1, 0, 0, tl2, ntl2);
+}
+
+// Gathered mode 1 (n % 16 == 0, zero blocks skipped): Psi's (i, j) pattern, for each group
+// of 16 columns j the list of rows i it touches, the groups' mode-1 masks and gathered phi.
+bool gathered_applies(int n) { return n % 16 == 0 && mode3_skips_zero_blocks(); }
+size_t gathered_bytes(int n)
+{
+    const long G = n / 16, Npad = round_up(n, v3::BN);
+    return Carve::need((size_t)n * n) + Carve::need(sizeof(int) * (size_t)G * n) +
+           Carve::need(sizeof(int) * (size_t)G) + Carve::need((size_t)G * G) +
+           Carve::need(2 * sizeof(__bf16) * (size_t)G * n * Npad);
 }
 
 size_t mcc3_v3_workspace_bytes(int n, bool inplace, long tiles)
@@ -1456,8 +1570,7 @@ size_t mcc3_v3_workspace_bytes(int n, bool inplace, long tiles)
     const size_t u = (!inplace && padk(n) != n) ? Carve::need(2 * sizeof(__bf16) * P * n) : 0;
     return phi3_bytes(n) + slabs * (inplace ? 2 : 1) + u +
            Carve::need(2 * sizeof(__bf16) * (size_t)x_rows(R, n) * padk(n)) +
-           mask_bytes(x_rows(R, n), n) + Carve::need((size_t)n * n) +
-           Carve::need((size_t)(n / 16 + 1) * (padk(n) / 16)) + 4096;
+           mask_bytes(x_rows(R, n), n) + (n % 16 == 0 ? gathered_bytes(n) : 0) + 4096;
 }
 
 // ---- symmetric-unique K3 (the "calculate only the unique part" TODO of src/mcc.jl:116-117)
@@ -1496,19 +1609,16 @@ void push_tiles(std::vector<int>& v, long M, int N, F need)
 }
 
 // Mode-1 and mode-2 tile lists of the slab batch [l0, l0 + cnt) for the unique entries.
-// permuted: mode 1 takes its rows as m = jb (16 cnt) + 16 l + j % 16 (else m = l n + j);
-// mode 2 always as m = l n + b.
-void unique_tiles(int n, int l0, int cnt, bool permuted, std::vector<int>& t1,
+// gathered: mode 1 is a batch of products over the column groups, each with rows
+// m = 16 l + j % 16 (the same tiles for every group); else one product with rows
+// m = l n + j.  Mode 2 always has rows m = l n + b.
+void unique_tiles(int n, int l0, int cnt, bool gathered, std::vector<int>& t1,
                   std::vector<int>& t2)
 {
     const long M = (long)cnt * n;
-    push_tiles(t1, M, n, [&](long r0, long r1, int c0, int) {
-        long lmax;
-        if (permuted) {
-            lmax = (r0 % (16L * cnt)) / 16 + 15;
-        } else {
-            lmax = std::min(r1 / n, (long)cnt - 1);
-        }
+    push_tiles(t1, gathered ? 16L * cnt : M, n, [&](long r0, long r1, int c0, int) {
+        const long lmax = gathered ? std::min(r1 / 16, (long)cnt - 1)
+                                   : std::min(r1 / n, (long)cnt - 1);
         return c0 <= l0 + lmax;   // some column b <= some slab l of the tile
     });
     push_tiles(t2, M, n, [&](long r0, long r1, int c0, int c1) {
@@ -1545,10 +1655,15 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
     __bf16* U = Kp == n ? reinterpret_cast<__bf16*>(k3) : cv.take<__bf16>((size_t)2 * P * n);
     const bool skip = mode3_skips_zero_blocks();
     unsigned char* mask = skip ? cv.take<unsigned char>(mask_bytes(x_rows(R, n), n)) : nullptr;
-    // Psi's (i, j) pattern and the mode-1 block mask (n % 16 == 0 only)
-    const bool skip1 = skip && n % 16 == 0;
-    unsigned char* pat = skip1 ? cv.take<unsigned char>((size_t)n * n) : nullptr;
-    unsigned char* mask1 = skip1 ? cv.take<unsigned char>((size_t)(n / 16) * (Kp / 16)) : nullptr;
+    // gathered mode 1 (n % 16 == 0): pattern, row lists and masks of the column groups, and
+    // the groups' gathered phi
+    const bool gath = skip && n % 16 == 0;
+    const int NG = n / 16, Npad = round_up(n, v3::BN);
+    unsigned char* pat = gath ? cv.take<unsigned char>((size_t)n * n) : nullptr;
+    int* glist = gath ? cv.take<int>((size_t)NG * n) : nullptr;
+    int* gcnt = gath ? cv.take<int>(NG) : nullptr;
+    unsigned char* gmask = gath ? cv.take<unsigned char>((size_t)NG * NG) : nullptr;
+    __bf16* Bg = gath ? cv.take<__bf16>((size_t)2 * NG * n * Npad) : nullptr;
     // unique entries only: every batch's mode-1 and mode-2 tile lists, uploaded once
     const size_t list_cap = k3u ? unique_list_bytes(n, nb) / sizeof(int) : 0;
     int* dl = k3u ? cv.take<int>(list_cap) : nullptr;
@@ -1559,7 +1674,7 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
         for (int l0 = 0; l0 < n; l0 += nb) {
             const int cnt = (n - l0) < nb ? (n - l0) : nb;
             std::vector<int> t1, t2;
-            unique_tiles(n, l0, cnt, skip1 && cnt % 16 == 0, t1, t2);
+            unique_tiles(n, l0, cnt, gath, t1, t2);
             off.push_back((long)lists.size());
             lists.insert(lists.end(), t1.begin(), t1.end());
             off.push_back((long)lists.size());
@@ -1573,40 +1688,67 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
         FC3_HIP(hipStreamSynchronize(s));
     }
     FC3_TRY(zero_kpad(s, S, n, nb));
-    if (pat) {
-        hipLaunchKernelGGL(v3::k_fill_u8, dim3(1024), dim3(256), 0, s, pat, n2, (unsigned char)0);
+    if (gath) {
+        // Psi's (i, j) pattern (one read of Psi), the groups' row lists and gathered phi
+        hipLaunchKernelGGL(v3::k_psi_pattern, dim3((unsigned)((n2 / 4 + 255) / 256)), dim3(256), 0,
+                           s, psi, n2, n, pat);
+        hipLaunchKernelGGL(v3::k_group_lists, dim3((unsigned)NG), dim3(256), 0, s, pat, n, glist,
+                           gcnt, NG, gmask);
+        hipLaunchKernelGGL(v3::k_gather_phi_list,
+                           dim3((unsigned)(Npad / 256), (unsigned)NG, (unsigned)NG), dim3(256), 0,
+                           s, phi, n, glist, gcnt, Npad, Bg);
         FC3_HIP(hipGetLastError());
-    }
-    // mode 3 by blocks of whole columns j (R is a multiple of n); it also writes U's
-    // K padding (zeros).  When every mode-1 launch takes the block mask, U's all-zero
-    // 16 i x 16 j blocks are never read, and mode 3 does not write them.
-    const bool skip_empty = mask1 && (R / n) % 16 == 0 && nb % 16 == 0;
-    for (long r0 = 0; r0 < n2; r0 += R) {
-        const long rows = (n2 - r0) < R ? (n2 - r0) : R;
-        FC3_TRY(mode3_block_u(s, psi, n, r0 / n, (int)(rows / n), B, X, U, P, n, mask, pat,
-                              skip_empty));
-    }
-    if (mask1) {
-        hipLaunchKernelGGL(v3::k_pattern_blocks, dim3((unsigned)(Kp / 16), (unsigned)(n / 16)),
-                           dim3(256), 0, s, pat, n, Kp / 16, mask1);
-        FC3_HIP(hipGetLastError());
-    }
-    if (!k3u) {
-        for (int l0 = 0; l0 < n; l0 += nb) {
-            const int cnt = (n - l0) < nb ? (n - l0) : nb;
-            FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0, mask1));
+        // mode 3 by blocks of whole column groups: U_l(t, j) = sum_k Psi[list(t), j, k] phi[k, l]
+        // in U's gathered layout U[l 2P + 16 n jb + 256 (t / 16) + 16 (j % 16) + t % 16];
+        // tiles past a group's rows are neither split nor multiplied nor written
+        const int G = (int)(R / n / 16);
+        for (int jb0 = 0; jb0 < NG; jb0 += G) {
+            const int g = std::min(G, NG - jb0);
+            const long rows = 16L * g * n;
+            hipLaunchKernelGGL(v3::k_split_gather, dim3((unsigned)(Kp / 16), (unsigned)(rows / 256)),
+                               dim3(256), 0, s, psi, n2, n, jb0, glist, gcnt, X, rows, mask,
+                               Kp / 16);
+            FC3_HIP(hipGetLastError());
+            const OutMap oU{U + 16L * n * jb0, rows, 0, 16, 2 * P, P};
+            FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, oU, true,
+                          rows, n, Kp, 0, 0, 0, 0, 0, mask, 1, 0, 0, nullptr, 0, true));
         }
-        return FC3_OK;
+    } else {
+        // mode 3 by blocks of whole columns j (R is a multiple of n); it also writes U's
+        // K padding (zeros)
+        for (long r0 = 0; r0 < n2; r0 += R) {
+            const long rows = (n2 - r0) < R ? (n2 - r0) : R;
+            FC3_TRY(mode3_block_u(s, psi, n, r0 / n, (int)(rows / n), B, X, U, P, n, mask));
+        }
     }
     int bi = 0;
     for (int l0 = 0; l0 < n; l0 += nb, bi += 2) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        const long a = off[bi], b = off[bi + 1], e = off[bi + 2];
-        FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0, mask1, dl + a,
-                           (int)(b - a), dl + b, (int)(e - b)));
-        hipLaunchKernelGGL(k_pack_unique, dim3((unsigned)n, (unsigned)cnt), dim3(256), 0, s,
-                           k3 + n2 * l0, n, l0, k3u);
-        FC3_HIP(hipGetLastError());
+        const int* t1 = k3u ? dl + off[bi] : nullptr;
+        const int* t2 = k3u ? dl + off[bi + 1] : nullptr;
+        const int n1 = k3u ? (int)(off[bi + 1] - off[bi]) : 0;
+        const int n2t = k3u ? (int)(off[bi + 2] - off[bi + 1]) : 0;
+        if (gath) {
+            // mode 1, one product per column group jb (grid y): S_l(j, b) = sum_t U_l(t, j)
+            // Bg_jb(t, b) over the group's ceil(cnt / 16) K blocks, written as the mode-2
+            // operand S[l 2P + 16 n jb + 16 b + j % 16]
+            const Split3 Bj{Bg, (long)n * Npad, 16L * Npad};
+            const OutMap oS{S, 16, 2 * P, 16, 16, P};
+            FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U + 2 * P * l0, P, 16, 2 * P, 256, Bj, oS,
+                          true, 16L * cnt, n, Kp, 0, 16L * n, NG, 16L * n, 2L * n * Npad, gmask,
+                          1, 0, 0, t1, n1));
+            // mode 2: A(l n + b, j) = S_l(j, b);  C(l n + b, c) = K3_l[b + n c]
+            const OutMap oT{k3 + n2 * l0, n, n2, 16, n, 0};
+            FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE2, S, P, n, 2 * P, 16L * n, B, oT, false,
+                          (long)cnt * n, n, Kp, 0, 0, 0, 0, 0, nullptr, 1, 0, 0, t2, n2t));
+        } else {
+            FC3_TRY(modes12_v3(s, U + 2 * P * l0, n, cnt, B, S, k3 + n2 * l0, t1, n1, t2, n2t));
+        }
+        if (k3u) {
+            hipLaunchKernelGGL(k_pack_unique, dim3((unsigned)n, (unsigned)cnt), dim3(256), 0, s,
+                               k3 + n2 * l0, n, l0, k3u);
+            FC3_HIP(hipGetLastError());
+        }
     }
     return FC3_OK;
 }

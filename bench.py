@@ -164,11 +164,15 @@ def cpu_baseline_assembly(cells=5, reps=5):
 
 def unique_timing(ctx, d32, dphi, k3, n, slabs, a, full_ms):
     """Symmetric-unique K3 (mcc3_unique, BF16x3) on the headline Psi and phi: time per
-    call, and a bitwise check of its entries
-    against the full K3 that k3 holds (the BF16x3 runs above) on the sampled slabs."""
+    call, and a bitwise check of its entries against the full block-sparse BF16x3 K3 on
+    the sampled slabs.  (The unique form runs the same block-sparse tiles; the dense form,
+    FC3_MCC3_DENSE=1, groups mode 1's K sum differently when n % 16 == 0 and agrees with
+    it only to rounding.)"""
     import ctypes
     import fc3
     from fc3._lib import check
+    os.environ.pop("FC3_MCC3_DENSE", None)
+    fc3.mcc3(d32, dphi, precision="bf16x3", out=k3)   # the full K3 to compare against
     du = fc3.DeviceArray((fc3.unique_count(n),), np.float32, ctx)
     for _ in range(a.warmup):
         fc3.mcc3_unique(d32, dphi, precision="bf16x3", out=du)
@@ -609,9 +613,11 @@ def main():
                     "mode1_ms_per_step": bf["mode12_ms_per_step"],
                     "mode3_launches_per_step": bf["mode3_launches_per_step"],
                     "mode3_ms_per_step": bf["mode3_ms_per_step"],
-                    "note": "modes 3 and 1 multiply only the 16-wide K blocks of Psi and of U "
-                            "that hold a nonzero (the same K3 bit for bit); dense: all blocks "
-                            "(mcc3.bf16x3_dense)"}}
+                    "note": "mode 3 multiplies only the 16-wide K blocks of Psi that hold a "
+                            "nonzero, and mode 1 only the rows i of U that Psi's (i, j) pattern "
+                            "touches, gathered per 16-column group (the skipped products are "
+                            "exact zeros; the gathering regroups the K sum, so K3 equals the "
+                            "dense form to rounding); dense: all blocks (mcc3.bf16x3_dense)"}}
     roofline_assembly = {"bound": "hbm", "kernel": "k_write_ifc3_pairwise",
                          "achieved": asm["writer_GB_per_s"], "peak": PEAK["hbm"] / 1e9,
                          "unit": "GB/s", "frac": asm["writer_GB_per_s"] / (PEAK["hbm"] / 1e9),

@@ -768,22 +768,27 @@ int fc3_third_order_sw_param_backward(fc3_handle h, const double* positions, int
             if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
     const int N = n_atoms;
     const long n = 3L * N;
-    const size_t fwd = tol > 0.0 ? sizeof(double) * (size_t)n * n * n : 0;
+    (void)n;
+    // with tol > 0 the tolerance mask needs the forward values: the assembly's reduced
+    // block table (MBs), recomputed here, not a dense 8 n^3-byte forward tensor
     Prepared P;
     FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
-        return std::max(sw_workspace_bytes(N, cap, true), sw_param_grad_bytes(N, cap)) +
-               Carve::need(fwd) + Carve::need(4 * sizeof(double)) + 4096;
+        return sw_param_grad_bytes(N, cap) + (tol > 0.0 ? sw_workspace_bytes(N, cap, true) : 0) +
+               Carve::need(4 * sizeof(double)) + 4096;
     }, P));
     Carve cv(P.extra, P.extra_bytes);
     double* out = cv.take<double>(4);
-    double* psi_fwd = fwd ? cv.take<double>((size_t)n * n * n) : nullptr;
+    char* gws = cv.take<char>(sw_param_grad_bytes(N, P.nl.cap));
     char* rest = cv.take<char>(0);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "SW pullback workspace too small");
     const size_t rest_bytes = P.extra_bytes - (size_t)(rest - P.extra);
-    if (psi_fwd)   // the forward tensor, for the tolerance mask
+    SwBlockTable fwd;
+    if (tol > 0.0)
         FC3_TRY(sw_assemble(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl,
-                            rest, rest_bytes, tol, nullptr, psi_fwd, nullptr, nullptr));
+                            rest, rest_bytes, tol, nullptr, nullptr, nullptr, nullptr, &fwd));
     FC3_TRY(sw_param_grad(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl, tol,
-                          psi_fwd, masses ? P.masses : nullptr, dpsi32, dpsi64, rest, out));
+                          tol > 0.0 ? &fwd : nullptr, masses ? P.masses : nullptr, dpsi32, dpsi64,
+                          gws, out));
     FC3_HIP(hipMemcpyAsync(grad, out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     FC3_HIP(hipStreamSynchronize(h->stream));
     return FC3_OK;

@@ -148,18 +148,28 @@ int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b
 // ---- Stillinger-Weber (sw.hip) ------------------------------------------------------
 // Workspace for one SW assembly with neighbour capacity `cap` (upper bound).
 size_t sw_workspace_bytes(int N, int cap, bool third);
+// The reduced IFC3 blocks of one SW assembly: U blocks sorted by key (C N + B) N + A,
+// block p's values uval[27 p + x1 + 3 x2 + 9 x3] after ASR! (before the tolerance).
+struct SwBlockTable {
+    const unsigned long long* ukey = nullptr;
+    const double* uval = nullptr;
+    int U = 0;
+};
 // IFC3 (psi64/psi32, either may be null) when third, else IFC2 into ifc2.  params: the
 // 10 SW parameters; c0_fd / c0_iik: cos(theta0) of the generated / runtime-built blocks.
+// With table non-null: the IFC3 blocks only, left in ws and described by *table (no
+// dense tensor is written).
 int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const double* params,
                 double c0_fd, double c0_iik, const NeighborList& nl, void* ws, size_t ws_bytes,
-                double tol, const double* d_masses, double* psi64, float* psi32, double* ifc2);
+                double tol, const double* d_masses, double* psi64, float* psi32, double* ifc2,
+                SwBlockTable* table = nullptr);
 // dL/d(A, B, lambda, gamma) into d_out[0..3] (device) for the cotangent dpsi32 (of the
 // mass-weighted FP32 Psi, d_masses as in the forward call) or dpsi64 (of the FP64 Psi);
-// psi_fwd: the forward FP64 Psi, read for the tolerance mask when tol > 0.
+// fwd: the forward assembly's block table, read for the tolerance mask when tol > 0.
 size_t sw_param_grad_bytes(int N, int cap);
 int sw_param_grad(hipStream_t s, const double* d_pos, const double box[3], const double* params,
                   double c0_fd, double c0_iik, const NeighborList& nl, double tol,
-                  const double* psi_fwd, const double* d_masses, const float* dpsi32,
+                  const SwBlockTable* fwd, const double* d_masses, const float* dpsi32,
                   const double* dpsi64, void* ws, double* d_out);
 
 // ---- mcc3 (mcc3.hip) --------------------------------------------------------------

@@ -492,21 +492,36 @@ int reduce_blocks(hipStream_t s, long R, int N, u64* key, double* val, Carve& cv
 // overwrites every (A, B, A) block, so a record with C == A contributes nothing, and a
 // record with C != A reaches its own position and, with a minus sign, the ASR entry
 // (3A+x1, 3B+x2, 3A+x3).  apply_tols! zeroes entries (and their derivatives) with
-// |value| < tol, read from the forward tensor psi_fwd when tol > 0.
+// |value| < tol.  The forward values come from the reduced block table of the forward
+// assembly (ukey sorted, uval after ASR!, the values the dense writer stores), so no
+// dense forward tensor is kept: block (A, B, C) is found by binary search on its key.
 template <typename DT>
 struct GradCtx {
     const DT* dpsi;
-    const double* psi_fwd;   // forward FP64 Psi (mask), or null when tol == 0
+    const u64* ukey;         // forward block keys (mask), or null when tol == 0
+    const double* uval;
+    int U, N;
     double tol;
     const double* masses;    // mass factors for a cotangent of the mass-weighted Psi, or null
     long n;
 };
 
+// The forward block (A, B, C)'s 27 values, or null when the block is absent (all zero).
 template <typename DT>
-__device__ double pos_cot(const GradCtx<DT>& g, int A, int B, int C, int x1, int x2, int x3)
+__device__ const double* fwd_block(const GradCtx<DT>& g, int A, int B, int C)
 {
+    const u64 want = key3of(A, B, C, g.N);
+    const int p = lower_bound_u64(g.ukey, g.U, want);
+    return (p < g.U && g.ukey[p] == want) ? g.uval + (long)p * 27 : nullptr;
+}
+
+// Cotangent at (3A+x1, 3B+x2, 3C+x3); fb: that block's forward values (tol > 0 only).
+template <typename DT>
+__device__ double pos_cot(const GradCtx<DT>& g, const double* fb, int A, int B, int C, int x1,
+                          int x2, int x3)
+{
+    if (g.ukey && !(fb && fabs(fb[x1 + 3 * x2 + 9 * x3]) >= g.tol)) return 0.0;
     const long p = (3L * A + x1) + g.n * (3L * B + x2) + g.n * g.n * (3L * C + x3);
-    if (g.psi_fwd && !(fabs(g.psi_fwd[p]) >= g.tol)) return 0.0;
     double w = (double)g.dpsi[p];
     if (g.masses) w /= sqrt((g.masses[A] * g.masses[B]) * g.masses[C]);
     return w;
@@ -520,6 +535,8 @@ __device__ void place_cot(const GradCtx<DT>& g, int A, int B, int C, int p1, int
 {
     if (C == A) return;
     const int pm[3] = {p1 - 1, p2 - 1, p3 - 1};
+    const double* fc = g.ukey ? fwd_block(g, A, B, C) : nullptr;
+    const double* fa = g.ukey ? fwd_block(g, A, B, A) : nullptr;
     for (int x3 = 0; x3 < 3; ++x3)
         for (int x2 = 0; x2 < 3; ++x2)
             for (int x1 = 0; x1 < 3; ++x1) {
@@ -527,7 +544,8 @@ __device__ void place_cot(const GradCtx<DT>& g, int A, int B, int C, int p1, int
                 jj[pm[0]] = x1;
                 jj[pm[1]] = x2;
                 jj[pm[2]] = x3;
-                const double w = pos_cot(g, A, B, C, x1, x2, x3) - pos_cot(g, A, B, A, x1, x2, x3);
+                const double w = pos_cot(g, fc, A, B, C, x1, x2, x3) -
+                                 pos_cot(g, fa, A, B, A, x1, x2, x3);
                 W[jj[0] + 3 * jj[1] + 9 * jj[2]] += sgn * w;
             }
 }
@@ -668,10 +686,11 @@ size_t sw_workspace_bytes(int N, int cap, bool third)
 
 int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const double* params,
                 double c0_fd, double c0_iik, const NeighborList& nl, void* ws, size_t ws_bytes,
-                double tol, const double* d_masses, double* psi64, float* psi32, double* ifc2)
+                double tol, const double* d_masses, double* psi64, float* psi32, double* ifc2,
+                SwBlockTable* table)
 {
     const int N = nl.n_atoms;
-    const bool third = (psi64 || psi32);
+    const bool third = (psi64 || psi32 || table);
     Carve cv(ws, ws_bytes);
     int* rec3 = cv.take<int>(N + 1);
     int* rec2 = cv.take<int>(N + 1);
@@ -735,6 +754,12 @@ int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const d
     const long n = 3L * N;
     if (third) {
         FC3_TRY(reduce_blocks<27>(s, RT3, N, key3, val3, cv, ukey, uval, U));
+        if (table) {   // the reduced blocks only (they stay in ws), no dense tensor
+            table->ukey = ukey;
+            table->uval = uval;
+            table->U = U;
+            return FC3_OK;
+        }
         long cpw = 65536 / n;
         if (cpw < 1) cpw = 1;
         if (cpw > kMaxCols) cpw = kMaxCols;
@@ -775,7 +800,7 @@ size_t sw_param_grad_bytes(int N, int cap)
 
 int sw_param_grad(hipStream_t s, const double* d_pos, const double box[3], const double* params,
                   double c0_fd, double c0_iik, const NeighborList& nl, double tol,
-                  const double* psi_fwd, const double* d_masses, const float* dpsi32,
+                  const SwBlockTable* fwd, const double* d_masses, const float* dpsi32,
                   const double* dpsi64, void* ws, double* d_out)
 {
     const int N = nl.n_atoms;
@@ -797,10 +822,12 @@ int sw_param_grad(hipStream_t s, const double* d_pos, const double box[3], const
     c.key3 = nullptr; c.val3 = nullptr; c.key2 = nullptr; c.val2 = nullptr;
     const long n = 3L * N;
     if (dpsi32) {
-        GradCtx<float> g{dpsi32, tol > 0.0 ? psi_fwd : nullptr, tol, d_masses, n};
+        GradCtx<float> g{dpsi32, tol > 0.0 ? fwd->ukey : nullptr, fwd ? fwd->uval : nullptr,
+                         fwd ? fwd->U : 0, N, tol, d_masses, n};
         hipLaunchKernelGGL(k_sw_param_grad<float>, dim3((unsigned)nblk), dim3(64), 0, s, c, g, partial);
     } else {
-        GradCtx<double> g{dpsi64, tol > 0.0 ? psi_fwd : nullptr, tol, nullptr, n};
+        GradCtx<double> g{dpsi64, tol > 0.0 ? fwd->ukey : nullptr, fwd ? fwd->uval : nullptr,
+                          fwd ? fwd->U : 0, N, tol, nullptr, n};
         hipLaunchKernelGGL(k_sw_param_grad<double>, dim3((unsigned)nblk), dim3(64), 0, s, c, g, partial);
     }
     FC3_HIP(hipGetLastError());

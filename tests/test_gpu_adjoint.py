@@ -242,10 +242,12 @@ def test_lj_param_backward_against_dual_numbers(ctx, cells, weighted):
         assert rel <= 1e-10
 
 
-@pytest.mark.parametrize("tol,weighted", [(0.0, False), (1e-12, True)])
+@pytest.mark.parametrize("tol,weighted", [(0.0, False), (1e-12, True), (0.1, False), (0.1, True)])
 def test_sw_param_backward_against_dual_numbers(ctx, tol, weighted):
     """dL/d(A, B, lambda, gamma) against the oracle's 4-direction hyper-dual derivative of
-    the SW tensor (192 DoF): <= 1e-10 relative."""
+    the SW tensor (192 DoF): <= 1e-10 relative.  tol = 0.1 zeroes 38% of the nonzero
+    entries: the pullback's mask, read from the forward block table, must be the
+    oracle's mask of the assembled values."""
     sysd = S.diamond(S.SI_A, (2, 2, 2), S.SI_MASS, 0.03, seed=64)
     calc = S.AutoDiffCalculator(tol, SW_POT.r_cut)
     n = sysd.n_dof

@@ -454,6 +454,42 @@ def adjoint_timing(ctx, cells=5, reps=2):
     return res
 
 
+def vendor_gemm(n, slabs=144, reps=5):
+    """The vendor library's BF16 GEMM (torch.matmul, which calls hipBLASLt) on the shape of
+    one mode-2 launch of the mcc3 above: (slabs n) x n times n x n, operands drawn from a
+    standard normal (random mantissa bits, as the split planes have), BF16 output.  Also
+    16384^3 for scale.  Context for k_gemm3's fraction of peak, not part of the path."""
+    import torch
+    out = {"what": "torch.matmul bf16 (hipBLASLt), randn operands, median of "
+                   f"{reps} after 2 warm-ups"}
+
+    def rate(M, N, K):
+        a_ = torch.randn(M, K, device="cuda", dtype=torch.bfloat16)
+        b_ = torch.randn(K, N, device="cuda", dtype=torch.bfloat16)
+        for _ in range(2):
+            c_ = a_ @ b_
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            c_ = a_ @ b_
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        del a_, b_, c_
+        torch.cuda.empty_cache()
+        ms = float(np.median(ts))
+        log(f"vendor gemm {M}x{N}x{K}: {ms:.3f} ms")
+        return {"shape_MxNxK": [M, N, K], "ms": ms,
+                "TFLOP_per_s": 2.0 * M * N * K / (ms * 1e-3) / 1e12,
+                "frac_of_2500": 2.0 * M * N * K / (ms * 1e-3) / 2.5e15}
+
+    out["mode2_shape"] = rate(slabs * n, n, n)
+    out["square_16384"] = rate(16384, 16384, 16384)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -474,7 +510,13 @@ def main():
                     help="skip the symmetric-unique K3 (mcc3_unique) section")
     ap.add_argument("--no-sw", action="store_true",
                     help="skip the Stillinger-Weber assembly timing (diamond 3x3x3, 648 DoF)")
+    ap.add_argument("--vendor-gemm", action="store_true",
+                    help="only time the vendor BF16 GEMM (torch.matmul: hipBLASLt) on the "
+                         "mode-2 GEMM shape and print that JSON line")
     a = ap.parse_args()
+    if a.vendor_gemm:
+        print(json.dumps(vendor_gemm(3 * 4 * a.cells ** 3)), flush=True)
+        return
     if a.modes_timing:
         print("--modes-timing is disabled: a profiled run of device modes followed by the batch "
               "aborted the GPU queue, and that sequence stays off until its cause is fixed",

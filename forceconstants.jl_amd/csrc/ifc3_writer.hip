@@ -217,15 +217,24 @@ k_mass_weight3(double* __restrict__ psi, int N, int D, const double* __restrict_
 
 // Sparse mode-1 product fused to the assembly (no dense Psi).  Slab kk = 3K + g holds
 // nonzeros only in the columns jj_t = 3 J_t + b of K's neighbourhood (t = 3 s + b; J_0 =
-// K, the self column; J_s = nbr[K][s - 1], pair columns).  For the 16 such columns
-// t = 16 blockIdx.x + c,
+// K, the self column; J_s = nbr[K][s - 1], pair columns):
 //   U_K(g n + p, t) = sum_i phiT[p + n i] Psi32[i, jj_t, kk]
 // over the nonzero rows i of column (jj_t, kk), with Psi32 the value the FP32 writer
-// stores there (tolerance, mass weighting and rounding as above).  U goes out split
-// into bf16 hi/lo, K-blocked over t: hi at U[K 2 M Kc + blockIdx.x 16 M + 16 (g n + p) +
-// c] with M = 3n, lo at + M Kc; columns t past the count are zero.  Pair columns have 6
-// entries and the 3 self columns (all in the first block) 3 + 3 count[K] each, so
-// 16 * 6 + 3 * (3 + 3 cap) entries bound a block (dynamic LDS: rows, then values).
+// stores there (tolerance, mass weighting and rounding as above):
+//   pair column (s >= 1, J = J_s, when K is in J's list at slot rs): rows 3J + a with
+//     E[J][rs][a + 3b + 9g], then rows 3K + a with -E[J][rs][b + 3g + 9a];
+//   self column (s = 0): rows 3K + a with S[K][a + 3b + 9g], then for every neighbour
+//     slot s' the rows 3 I + a (I = nbr[K][s']) with E[K][s'][b + 3g + 9a].
+// Each sum is an fmaf chain from zero in that row order.  U goes out split into bf16
+// hi/lo, K-blocked over t: hi at U[K 2 M Kc + (t / 16) 16 M + 16 (g n + p) + t % 16] with
+// M = 3n, lo at + M Kc; columns t past the count are zero.
+// Workgroup per (256 consecutive p, atom K = blockIdx.y); thread p computes every column
+// and slab of its row.  The coefficients of K's neighbourhood (81 per slot: the pair
+// column's J and K rows and the slot's share of the self column, tolerance, mass weight
+// and FP32 rounding applied) go to LDS once per workgroup; a neighbour's three phi
+// values then serve its 3 columns x 3 slabs, and every U row is written once, as 32-byte
+// runs per (16 columns, slab) that are contiguous across the workgroup's lanes.
+// Dynamic LDS: lj_mode1_sparse_lds(cap) bytes.
 template <bool MW>
 __global__ void __launch_bounds__(256)
 k_lj_mode1_gathered(const int* __restrict__ count, const int* __restrict__ nbr,
@@ -235,107 +244,122 @@ k_lj_mode1_gathered(const int* __restrict__ count, const int* __restrict__ nbr,
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
     const int n = 3 * N;
-    const int kk = blockIdx.y;
-    const int K = kk / 3, g = kk - 3 * (kk / 3);
-    const int tb = blockIdx.x;
-    __shared__ int s_cnt[16], s_off[17];
-    int* e_row = reinterpret_cast<int*>(dyn);
-    float* e_val = reinterpret_cast<float*>(dyn) + (16 * 6 + 3 * (3 + 3 * cap));
+    const int K = blockIdx.y;
     const int cntK = count[K];
     const int* nbrK = nbr + (size_t)K * cap;
+    // LDS: per slot s' < cntK: cJ[27], cK[27], cS[27] (floats), then cSelf[27], then the
+    // slots' first rows 3 J (ints)
+    float* coef = reinterpret_cast<float*>(dyn);
+    float* cSelf = coef + 81 * cap;
+    int* jrow = reinterpret_cast<int*>(cSelf + 27);
     const double mK = MW ? masses[K] : 0.0;
-    // 1. entries per column (thread c < 16 owns column c)
-    const int t = threadIdx.x;
-    int J = -1, b = 0, type = 0, rs = -1;
-    if (t < 16) {
-        const int col = 16 * tb + t, sl = col / 3;
-        b = col - 3 * sl;
-        if (sl == 0) {
-            J = K;
-            type = 2;
-        } else if (sl - 1 < cntK) {
-            J = nbrK[sl - 1];
-            rs = rev[(size_t)K * cap + sl - 1];   // slot of K in J's list
-            if (rs >= 0) type = 1;
+    for (int q = threadIdx.x; q < 27 * cntK + 27; q += blockDim.x) {
+        if (q >= 27 * cntK) {
+            const int e = q - 27 * cntK;
+            const double v = tol_cut(S[(size_t)K * 27 + e], tol);
+            cSelf[e] = MW ? (float)(v != 0.0 ? v / sqrt((mK * mK) * mK) : v) : (float)v;
+            continue;
         }
-        s_cnt[t] = type == 1 ? 6 : type == 2 ? 3 + 3 * cntK : 0;
+        const int sl = q / 27, e = q - 27 * sl;   // e = a + 3 b + 9 g
+        const int a = e % 3, b = (e / 3) % 3, g = e / 9;
+        const int J = nbrK[sl];
+        const int rs = rev[(size_t)K * cap + sl];   // slot of K in J's list, or < 0
+        const double mJ = MW ? masses[J] : 0.0;
+        float fj = 0.0f, fk = 0.0f;
+        if (rs >= 0) {
+            const double* ej = E + ((size_t)J * cap + rs) * 27;
+            const double vJ = tol_cut(ej[a + 3 * b + 9 * g], tol);
+            const double vK = tol_cut(-ej[b + 3 * g + 9 * a], tol);
+            fj = MW ? (float)(vJ != 0.0 ? vJ / sqrt((mJ * mJ) * mK) : vJ) : (float)vJ;
+            fk = MW ? (float)(vK != 0.0 ? vK / sqrt((mK * mJ) * mK) : vK) : (float)vK;
+        }
+        const double vS = tol_cut(E[((size_t)K * cap + sl) * 27 + b + 3 * g + 9 * a], tol);
+        const float fs = MW ? (float)(vS != 0.0 ? vS / sqrt((mJ * mK) * mK) : vS) : (float)vS;
+        coef[81 * sl + e] = fj;
+        coef[81 * sl + 27 + e] = fk;
+        coef[81 * sl + 54 + e] = fs;
+        if (e == 0) jrow[sl] = 3 * J;
     }
     __syncthreads();
-    if (t == 0) {
-        int o = 0;
-        for (int c = 0; c < 16; ++c) { s_off[c] = o; o += s_cnt[c]; }
-        s_off[16] = o;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;   // no barrier follows
+    const float* ph = phiT + p;
+    float fK[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fK[a] = ph[(long)n * (3 * K + a)];
+    // the self column's 9 values (b, g): K's rows, then every neighbour's rows
+    float sv[9];
+#pragma unroll
+    for (int bg = 0; bg < 9; ++bg) {
+        float x = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x = fmaf(fK[a], cSelf[a + 3 * bg], x);
+        sv[bg] = x;
     }
-    __syncthreads();
-    if (t < 16 && type != 0) {
-        int* row = e_row + s_off[t];
-        float* val = e_val + s_off[t];
-        if (type == 1) {
-            const double* e = E + ((size_t)J * cap + rs) * 27;
-            const double mJ = MW ? masses[J] : 0.0;
-            for (int a = 0; a < 3; ++a) {
-                const double vJ = tol_cut(e[a + 3 * b + 9 * g], tol);
-                const double vK = tol_cut(-e[b + 3 * g + 9 * a], tol);
-                row[a] = 3 * J + a;
-                row[3 + a] = 3 * K + a;
-                if (MW) {
-                    val[a] = (float)(vJ != 0.0 ? vJ / sqrt((mJ * mJ) * mK) : vJ);
-                    val[3 + a] = (float)(vK != 0.0 ? vK / sqrt((mK * mJ) * mK) : vK);
-                } else {
-                    val[a] = (float)vJ;
-                    val[3 + a] = (float)vK;
-                }
-            }
-        } else {
-            // self column: rows of K (ASR! self block) and of every neighbour I of K
-            for (int a = 0; a < 3; ++a) {
-                const double v = tol_cut(S[(size_t)K * 27 + a + 3 * b + 9 * g], tol);
-                row[a] = 3 * K + a;
-                val[a] = MW ? (float)(v != 0.0 ? v / sqrt((mK * mK) * mK) : v) : (float)v;
-            }
-            for (int sl = 0; sl < cntK; ++sl) {
-                const int I = nbrK[sl];
-                const double mI = MW ? masses[I] : 0.0;
-                for (int a = 0; a < 3; ++a) {
-                    const double v =
-                        tol_cut(E[((size_t)K * cap + sl) * 27 + b + 3 * g + 9 * a], tol);
-                    row[3 + 3 * sl + a] = 3 * I + a;
-                    val[3 + 3 * sl + a] =
-                        MW ? (float)(v != 0.0 ? v / sqrt((mI * mK) * mK) : v) : (float)v;
-                }
-            }
+    for (int sl = 0; sl < cntK; ++sl) {
+        const float* fI = ph + (long)n * jrow[sl];
+        const float f0 = fI[0], f1 = fI[n], f2 = fI[2L * n];
+        const float* cs = coef + 81 * sl + 54;
+#pragma unroll
+        for (int bg = 0; bg < 9; ++bg) {
+            float x = sv[bg];
+            x = fmaf(f0, cs[3 * bg], x);
+            x = fmaf(f1, cs[3 * bg + 1], x);
+            x = fmaf(f2, cs[3 * bg + 2], x);
+            sv[bg] = x;
         }
     }
-    __syncthreads();
-    // 2. thread p: the 16 columns' sums, split and stored as two 32-byte runs
     const long M = 3L * n;
-    __bf16* Uh = U + (long)K * 2 * M * Kc + (long)tb * 16 * M + 16L * g * n;
-    for (int p = t; p < n; p += blockDim.x) {
-        float acc[16];
+    __bf16* Uk = U + (long)K * 2 * M * Kc + 16L * p;
+    for (int tb = 0; tb < Kc / 16; ++tb) {
+        float acc[3][16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
-            float a = 0.0f;
-            for (int e = s_off[c]; e < s_off[c + 1]; ++e)
-                a = fmaf(phiT[p + (long)n * e_row[e]], e_val[e], a);
-            acc[c] = a;
-        }
-        bf16x8 h0, h1, l0, l1;
+            const int t = 16 * tb + c, sl = t / 3, b = t - 3 * sl;
+            if (sl == 0) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            __bf16 hh, ll;
-            split_bf16(acc[c], hh, ll);
-            h0[c] = hh;
-            l0[c] = ll;
-            split_bf16(acc[8 + c], hh, ll);
-            h1[c] = hh;
-            l1[c] = ll;
+                for (int g = 0; g < 3; ++g) acc[g][c] = sv[b + 3 * g];
+            } else if (sl - 1 < cntK) {
+                const float* fJ = ph + (long)n * jrow[sl - 1];
+                const float j0 = fJ[0], j1 = fJ[n], j2 = fJ[2L * n];
+                const float* cj = coef + 81 * (sl - 1) + 3 * b;
+#pragma unroll
+                for (int g = 0; g < 3; ++g) {
+                    const float* cg = cj + 9 * g;
+                    float x = fmaf(j0, cg[0], 0.0f);
+                    x = fmaf(j1, cg[1], x);
+                    x = fmaf(j2, cg[2], x);
+                    x = fmaf(fK[0], cg[27], x);
+                    x = fmaf(fK[1], cg[28], x);
+                    x = fmaf(fK[2], cg[29], x);
+                    acc[g][c] = x;
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < 3; ++g) acc[g][c] = 0.0f;
+            }
         }
-        bf16x8* dh = reinterpret_cast<bf16x8*>(Uh + 16L * p);
-        bf16x8* dl = reinterpret_cast<bf16x8*>(Uh + M * Kc + 16L * p);
-        dh[0] = h0;
-        dh[1] = h1;
-        dl[0] = l0;
-        dl[1] = l1;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            bf16x8 h0, h1, l0, l1;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                __bf16 hh, ll;
+                split_bf16(acc[g][c], hh, ll);
+                h0[c] = hh;
+                l0[c] = ll;
+                split_bf16(acc[g][8 + c], hh, ll);
+                h1[c] = hh;
+                l1[c] = ll;
+            }
+            __bf16* Uh = Uk + (long)tb * 16 * M + 16L * g * n;
+            bf16x8* dh = reinterpret_cast<bf16x8*>(Uh);
+            bf16x8* dl = reinterpret_cast<bf16x8*>(Uh + M * Kc);
+            dh[0] = h0;
+            dh[1] = h1;
+            dl[0] = l0;
+            dl[1] = l1;
+        }
     }
 }
 
@@ -360,7 +384,7 @@ int write_ifc3_pairwise(hipStream_t s, const NeighborList& nl, const Ifc3Blocks&
 
 size_t lj_mode1_sparse_lds(int cap)
 {
-    return (size_t)(16 * 6 + 3 * (3 + 3 * cap)) * (sizeof(int) + sizeof(float));
+    return (size_t)(81 * cap + 27) * sizeof(float) + (size_t)cap * sizeof(int);
 }
 
 int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
@@ -372,7 +396,7 @@ int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b
         return set_error(FC3_ERR_UNSUPPORTED, "fused mode 1: too many neighbours per atom");
     if (Kc % 16 != 0 || Kc < 3 * (nl.cap + 1))
         return set_error(FC3_ERR_ARGUMENT, "fused mode 1: gathered K too small");
-    dim3 grid((unsigned)(Kc / 16), (unsigned)n);
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)nl.n_atoms);
     ProfScope prof(FC3_KERNEL_IFC3_WRITE, s);
     if (d_masses)
         hipLaunchKernelGGL(k_lj_mode1_gathered<true>, grid, dim3(256), lds, s, nl.count, nl.nbr,

@@ -247,8 +247,10 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
     (seeded), phi per snapshot from the host FP64 eigendecomposition of its dynamical
     matrix (with modes_timing, the device modes are timed beside it for one snapshot per
     size); assembly + BF16x3 mcc3 of all snapshots in one graph launch.  K3 outputs
-    rotate through two buffers per size (the whole set would need 494 GB); the error
-    is checked on sampled slabs of the last two snapshots of each size."""
+    rotate through 8 / 4 / 2 buffers for 324 / 768 / 1500 DoF (36 GB; the whole set
+    would need 494 GB): snapshots sharing a buffer form one lane of the graph, the lanes
+    run concurrently.  The error is checked on sampled slabs of the last two snapshots
+    of each size."""
     import fc3
     from fc3 import systems as S
     from oracle import fc3_oracle as O
@@ -270,15 +272,17 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
             w2, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, S.ARGON_LJ, calc, ctx=ctx))
             dph = fc3.DeviceArray.from_host(np.asfortranarray(phi.astype(np.float32)), ctx=ctx)
             phi_s += time.perf_counter() - tp
+            nbuf = {3: 8, 4: 4}.get(cells, 2)
             if n not in out_bufs:
-                out_bufs[n] = [fc3.DeviceArray((n, n, n), np.float32, ctx) for _ in range(2)]
-            snaps.append((sysd, dph, out_bufs[n][j % 2]))
+                out_bufs[n] = [fc3.DeviceArray((n, n, n), np.float32, ctx) for _ in range(nbuf)]
+            snaps.append((sysd, dph, out_bufs[n][j % nbuf]))
             meta.append((sysd, dph, seed, su, w2))
         log(f"batched: phi for {per_size} snapshots of FCC {cells}^3 ready "
             f"({time.perf_counter() - t0:.0f} s)")
     fc3.batch_third_order_mcc3(snaps, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)  # tune + capture
     ctx.synchronize()
     nodes = fc3.batch_graph_nodes(ctx)
+    lanes = fc3.batch_graph_lanes(ctx)
     ctx.timer_start()
     for _ in range(replays):
         fc3.batch_third_order_mcc3(snaps, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)
@@ -289,6 +293,21 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
     for n in sorted(out_bufs):
         v, tiles, ms = fc3.autotune_query(n, "bf16x3", ctx=ctx)
         tune[str(n)] = {"variant": v, "tiles": tiles, "snapshot_ms": ms}
+    # each size class as a batch of its own: the lanes' concurrency for small snapshots
+    by_size = {}
+    for n in sorted(out_bufs):
+        sub = [sn for sn, m in zip(snaps, meta) if 3 * m[0].n_atoms == n]
+        fc3.batch_third_order_mcc3(sub, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(replays):
+            fc3.batch_third_order_mcc3(sub, S.ARGON_LJ, calc, precision="bf16x3", ctx=ctx)
+        ts = ctx.timer_stop() / replays
+        by_size[str(n)] = {
+            "snapshots": len(sub), "graph_lanes": fc3.batch_graph_lanes(ctx), "ms": ts,
+            "serial_sum_ms": len(sub) * tune[str(n)]["snapshot_ms"],
+            "K3_TFLOP_per_s": len(sub) * 6.0 * n ** 4 / (ts * 1e-3) / 1e12}
+    log(f"batched per size: {by_size}")
     # error of the surviving outputs (last two per size) on sampled slabs
     import ctypes
     from fc3._lib import check
@@ -356,7 +375,12 @@ def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
         m[1].free()
     res = {"snapshots": nsnap, "sizes_dof": sorted(out_bufs), "ms_per_batch": t,
            "snapshots_per_s": nsnap / (t * 1e-3), "K3_TFLOP_per_s": flop / (t * 1e-3) / 1e12,
-           "graph_nodes": nodes, "replays_timed": replays, "autotuned": tune,
+           "graph_nodes": nodes, "graph_lanes": lanes, "replays_timed": replays,
+           "per_size": by_size,
+           "autotuned": tune,
+           # one snapshot after another at the tuner's per-snapshot times: what the
+           # batch would take without concurrent lanes
+           "serial_sum_ms": sum(tune[str(3 * m[0].n_atoms)]["snapshot_ms"] for m in meta),
            "errors_sampled": errs, "modes_setup_per_snapshot": modes,
            "phi_setup_s": phi_s, "device_modes": dev,
            "note": "assembly (FP64 blocks, mass-weighted FP32 Psi) + BF16x3 mcc3 per snapshot, "

@@ -413,8 +413,8 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
     b.S = S3;
     FC3_TRY(transpose_square(h->stream, phi, (int)n, phiT));
     FC3_TRY(lj_mode1_gathered(h->stream, P.nl, b, tol, masses ? P.masses : nullptr, phiT, Kc, U));
-    return mcc3_from_gathered(h->stream, U, phi, (int)n, P.nl.count, P.nl.nbr, P.nl.cap, k3, rest,
-                              cv.cap - cv.off);
+    return mcc3_from_gathered(h->stream, U, phi, phiT, (int)n, P.nl.count, P.nl.nbr, P.nl.cap,
+                              k3, rest, cv.cap - cv.off);
 }
 
 int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int dim,

@@ -37,6 +37,11 @@ struct BatchState {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     size_t nodes = 0;
+    int lanes = 0;                  // parallel branches of the cached graph
+    // capture streams of the lanes (default priority) and their fork / join events
+    std::vector<hipStream_t> lane;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
     struct Tuned {
         Mcc3Config cfg;
         float ms = 0.0f;
@@ -51,6 +56,9 @@ void release_batch(fc3_context* h)
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     if (b->ws) hipFree(b->ws);
+    for (hipStream_t ls : b->lane) hipStreamDestroy(ls);
+    for (hipEvent_t e : b->join) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
     delete b;
     h->batch = nullptr;
 }
@@ -59,6 +67,49 @@ static BatchState* state(fc3_context* h)
 {
     if (!h->batch) h->batch = new BatchState();
     return static_cast<BatchState*>(h->batch);
+}
+
+// Grow the batch workspace to at least `need` bytes (1 MiB granularity); the old buffer
+// is freed first (after the stream has drained) and the cached graph, which refers to it,
+// is dropped.  *moved tells whether the buffer changed (its contents are then lost).
+static int grow_batch_ws(BatchState* B, size_t need, hipStream_t st, bool* moved)
+{
+    *moved = false;
+    if (need <= B->ws_bytes) return FC3_OK;
+    if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+    if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
+    B->key.clear();
+    if (B->ws) {
+        FC3_HIP(hipStreamSynchronize(st));
+        FC3_HIP(hipFree(B->ws));
+        B->ws = nullptr;
+        B->ws_bytes = 0;
+    }
+    const size_t want = (need + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+    hipError_t e = hipMalloc(&B->ws, want);
+    if (e != hipSuccess) {
+        B->ws = nullptr;
+        return set_error(FC3_ERR_OUT_OF_MEMORY, "batch workspace of " + std::to_string(want) +
+                                                    " bytes: " + hipGetErrorString(e));
+    }
+    B->ws_bytes = want;
+    *moved = true;
+    return FC3_OK;
+}
+
+// At least `n` lane capture streams (default priority) with their join events.
+static int ensure_lane_streams(BatchState* B, int n)
+{
+    if (!B->fork) FC3_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+    while ((int)B->lane.size() < n) {
+        hipStream_t ls = nullptr;
+        FC3_HIP(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+        B->lane.push_back(ls);
+        hipEvent_t e = nullptr;
+        FC3_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        B->join.push_back(e);
+    }
+    return FC3_OK;
 }
 
 namespace {
@@ -145,11 +196,152 @@ int snapshot_work(hipStream_t s, char* base, const SnapPlan& P, const fc3_snapsh
         if (used > mws_bytes) return set_error(FC3_ERR_OUT_OF_MEMORY, "batch fused scratch");
         FC3_TRY(transpose_square(s, sn.phi, n, phiT));
         FC3_TRY(lj_mode1_gathered(s, nl, b, prm.tol, m, phiT, Kc, U));
-        return mcc3_from_gathered(s, U, sn.phi, n, nl.count, nl.nbr, P.cap, sn.k3, w,
+        return mcc3_from_gathered(s, U, sn.phi, phiT, n, nl.count, nl.nbr, P.cap, sn.k3, w,
                                   mws_bytes - used);
     }
     FC3_TRY(write_ifc3_pairwise(s, nl, b, prm.tol, m, nullptr, psi32));
     return mcc3_run(s, psi32, sn.phi, (int)P.n, prm.precision, sn.k3, mws, mws_bytes, &P.cfg);
+}
+
+// ---- lanes ----------------------------------------------------------------------
+// The batch runs as parallel branches of one graph.  Snapshots whose K3 outputs overlap
+// form a chain that stays in one lane in submission order (so a shared output buffer ends
+// with the result a sequential loop would leave); the chains of one size class (largest
+// n of the chain) are dealt round robin over that class's lanes.  A lane owns a Psi
+// buffer and an mcc3 workspace sized for its snapshots' configurations and runs its
+// snapshots one after another; the lanes run concurrently.  Lane counts shrink, the
+// class with the largest per-lane need first, until the lanes fit `budget` bytes; at one
+// lane per class and still too large, everything runs in a single lane.
+constexpr int kMaxLanes = 16;
+constexpr int kMaxLanesPerClass = 8;
+
+struct LanePlan {
+    std::vector<std::vector<int>> lanes;   // snapshot indices per lane, ascending
+    std::vector<size_t> off, off_mws, mws; // lane start, its workspace offset and bytes
+    size_t bytes = 0;                      // the whole lane region
+};
+
+size_t snap_psi_bytes(const SnapPlan& p)
+{
+    return p.cfg.variant == kVariantFused ? 0 : sizeof(float) * (size_t)p.n * p.n * p.n;
+}
+
+size_t snap_mws_bytes(const SnapPlan& p, int precision)
+{
+    return p.cfg.variant == kVariantFused ? fused_bytes(p.n, p.cap)
+                                          : mcc3_workspace_bytes((int)p.n, precision, false, &p.cfg);
+}
+
+LanePlan layout_lanes(const std::vector<std::vector<int>>& lanes, const std::vector<SnapPlan>& P,
+                      int precision)
+{
+    LanePlan L;
+    L.lanes = lanes;
+    for (const auto& ln : lanes) {
+        size_t psi = 0, mws = 0;
+        for (int i : ln) {
+            psi = std::max(psi, snap_psi_bytes(P[i]));
+            mws = std::max(mws, snap_mws_bytes(P[i], precision));
+        }
+        L.off.push_back(L.bytes);
+        L.off_mws.push_back(align256(psi));
+        L.mws.push_back(mws);
+        L.bytes += align256(psi) + align256(mws);
+    }
+    return L;
+}
+
+LanePlan plan_lanes(const std::vector<SnapPlan>& P, const fc3_snapshot* snaps, int count,
+                    int precision, size_t budget)
+{
+    // chains: union of snapshots whose K3 byte ranges overlap
+    std::vector<int> parent(count);
+    for (int i = 0; i < count; ++i) parent[i] = i;
+    auto find = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    std::vector<int> ord(count);
+    for (int i = 0; i < count; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return reinterpret_cast<uintptr_t>(snaps[a].k3) < reinterpret_cast<uintptr_t>(snaps[b].k3);
+    });
+    uintptr_t run_end = 0;
+    int run = -1;
+    for (int i : ord) {
+        const uintptr_t s0 = reinterpret_cast<uintptr_t>(snaps[i].k3);
+        const uintptr_t e0 = s0 + sizeof(float) * (uintptr_t)P[i].n * P[i].n * P[i].n;
+        if (run >= 0 && s0 < run_end) {
+            parent[find(i)] = find(run);
+            run_end = std::max(run_end, e0);
+        } else {
+            run = i;
+            run_end = e0;
+        }
+    }
+    // chains in order of their first snapshot, grouped by class
+    std::map<int, std::vector<int>> chain;   // root -> members (ascending)
+    std::vector<int> first;                  // roots in order of first member
+    for (int i = 0; i < count; ++i) {
+        const int r = find(i);
+        if (!chain.count(r)) first.push_back(r);
+        chain[r].push_back(i);
+    }
+    std::map<long, std::vector<int>> cls;    // class n -> chain roots
+    for (int r : first) {
+        long n = 0;
+        for (int i : chain[r]) n = std::max(n, P[i].n);
+        cls[n].push_back(r);
+    }
+    std::map<long, int> nl;
+    int total = 0;
+    for (const auto& kv : cls) {
+        nl[kv.first] = std::min<int>((int)kv.second.size(), kMaxLanesPerClass);
+        total += nl[kv.first];
+    }
+    while (total > kMaxLanes) {
+        auto big = std::max_element(nl.begin(), nl.end(),
+                                    [](const auto& a, const auto& b) { return a.second < b.second; });
+        --big->second;
+        --total;
+    }
+    auto build = [&]() {
+        std::vector<std::vector<int>> lanes;
+        std::vector<long> lane_cls;
+        for (const auto& kv : cls) {
+            const int k = nl[kv.first];
+            const size_t l0 = lanes.size();
+            lanes.resize(l0 + k);
+            lane_cls.resize(l0 + k, kv.first);
+            for (size_t c = 0; c < kv.second.size(); ++c) {
+                auto& dst = lanes[l0 + c % k];
+                const auto& m = chain[kv.second[c]];
+                dst.insert(dst.end(), m.begin(), m.end());
+            }
+            for (size_t l = l0; l < lanes.size(); ++l) std::sort(lanes[l].begin(), lanes[l].end());
+        }
+        return std::make_pair(lanes, lane_cls);
+    };
+    for (;;) {
+        auto b = build();
+        LanePlan L = layout_lanes(b.first, P, precision);
+        if (L.bytes <= budget) return L;
+        // the class whose lanes need the most shrinks by one lane
+        long worst = -1;
+        size_t worst_need = 0;
+        for (size_t l = 0; l < L.lanes.size(); ++l) {
+            const size_t need = L.off_mws[l] + align256(L.mws[l]);
+            if (nl[b.second[l]] > 1 && need > worst_need) {
+                worst_need = need;
+                worst = b.second[l];
+            }
+        }
+        if (worst < 0) break;
+        --nl[worst];
+    }
+    std::vector<int> all(count);
+    for (int i = 0; i < count; ++i) all[i] = i;
+    return layout_lanes({all}, P, precision);
 }
 
 // Device modes of every snapshot (FC3_BATCH_DEVICE_MODES): get_modes(dynamical_matrix(sys,
@@ -385,41 +577,38 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
         else if (P[i].cfg.variant == kVariantFused)
             P[i].cfg = Mcc3Config{};   // a loaded table may name it where it does not apply
     }
+    // tuning scratch (one shared Psi of the largest size and an mcc3 workspace for every
+    // candidate), only while a size still has to be tuned; the lanes reuse these bytes
+    bool need_tune = false;
+    if (tune)
+        for (int i = 0; i < count; ++i)
+            if (!B->tuned.count({(int)P[i].n, precision})) need_tune = true;
     const size_t off_psi = off;
-    off = align256(off + sizeof(float) * (size_t)max_n * max_n * max_n);
-    const size_t off_mws = off;
-    off = align256(off + max_mws);
-    if (off > B->ws_bytes) {
-        if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
-        if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
-        B->key.clear();
-        if (B->ws) { FC3_HIP(hipStreamSynchronize(st)); FC3_HIP(hipFree(B->ws)); B->ws = nullptr; B->ws_bytes = 0; }
-        const size_t want = (off + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
-        hipError_t e = hipMalloc(&B->ws, want);
-        if (e != hipSuccess) {
-            B->ws = nullptr;
-            return set_error(FC3_ERR_OUT_OF_MEMORY, "batch workspace of " + std::to_string(want) +
-                                                        " bytes: " + hipGetErrorString(e));
-        }
-        B->ws_bytes = want;
-    }
+    const size_t off_mws = align256(off_psi + sizeof(float) * (size_t)max_n * max_n * max_n);
+    const size_t regions = off;
+    if (need_tune) off = align256(off_mws + max_mws);
+    bool moved = false;
+    FC3_TRY(grow_batch_ws(B, off, st, &moved));
     char* base = static_cast<char*>(B->ws);
     // inputs and neighbour counts into the batch workspace (outside the graph)
-    for (int i = 0; i < count; ++i) {
-        const SnapPlan& p = P[i];
-        FC3_HIP(hipMemcpyAsync(base + p.off_pos, snaps[i].positions, sizeof(double) * 3 * p.N,
-                               hipMemcpyHostToDevice, st));
-        if (snaps[i].masses)
-            FC3_HIP(hipMemcpyAsync(base + p.off_m, snaps[i].masses, sizeof(double) * p.N,
+    auto upload = [&]() {
+        for (int i = 0; i < count; ++i) {
+            const SnapPlan& p = P[i];
+            FC3_HIP(hipMemcpyAsync(base + p.off_pos, snaps[i].positions, sizeof(double) * 3 * p.N,
                                    hipMemcpyHostToDevice, st));
-        int hmax = 0;
-        int* cnt = reinterpret_cast<int*>(base + p.off_count);
-        FC3_TRY(neighbor_count(st, reinterpret_cast<double*>(base + p.off_pos), p.N, snaps[i].box,
-                               r_cut, cnt, cnt + p.N, &hmax));
-        if (hmax > p.cap) return set_error(FC3_ERR_CAPACITY, "neighbour count changed between passes");
-    }
-    float* psi32 = reinterpret_cast<float*>(base + off_psi);
-    void* mws = base + off_mws;
+            if (snaps[i].masses)
+                FC3_HIP(hipMemcpyAsync(base + p.off_m, snaps[i].masses, sizeof(double) * p.N,
+                                       hipMemcpyHostToDevice, st));
+            int hmax = 0;
+            int* cnt = reinterpret_cast<int*>(base + p.off_count);
+            FC3_TRY(neighbor_count(st, reinterpret_cast<double*>(base + p.off_pos), p.N,
+                                   snaps[i].box, r_cut, cnt, cnt + p.N, &hmax));
+            if (hmax > p.cap)
+                return set_error(FC3_ERR_CAPACITY, "neighbour count changed between passes");
+        }
+        return (int)FC3_OK;
+    };
+    FC3_TRY(upload());
 
     // ---- phi on the device (before tuning, which runs the snapshots) ----
     if (flags & FC3_BATCH_DEVICE_MODES) {
@@ -432,13 +621,17 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
 
     // ---- tune sizes not seen before ----
     const bool prof_was = h->prof.enabled;
-    if (tune) {
+    if (need_tune) {
+        float* tpsi = reinterpret_cast<float*>(base + off_psi);
+        void* tmws = base + off_mws;
         h->prof.enabled = false;
         for (int i = 0; i < count; ++i) {
             if (B->tuned.count({(int)P[i].n, precision})) continue;
-            FC3_TRY(tune_size(h, B, base, P[i], snaps[i], prm, psi32, mws, max_mws));
+            FC3_TRY(tune_size(h, B, base, P[i], snaps[i], prm, tpsi, tmws, max_mws));
         }
         h->prof.enabled = prof_was;
+    }
+    if (tune) {
         for (int i = 0; i < count; ++i) {
             P[i].cfg = B->tuned[{(int)P[i].n, precision}].cfg;
             if (P[i].cfg.variant == kVariantFused &&
@@ -447,9 +640,26 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
         }
     }
 
+    // ---- lanes: parallel branches of the graph, each with its own Psi and scratch ----
+    size_t free_b = 0, total_b = 0;
+    FC3_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t reusable = B->ws_bytes - regions;   // bytes past the snapshot regions
+    const size_t reserve = size_t(2) << 30;
+    const size_t budget = free_b + reusable > reserve ? free_b + reusable - reserve : 0;
+    const LanePlan L = plan_lanes(P, snaps, count, precision, budget);
+    FC3_TRY(grow_batch_ws(B, regions + L.bytes, st, &moved));
+    base = static_cast<char*>(B->ws);
+    if (moved) FC3_TRY(upload());   // a new buffer: positions, masses and counts again
+    FC3_TRY(ensure_lane_streams(B, (int)L.lanes.size()));
+    auto lane_work = [&](hipStream_t s, int l, int i) {
+        char* lb = base + regions + L.off[l];
+        return snapshot_work(s, base, P[i], snaps[i], prm,
+                             reinterpret_cast<float*>(lb), lb + L.off_mws[l], L.mws[l]);
+    };
+
     if (flags & FC3_BATCH_NO_GRAPH) {
-        for (int i = 0; i < count; ++i)
-            FC3_TRY(snapshot_work(st, base, P[i], snaps[i], prm, psi32, mws, max_mws));
+        for (size_t l = 0; l < L.lanes.size(); ++l)
+            for (int i : L.lanes[l]) FC3_TRY(lane_work(st, (int)l, i));
         return FC3_OK;
     }
 
@@ -473,23 +683,52 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
         put(key, snaps[i].masses != nullptr);
         put(key, snaps[i].box);
     }
+    put(key, regions);
+    for (size_t l = 0; l < L.lanes.size(); ++l) {
+        put(key, L.off[l]);
+        put(key, L.off_mws[l]);
+        put(key, L.mws[l]);
+        put(key, L.lanes[l].size());
+        for (int i : L.lanes[l]) put(key, i);
+    }
     if (!B->exec || key != B->key) {
         if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
         if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
+        B->key.clear();
         h->prof.enabled = false;   // no event records inside the capture
         FC3_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
         int rc = FC3_OK;
-        for (int i = 0; i < count && rc == FC3_OK; ++i)
-            rc = snapshot_work(st, base, P[i], snaps[i], prm, psi32, mws, max_mws);
+        hipError_t fe = hipSuccess;
+        if (L.lanes.size() == 1) {
+            for (int i : L.lanes[0])
+                if (rc == FC3_OK) rc = lane_work(st, 0, i);
+        } else {
+            // fork: every lane waits for the capture origin; join: the origin waits for
+            // every lane (also after an error, so that the capture can end)
+            if (fe == hipSuccess) fe = hipEventRecord(B->fork, st);
+            for (size_t l = 0; l < L.lanes.size(); ++l) {
+                hipStream_t ls = B->lane[l];
+                if (fe == hipSuccess) fe = hipStreamWaitEvent(ls, B->fork, 0);
+                for (int i : L.lanes[l])
+                    if (rc == FC3_OK && fe == hipSuccess) rc = lane_work(ls, (int)l, i);
+                if (fe == hipSuccess) fe = hipEventRecord(B->join[l], ls);
+                if (fe == hipSuccess) fe = hipStreamWaitEvent(st, B->join[l], 0);
+            }
+        }
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(st, &g);
         h->prof.enabled = prof_was;
         if (rc != FC3_OK) { if (g) hipGraphDestroy(g); return rc; }
+        if (fe != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return set_error(FC3_ERR_HIP, std::string("graph fork/join: ") + hipGetErrorString(fe));
+        }
         if (e != hipSuccess) return set_error(FC3_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         B->graph = g;
         FC3_HIP(hipGraphGetNodes(g, nullptr, &B->nodes));
         FC3_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
         B->key = key;
+        B->lanes = (int)L.lanes.size();
     }
     FC3_HIP(hipGraphLaunch(B->exec, st));
     return FC3_OK;
@@ -499,6 +738,13 @@ int fc3_batch_graph_nodes(fc3_handle h, size_t* nodes)
 {
     if (!h || !nodes) return set_error(FC3_ERR_ARGUMENT, "null argument");
     *nodes = h->batch ? static_cast<BatchState*>(h->batch)->nodes : 0;
+    return FC3_OK;
+}
+
+int fc3_batch_graph_lanes(fc3_handle h, int* lanes)
+{
+    if (!h || !lanes) return set_error(FC3_ERR_ARGUMENT, "null argument");
+    *lanes = h->batch ? static_cast<BatchState*>(h->batch)->lanes : 0;
     return FC3_OK;
 }
 

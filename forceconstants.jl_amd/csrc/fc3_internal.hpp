@@ -217,10 +217,12 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
 // Modes 2 and 3 of mcc3 on the BF16X3 kernels from the gathered mode-1 result U of
 // lj_mode1_gathered: mode 2 as one batched product per atom over its neighbourhood's
 // columns (gathered phi rows), mode 3 in place; K3 FP32 in k3 (n % 4 == 0, n >= 16).
+// phiT: phi transposed (phiT[q + n i] = phi[i + n q]), for the row gather.
 int mcc3_gather_k(int cap);
 size_t mcc3_from_gathered_bytes(int n, int cap);
-int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, int n, const int* count,
-                       const int* nbr, int cap, float* k3, void* ws, size_t ws_bytes);
+int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, const float* phiT, int n,
+                       const int* count, const int* nbr, int cap, float* k3, void* ws,
+                       size_t ws_bytes);
 // Bounded-memory mcc3 (fc3_mcc3_blocked): K3 slabs [l0, l0 + cnt), cnt <= bs, into `out`
 // (n^2 cnt floats, device) from the whole Psi, bitwise equal to the same slabs of
 // mcc3_run.  v3 = mcc3_blocked_v3(n, precision, psi) selects the BF16X3 kernels (else

@@ -1920,7 +1920,7 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
 // row_K(3 s + b) = 3 K + b (s = 0) or 3 nbr[K][s - 1] + b (s <= count[K]), zero past
 // the count or past n; split, K-blocked, hi at Bg[K 2 Kc Npad + (t / 16) 16 Npad + 16 q +
 // t % 16], lo at + Kc Npad.  Thread (q, t block, K) writes 16 t as two 32-byte runs.
-__global__ void __launch_bounds__(256) k_gather_phi_nbr(const float* __restrict__ phi, int n,
+__global__ void __launch_bounds__(256) k_gather_phi_nbr(const float* __restrict__ phiT, int n,
                                                          const int* __restrict__ count,
                                                          const int* __restrict__ nbr, int cap,
                                                          int Kc, int Npad, __bf16* __restrict__ Bg)
@@ -1936,7 +1936,7 @@ __global__ void __launch_bounds__(256) k_gather_phi_nbr(const float* __restrict_
         int row = -1;
         if (sl == 0) row = 3 * K + b;
         else if (sl - 1 < cnt) row = 3 * nbr[(size_t)K * cap + sl - 1] + b;
-        const float x = (row >= 0 && q < n) ? phi[row + (long)n * q] : 0.0f;
+        const float x = (row >= 0 && q < n) ? phiT[q + (long)n * row] : 0.0f;
         __bf16 hh, ll;
         split_bf16(x, hh, ll);
         if (c < 8) { h0[c] = hh; l0[c] = ll; } else { h1[c - 8] = hh; l1[c - 8] = ll; }
@@ -1960,8 +1960,9 @@ size_t mcc3_from_gathered_bytes(int n, int cap)
            Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n)) + 4096;
 }
 
-int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, int n, const int* count,
-                       const int* nbr, int cap, float* k3, void* ws, size_t ws_bytes)
+int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, const float* phiT, int n,
+                       const int* count, const int* nbr, int cap, float* k3, void* ws,
+                       size_t ws_bytes)
 {
     if (!v3_applies(n, FC3_PREC_BF16X3) || !aligned16(U) || !aligned16(k3) || n % 3 != 0)
         return set_error(FC3_ERR_UNSUPPORTED, "fused mcc3: needs n % 4 == 0, n >= 16");
@@ -1976,7 +1977,7 @@ int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, int n, 
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "fused mcc3 workspace too small");
     hipLaunchKernelGGL(k_gather_phi_nbr, dim3((unsigned)((Npad + 255) / 256), (unsigned)(Kc / 16),
                                               (unsigned)N),
-                       dim3(256), 0, s, phi, n, count, nbr, cap, Kc, Npad, Bg);
+                       dim3(256), 0, s, phiT, n, count, nbr, cap, Kc, Npad, Bg);
     FC3_HIP(hipGetLastError());
     // mode 2, one product per atom K over its 3 slabs: C(g n + p, q) = K3[p + n q + n^2 (3K + g)]
     // = sum_t U_K(g n + p, t) B_K(t, q)

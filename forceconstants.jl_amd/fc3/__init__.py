@@ -6,7 +6,8 @@ hand-written HIP kernels for gfx950 in libfc3.so.  See api.py for the
 reference-named entry points.
 """
 from . import systems
-from .api import (asr_satisfied, autotune_load, autotune_query, autotune_save, batch_graph_nodes,
+from .api import (asr_satisfied, autotune_load, autotune_query, autotune_save, batch_graph_lanes,
+                  batch_graph_nodes,
                   batch_third_order_mcc3, Context, DenseForceConstants, DeviceArray,
                   dynamical_matrix, format_float, get_modes, issymmetric,
                   k3_from_unique, mass_weight_third_order_, mcc3, mcc3_unique, unique_count,
@@ -18,7 +19,8 @@ from .systems import (AnalyticalCalculator, AutoDiffCalculator, LJ, StillingerWe
                       SuperCellSystem, SWParams, diamond, fcc)
 
 __all__ = [
-    "asr_satisfied", "autotune_load", "autotune_query", "autotune_save", "batch_graph_nodes",
+    "asr_satisfied", "autotune_load", "autotune_query", "autotune_save", "batch_graph_lanes",
+    "batch_graph_nodes",
     "batch_third_order_mcc3", "Context", "DenseForceConstants", "DeviceArray",
     "dynamical_matrix", "format_float", "get_modes", "issymmetric", "k3_from_unique", "mass_weight_third_order_", "mcc3",
     "mcc3_unique", "unique_count", "unique_index",

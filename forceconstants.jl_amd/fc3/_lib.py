@@ -98,6 +98,7 @@ SIGNATURES = {
                                                _c_double, _c_double, _c_double, _c_double, _c_int,
                                                _c_int]),
     "fc3_batch_graph_nodes": (_c_int, [_handle, ctypes.POINTER(_c_size)]),
+    "fc3_batch_graph_lanes": (_c_int, [_handle, ctypes.POINTER(_c_int)]),
     "fc3_autotune_query": (_c_int, [_handle, _c_int, _c_int, ctypes.POINTER(_c_int),
                                     ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_float)]),
     "fc3_autotune_save": (_c_int, [_handle, ctypes.c_char_p]),

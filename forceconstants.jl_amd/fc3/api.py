@@ -684,6 +684,14 @@ def batch_graph_nodes(ctx: Optional[Context] = None) -> int:
     return int(v.value)
 
 
+def batch_graph_lanes(ctx: Optional[Context] = None) -> int:
+    """Parallel branches of the last captured batch graph."""
+    ctx = ctx or Context.default()
+    v = ctypes.c_int()
+    check(ctx.lib.fc3_batch_graph_lanes(ctx.h, ctypes.byref(v)))
+    return int(v.value)
+
+
 def autotune_query(n: int, precision="bf16x3", ctx: Optional[Context] = None):
     """(variant, tiles, ms) chosen for size n: variant 2 = FP32-operand kernels, 3 =
     pre-split BF16x3 kernels."""

@@ -298,6 +298,10 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
                                   double r_cut, double tol, int precision, int flags);
 /* Nodes of the last captured batch graph. */
 int fc3_batch_graph_nodes(fc3_handle h, size_t* nodes);
+/* Lanes (parallel branches) of the last captured batch graph: snapshots whose K3
+ * outputs overlap share a lane and run in submission order; lanes run concurrently,
+ * each with its own Psi and mcc3 scratch, as many as free device memory allows. */
+int fc3_batch_graph_lanes(fc3_handle h, int* lanes);
 /* Tuning table: the chosen configuration of size n (variant 2 = dense Psi + FP32
  * kernels, 3 = dense Psi + pre-split BF16x3 kernels, 4 = the fused path; tiles =
  * workgroups per GEMM launch) and the time of one snapshot (assembly included);

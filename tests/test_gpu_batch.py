@@ -189,3 +189,47 @@ def test_batch_device_modes(ctx):
         k3.free()
     phi_single.free()
     assert w_single.shape == (81 * 4,)
+
+
+def test_batch_lanes_with_shared_outputs(ctx):
+    """Lanes: snapshots whose K3 outputs are the same buffer form a chain that one lane
+    runs in submission order, so each buffer must end with its last writer's K3, bit for
+    bit the K3 of that snapshot computed alone.  Eight 96-DoF snapshots rotate through
+    three buffers and four 324-DoF snapshots through two: five chains, five lanes (the
+    graph's parallel branches); the direct launches give the same bits."""
+    spec = [((2, 2, 2), 70 + j) for j in range(8)] + [((3, 3, 3), 80 + j) for j in range(4)]
+    snaps = _snapshots(ctx, spec)
+    bufs = {96: [fc3.DeviceArray((96,) * 3, np.float32, ctx) for _ in range(3)],
+            324: [fc3.DeviceArray((324,) * 3, np.float32, ctx) for _ in range(2)]}
+    args, last = [], {}
+    for j, (sysd, dphi, own, _) in enumerate(snaps):
+        n = sysd.n_dof
+        b = bufs[n][(j if n == 96 else j - 8) % len(bufs[n])]
+        args.append((sysd, dphi, b))
+        last[id(b)] = j
+    fc3.batch_third_order_mcc3(args, LJ, CALC, precision="bf16x3", ctx=ctx)
+    ctx.synchronize()
+    lanes = fc3.batch_graph_lanes(ctx)
+    print(f"shared outputs: {lanes} lanes, {fc3.batch_graph_nodes(ctx)} graph nodes", flush=True)
+    assert lanes == 5
+    got = {id(b): b.to_host() for bl in bufs.values() for b in bl}
+    for key, j in last.items():
+        sysd, dphi, own, phi32 = snaps[j]
+        fc3.batch_third_order_mcc3([(sysd, dphi, own)], LJ, CALC, precision="bf16x3", ctx=ctx)
+        ctx.synchronize()
+        alone = own.to_host()
+        print(f"buffer of snapshot {j} (n={sysd.n_dof}): equal to it alone "
+              f"{np.array_equal(got[key], alone)}", flush=True)
+        assert np.array_equal(got[key], alone)
+        assert _rel(alone, _ref(sysd, phi32)) <= TOL["bf16x3"]
+    fc3.batch_third_order_mcc3(args, LJ, CALC, precision="bf16x3", graph=False, ctx=ctx)
+    ctx.synchronize()
+    for bl in bufs.values():
+        for b in bl:
+            assert np.array_equal(got[id(b)], b.to_host())
+    for s in snaps:
+        s[1].free()
+        s[2].free()
+    for bl in bufs.values():
+        for b in bl:
+            b.free()

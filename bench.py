@@ -410,11 +410,12 @@ def modes_setup_timing(ctx, cells, calc):
 
 def adjoint_timing(ctx, cells=5, reps=2):
     """The differentiation configuration: LJ argon FCC 5x5x5 (1500 DoF), sigma_u = 0.05 A
-    (seeded).  Forward = assembly (mass-weighted FP32 Psi) + BF16x3 mcc3; backward =
-    mcc3 pullback (dL/dPsi and dL/dphi) + the sigma/epsilon pullback of the assembly.
-    Timed with the cotangent dL/dK3 = K3 (L = 1/2 sum K3^2: the timing does not depend on
-    the weights).  Gradients are checked in tests at sizes where FP64 differences are
-    affordable; here dL/depsilon is also checked against <dL/dPsi, Psi> / epsilon, exact
+    (seeded), L = sum w K3^2 with w ~ U(0, 1) from Philox4x32-10 (seed 1500), generated on
+    the device and regenerable on any host.  Forward = assembly (mass-weighted FP32 Psi)
+    + BF16x3 mcc3 + L; backward = the cotangent 2 w K3, the mcc3 pullback (dL/dPsi and
+    dL/dphi) and the sigma/epsilon pullback of the assembly.  Gradients are checked in
+    tests against FP64 central differences and the dual-number oracle at sizes where those
+    are affordable; here dL/depsilon is checked against <dL/dPsi, Psi> / epsilon, exact
     because Psi is linear in epsilon."""
     import fc3
     from fc3 import systems as S
@@ -426,16 +427,21 @@ def adjoint_timing(ctx, cells=5, reps=2):
     psi = fc3.DeviceArray((n, n, n), np.float32, ctx)
     k3 = fc3.DeviceArray((n, n, n), np.float32, ctx)
 
+    seed = 1500
+
     def forward():
         fc3.third_order_device(sysd, S.ARGON_LJ, calc, psi32=psi, masses=sysd.masses, ctx=ctx)
         fc3.mcc3(psi, dphi, precision="bf16x3", out=k3)
+        return fc3.weighted_square_loss(k3, seed, grad=False)[0]
 
     def backward():
-        dpsi, dph = fc3.mcc3_backward(psi, dphi, k3, precision="bf16x3")
+        _, G = fc3.weighted_square_loss(k3, seed)
+        dpsi, dph = fc3.mcc3_backward(psi, dphi, G, precision="bf16x3")
+        G.free()
         g = fc3.third_order_param_backward(sysd, S.ARGON_LJ, calc, dpsi, masses=sysd.masses)
         return dpsi, dph, g
 
-    forward()
+    loss = forward()
     dpsi, dph, g = backward()
     ctx.synchronize()
     # exact check: dL/depsilon = <dL/dPsi, Psi> / epsilon
@@ -467,7 +473,9 @@ def adjoint_timing(ctx, cells=5, reps=2):
     t_fb = ctx.timer_stop() / reps
     for d in (psi, k3, dphi):
         d.free()
-    res = {"config": f"LJ argon FCC {cells}x{cells}x{cells} ({n} DoF)", "forward_ms": t_fwd,
+    res = {"config": f"LJ argon FCC {cells}x{cells}x{cells} ({n} DoF)",
+           "loss": "L = sum w K3^2, w ~ U(0,1) Philox4x32-10 seed 1500 (device)", "L": loss,
+           "forward_ms": t_fwd,
            "forward_backward_ms": t_fb, "ratio": t_fb / t_fwd,
            "dL_dsigma": g["sigma"], "dL_depsilon": g["epsilon"],
            "dL_depsilon_vs_exact_rel": abs(g["epsilon"] - eps_check) / abs(eps_check),

@@ -159,6 +159,86 @@ k_lj_param_grad(const double* __restrict__ pos, double Lx, double Ly, double Lz,
     }
 }
 
+// ---- weighted square loss L = sum_e w_e K3_e^2 (the differentiation workload) -------
+// w_e ~ U[0, 1) from Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC11): element e takes
+// word e % 4 of the block for counter (e / 4 low word, e / 4 high word, 0, 0) under key
+// (seed low word, seed high word); w_e = (word >> 8) 2^-24, exact in FP32.  Every term
+// w K3^2 is formed in FP64; thread sums run over a fixed grid stride and the block sums
+// combine in a fixed tree, so L is the same on every run.  The cotangent 2 w K3 is one
+// FP32 rounding of the exact (2 w) K3.
+__device__ __forceinline__ void philox4x32_10(unsigned c[4], unsigned k0, unsigned k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;   // Weyl key increments
+            k1 += 0xBB67AE85u;
+        }
+        const unsigned long long p0 = 0xD2511F53ull * c[0];   // round multipliers
+        const unsigned long long p1 = 0xCD9E8D57ull * c[2];
+        const unsigned n0 = (unsigned)(p1 >> 32) ^ c[1] ^ k0;
+        const unsigned n2 = (unsigned)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = (unsigned)p1;
+        c[2] = n2;
+        c[3] = (unsigned)p0;
+    }
+}
+
+constexpr int kLossBlocks = 2048;
+
+__global__ void __launch_bounds__(256)
+k_weighted_square_loss(const float* __restrict__ k3, long count, unsigned long long seed,
+                       float* __restrict__ dk3, double* __restrict__ partial)
+{
+    double acc = 0.0;
+    const long nq = (count + 3) / 4;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+         q += (long)gridDim.x * blockDim.x) {
+        unsigned c[4] = {(unsigned)q, (unsigned)((unsigned long long)q >> 32), 0u, 0u};
+        philox4x32_10(c, (unsigned)seed, (unsigned)(seed >> 32));
+        const long e0 = 4 * q;
+        float x[4];
+        if (e0 + 3 < count) {
+            const float4 v = reinterpret_cast<const float4*>(k3)[q];   // k3 16-byte aligned
+            x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+        } else {
+            for (int j = 0; j < 4; ++j) x[j] = e0 + j < count ? k3[e0 + j] : 0.0f;
+        }
+        float g[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float w = (float)(c[j] >> 8) * 5.9604644775390625e-8f;   // 2^-24
+            if (e0 + j < count) acc += (double)w * (double)x[j] * (double)x[j];
+            g[j] = (2.0f * w) * x[j];
+        }
+        if (dk3) {
+            if (e0 + 3 < count) {
+                reinterpret_cast<float4*>(dk3)[q] = make_float4(g[0], g[1], g[2], g[3]);
+            } else {
+                for (int j = 0; j < 4; ++j)
+                    if (e0 + j < count) dk3[e0 + j] = g[j];
+            }
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_in_order(const double* __restrict__ partial, int cnt, double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0;
+    for (int i = 0; i < cnt; ++i) a += partial[i];
+    out[0] = a;
+}
+
 __global__ void k_sum_partials(const double* __restrict__ partial, int cnt, double* __restrict__ out)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -208,6 +288,24 @@ int blas_handle(fc3_context* h, void** out)
     rocblas_handle b = nullptr;
     FC3_TRY(get_blas(h, &b));
     *out = b;
+    return FC3_OK;
+}
+
+size_t weighted_square_loss_bytes() { return Carve::need(sizeof(double) * (kLossBlocks + 1)); }
+
+int weighted_square_loss(hipStream_t s, const float* k3, long count, unsigned long long seed,
+                         float* dk3, void* ws, double* d_loss)
+{
+    Carve cv(ws, weighted_square_loss_bytes());
+    double* partial = cv.take<double>(kLossBlocks + 1);
+    const long nq = (count + 3) / 4;
+    const long blocks_needed = (nq + 255) / 256;
+    const int grid = (int)(blocks_needed < kLossBlocks ? (blocks_needed > 0 ? blocks_needed : 1)
+                                                       : kLossBlocks);
+    hipLaunchKernelGGL(k_weighted_square_loss, dim3(grid), dim3(256), 0, s, k3, count, seed, dk3,
+                       partial);
+    hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, s, partial, grid, d_loss);
+    FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
 

@@ -706,6 +706,33 @@ int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const fl
     return mcc3_backward(h, psi, phi, dk3, n, precision, dpsi, dphi);
 }
 
+int fc3_weighted_square_loss(fc3_handle h, const float* k3, long long count,
+                             unsigned long long seed, float* dk3, double* loss)
+{
+    FC3_TRY(check_handle(h));
+    if (!k3 || count < 0) return set_error(FC3_ERR_ARGUMENT, "k3 must be non-null, count >= 0");
+    if (!dk3 && !loss) return set_error(FC3_ERR_ARGUMENT, "dk3 and loss are both null");
+    if (reinterpret_cast<uintptr_t>(k3) % 16 != 0 || reinterpret_cast<uintptr_t>(dk3) % 16 != 0)
+        return set_error(FC3_ERR_ARGUMENT, "k3 and dk3 must be 16-byte aligned");
+    if (dk3 && (const void*)dk3 != (const void*)k3) {
+        const uintptr_t a = reinterpret_cast<uintptr_t>(k3), b = reinterpret_cast<uintptr_t>(dk3);
+        const uintptr_t bytes = sizeof(float) * (uintptr_t)count;
+        if (a < b + bytes && b < a + bytes)
+            return set_error(FC3_ERR_ARGUMENT, "dk3 must be k3 itself or not overlap it");
+    }
+    FC3_TRY(ensure_workspace(h, weighted_square_loss_bytes() + Carve::need(sizeof(double))));
+    Carve cv(h->ws, h->ws_bytes);
+    double* d_loss = cv.take<double>(1);
+    char* rest = cv.take<char>(weighted_square_loss_bytes());
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "loss workspace");
+    FC3_TRY(weighted_square_loss(h->stream, k3, (long)count, seed, dk3, rest, d_loss));
+    if (loss) {
+        FC3_HIP(hipMemcpyAsync(loss, d_loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        FC3_HIP(hipStreamSynchronize(h->stream));
+    }
+    return FC3_OK;
+}
+
 int fc3_third_order_lj_param_backward(fc3_handle h, const double* positions, int n_atoms,
                                       const double box[3], double sigma, double epsilon,
                                       double pot_r_cut, double r_cut, double tol,

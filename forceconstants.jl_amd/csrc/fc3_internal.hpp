@@ -262,6 +262,11 @@ void release_blas(fc3_context* h);
 int blas_handle(fc3_context* h, void** out);
 void release_batch(fc3_context* h);
 size_t mcc3_backward_workspace_bytes(int n, int precision, bool need_dphi);
+// L = sum_e w_e k3[e]^2 into d_loss (device) and, if dk3, the cotangent 2 w_e k3[e];
+// w from Philox4x32-10 keyed by seed (adjoint.hip); k3 and dk3 16-byte aligned.
+size_t weighted_square_loss_bytes();
+int weighted_square_loss(hipStream_t s, const float* k3, long count, unsigned long long seed,
+                         float* dk3, void* ws, double* d_loss);
 // dpsi (n^3) and/or dphi (n x n), device FP32; uses h's workspace (sized by the above).
 int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const float* G, int n,
                   int precision, float* dpsi, float* dphi);

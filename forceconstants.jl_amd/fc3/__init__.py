@@ -14,7 +14,8 @@ from .api import (asr_satisfied, autotune_load, autotune_query, autotune_save, b
                   unique_index,
                   mcc3_, mcc3_backward, mcc3_rrule, modes_device, save_second_order,
                   save_third_order, second_order, second_order_device, third_order,
-                  third_order_device, third_order_mcc3, third_order_param_backward)
+                  third_order_device, third_order_mcc3, third_order_param_backward,
+                  weighted_square_loss)
 from .systems import (AnalyticalCalculator, AutoDiffCalculator, LJ, StillingerWeberSilicon,
                       SuperCellSystem, SWParams, diamond, fcc)
 
@@ -27,6 +28,7 @@ __all__ = [
     "mcc3_", "mcc3_backward", "mcc3_rrule", "modes_device", "save_second_order",
     "save_third_order", "second_order", "second_order_device", "third_order",
     "third_order_device", "third_order_mcc3", "third_order_param_backward", "systems",
+    "weighted_square_loss",
     "AnalyticalCalculator", "AutoDiffCalculator", "LJ", "StillingerWeberSilicon",
     "SuperCellSystem", "SWParams", "diamond", "fcc",
 ]

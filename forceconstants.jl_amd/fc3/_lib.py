@@ -99,6 +99,8 @@ SIGNATURES = {
                                                _c_int]),
     "fc3_batch_graph_nodes": (_c_int, [_handle, ctypes.POINTER(_c_size)]),
     "fc3_batch_graph_lanes": (_c_int, [_handle, ctypes.POINTER(_c_int)]),
+    "fc3_weighted_square_loss": (_c_int, [_handle, _vp, ctypes.c_longlong, ctypes.c_ulonglong,
+                                          _vp, ctypes.POINTER(_c_double)]),
     "fc3_autotune_query": (_c_int, [_handle, _c_int, _c_int, ctypes.POINTER(_c_int),
                                     ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_float)]),
     "fc3_autotune_save": (_c_int, [_handle, ctypes.c_char_p]),

@@ -580,6 +580,23 @@ def mcc3_backward(psi: DeviceArray, phi: DeviceArray, dk3: DeviceArray, precisio
     return gp, gf
 
 
+def weighted_square_loss(k3: DeviceArray, seed: int, grad: bool = True):
+    """L = sum_e w_e K3_e^2 with w_e ~ U[0, 1) from Philox4x32-10 keyed by `seed` (the
+    differentiation workload's scalar; the weights are regenerated, never stored).
+    Returns (L, dL/dK3 as a device array, or None without grad)."""
+    ctx = k3.ctx
+    if not isinstance(k3, DeviceArray) or k3.dtype != np.float32:
+        raise TypeError("k3 must be a float32 DeviceArray")
+    if not 0 <= int(seed) < 2 ** 64:
+        raise ValueError("seed must be in [0, 2^64)")
+    g = DeviceArray(k3.shape, np.float32, ctx) if grad else None
+    L = ctypes.c_double()
+    count = int(np.prod(k3.shape, dtype=np.int64))
+    check(ctx.lib.fc3_weighted_square_loss(ctx.h, k3.ptr, count, int(seed),
+                                           g.ptr if g else None, ctypes.byref(L)))
+    return float(L.value), g
+
+
 def mcc3_rrule(psi: DeviceArray, phi: DeviceArray, precision="fp32"):
     """(K3, pullback) as a ChainRulesCore rrule returns them; pullback(dk3) ->
     (dL/dPsi, dL/dphi)."""

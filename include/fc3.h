@@ -324,6 +324,17 @@ int fc3_autotune_load(fc3_handle h, const char* path);
 int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const float* dk3,
                       int n, int precision, float* dpsi, float* dphi);
 
+/* The scalar of the differentiation workload (no counterpart in the reference, which has
+ * no adjoints): L = sum_e w_e k3[e]^2 over `count` device FP32 values, with weights
+ * w_e ~ U[0, 1) from Philox4x32-10 keyed by `seed` (element e: word e % 4 of the block
+ * for counter (e / 4 low, e / 4 high, 0, 0), key (seed low, seed high);
+ * w_e = (word >> 8) 2^-24), so that any host can regenerate them.
+ *   loss: L on the host (FP64 terms, a fixed summation order), or NULL;
+ *   dk3:  the cotangent dL/dk3 = 2 w k3 (device FP32; may be k3 itself), or NULL.
+ * k3 and dk3 are 16-byte aligned.                                                     */
+int fc3_weighted_square_loss(fc3_handle h, const float* k3, long long count,
+                             unsigned long long seed, float* dk3, double* loss);
+
 /* Reverse mode of third_order(sys, LJ(sigma, epsilon, pot_r_cut), AnalyticalCalculator)
  * (src/ifc.jl:20-35 -> third_order_analytical.jl:4-69, asr.jl:19-40,
  * helper_funcs.jl:20-28) with respect to the potential parameters:

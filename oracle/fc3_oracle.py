@@ -401,3 +401,58 @@ def save_txt_lines(values: np.ndarray, N_atoms: int, D: int):
             cols += [str(int(atoms[q][t]) + 1), str(int(comps[q][t]) + 1)]
         lines.append(" ".join(cols) + " " + julia_float_str(float(v[tuple(ix[t] for ix in idx)])))
     return lines
+
+
+# ---- weighted square loss of the differentiation workload --------------------------
+# L = sum_e w_e K3_e^2 with w_e ~ U[0, 1) from a counter-based generator, so that device
+# and host produce the same weights without storing n^3 of them.  The generator is
+# Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2,
+# 3", SC11): 10 rounds of two 32x32->64-bit multiplies with the key bumped by the Weyl
+# constants between rounds.  Element e takes word e % 4 of the block for counter
+# (e / 4 low word, e / 4 high word, 0, 0) under key (seed low word, seed high word), and
+# w_e = (word >> 8) 2^-24.  The product path is fc3_weighted_square_loss (adjoint.hip).
+_PHILOX_M0, _PHILOX_M1 = 0xD2511F53, 0xCD9E8D57
+_PHILOX_W0, _PHILOX_W1 = 0x9E3779B9, 0xBB67AE85
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 on arrays: ctr (..., 4) and key (..., 2) of uint32 -> (..., 4) uint32."""
+    c = [np.asarray(ctr[..., i], dtype=np.uint64) for i in range(4)]
+    k0 = np.asarray(key[..., 0], dtype=np.uint64)
+    k1 = np.asarray(key[..., 1], dtype=np.uint64)
+    mask = np.uint64(0xFFFFFFFF)
+    for r in range(10):
+        if r > 0:
+            k0 = (k0 + np.uint64(_PHILOX_W0)) & mask
+            k1 = (k1 + np.uint64(_PHILOX_W1)) & mask
+        p0 = np.uint64(_PHILOX_M0) * c[0]
+        p1 = np.uint64(_PHILOX_M1) * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+    return np.stack(c, axis=-1).astype(np.uint32)
+
+
+def philox_uniform(seed: int, count: int, start: int = 0) -> np.ndarray:
+    """w_e for e in [start, start + count) as float32 in [0, 1) (see above)."""
+    e = np.arange(start, start + count, dtype=np.uint64)
+    q = e >> np.uint64(2)
+    ctr = np.zeros((count, 4), dtype=np.uint32)
+    ctr[:, 0] = (q & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    ctr[:, 1] = (q >> np.uint64(32)).astype(np.uint32)
+    key = np.empty((count, 2), dtype=np.uint32)
+    key[:, 0] = np.uint32(seed & 0xFFFFFFFF)
+    key[:, 1] = np.uint32((seed >> 32) & 0xFFFFFFFF)
+    words = philox4x32_10(ctr, key)
+    w = words[np.arange(count), (e & np.uint64(3)).astype(np.int64)]
+    return ((w >> np.uint32(8)).astype(np.float64) * 2.0 ** -24).astype(np.float32)
+
+
+def weighted_square_loss(k3: np.ndarray, seed: int):
+    """(L, dL/dK3) for L = sum_e w_e K3_e^2 over K3 in column-major order (FP64 sum of
+    w K3^2 in FP64; the cotangent 2 w K3 rounded to FP32 as the device forms it)."""
+    flat = np.asarray(k3, dtype=np.float32).ravel(order="F")
+    w = philox_uniform(seed, flat.size)
+    L = float(np.sum(w.astype(np.float64) * flat.astype(np.float64) * flat.astype(np.float64)))
+    g = (np.float32(2.0) * w) * flat
+    return L, g.reshape(k3.shape, order="F")

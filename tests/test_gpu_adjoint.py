@@ -182,6 +182,64 @@ def test_lj_chain_sum_w_k3_squared(ctx):
         assert abs(fd - g[name]) <= 2e-4 * abs(fd)
 
 
+@pytest.mark.parametrize("shape,seed", [((96, 96, 96), 1500), ((7, 5, 3), (1 << 40) + 9)])
+def test_weighted_square_loss_matches_oracle(ctx, shape, seed):
+    """fc3_weighted_square_loss against the oracle's Philox restatement: the cotangent
+    2 w K3 bit for bit, L to FP64 summation order (1e-13); a count that is not a
+    multiple of 4 takes the scalar tail."""
+    k3 = np.asfortranarray(np.random.default_rng(5).standard_normal(shape).astype(np.float32))
+    d = fc3.DeviceArray.from_host(k3, ctx=ctx)
+    L, g = fc3.weighted_square_loss(d, seed)
+    L0, g0 = O.weighted_square_loss(k3, seed)
+    print(f"loss {shape}: device {L:.15e} oracle {L0:.15e}", flush=True)
+    assert abs(L - L0) <= 1e-13 * abs(L0)
+    assert np.array_equal(g.to_host(), g0)
+    L1, none = fc3.weighted_square_loss(d, seed, grad=False)
+    assert none is None and L1 == L
+    g.free()
+    d.free()
+
+
+def test_lj_chain_device_weights_central_difference(ctx):
+    """The differentiation workload's chain with the weights on the device (L = sum w K3^2,
+    w from Philox keyed by the seed): forward, loss cotangent, mcc3 backward and the
+    sigma / epsilon pullback, against FP64 central differences of the oracle chain with
+    the same regenerated weights (FCC 2x2x2, 96 DoF)."""
+    sysd = _lj_sys((2, 2, 2), 0.05, 43)
+    calc = S.AnalyticalCalculator(1e-12, LJ.r_cut)
+    n = sysd.n_dof
+    seed = 20260
+    f2 = O.lj_second_order(sysd, LJ, calc)
+    _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, LJ, calc, psi32=psi32, masses=sysd.masses, ctx=ctx)
+    dphi = fc3.DeviceArray.from_host(phi32, ctx=ctx)
+    k3 = fc3.mcc3(psi32, dphi)
+    _, G = fc3.weighted_square_loss(k3, seed)
+    dpsi, _ = fc3.mcc3_backward(psi32, dphi, G, dphi=False)
+    g = fc3.third_order_param_backward(sysd, LJ, calc, dpsi, masses=sysd.masses)
+    w = O.philox_uniform(seed, n ** 3).reshape((n, n, n), order="F").astype(np.float64)
+
+    def L(pot):
+        psi = O.lj_third_order(sysd, pot, calc)
+        O.mass_weight_third_order(psi, sysd.masses)
+        K = O.mcc3_fp64(psi, phi32)
+        return float(np.sum(w * K * K))
+
+    for name, plus, minus, h in (
+            ("sigma", S.LJ(LJ.sigma * (1 + 1e-6), LJ.epsilon, LJ.r_cut),
+             S.LJ(LJ.sigma * (1 - 1e-6), LJ.epsilon, LJ.r_cut), LJ.sigma * 1e-6),
+            ("epsilon", S.LJ(LJ.sigma, LJ.epsilon * (1 + 1e-6), LJ.r_cut),
+             S.LJ(LJ.sigma, LJ.epsilon * (1 - 1e-6), LJ.r_cut), LJ.epsilon * 1e-6)):
+        fd = (L(plus) - L(minus)) / (2 * h)
+        print(f"device-weight chain dL/d{name}: adjoint {g[name]:.9e} FD {fd:.9e} "
+              f"rel {abs(fd - g[name]) / abs(fd):.2e}", flush=True)
+        assert abs(fd - g[name]) <= 2e-4 * abs(fd)
+    for d in (psi32, dphi, k3, G, dpsi):
+        d.free()
+
+
 SW_POT = S.StillingerWeberSilicon()
 
 

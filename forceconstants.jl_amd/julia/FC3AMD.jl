@@ -34,7 +34,7 @@ import ForceConstants
 
 export DeviceArray, second_order, third_order, mass_weight_third_order!, mcc3, mcc3!,
        mcc3_unique, asr_satisfied, save_second_order, save_third_order, set_precision!,
-       get_modes_device, FP32, BF16X3
+       get_modes_device, weighted_square_loss, FP32, BF16X3
 
 const libfc3 = get(ENV, "FC3_LIBRARY", joinpath(@__DIR__, "..", "fc3", "libfc3.so"))
 
@@ -310,6 +310,23 @@ function mcc3_unique(Ψ::DeviceArray{Float32, 3}, phi::DeviceArray{Float32, 2})
                 (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Cint, Cint, Ptr{Float32}),
                 Ψ.h.ptr, Ψ.ptr, phi.ptr, n, PRECISION[], K3u.ptr))
     return K3u
+end
+
+"""
+    weighted_square_loss(K3, seed; grad = true) -> (L, dK3 or nothing)
+
+L = sum w .* K3.^2 with w ~ U[0, 1) from Philox4x32-10 keyed by `seed`, generated on the
+device and reproducible on any host (the differentiation workload's scalar; no
+counterpart in the reference). `dK3 = 2 w .* K3` is the cotangent for the mcc3 pullback.
+"""
+function weighted_square_loss(K3::DeviceArray{Float32}, seed::Integer; grad::Bool = true)
+    dK3 = grad ? DeviceArray{Float32}(K3.h, size(K3)...) : nothing
+    L = Ref{Float64}(0.0)
+    check(ccall((:fc3_weighted_square_loss, libfc3), Cint,
+                (Ptr{Cvoid}, Ptr{Float32}, Clonglong, Culonglong, Ptr{Float32}, Ref{Float64}),
+                K3.h.ptr, K3.ptr, prod(size(K3)), UInt64(seed),
+                grad ? dK3.ptr : Ptr{Float32}(C_NULL), L))
+    return L[], dK3
 end
 
 # get_modes(dynamical_matrix(sys, pot, calc)) (src/dynmat.jl:5-33,123-131) on the device,

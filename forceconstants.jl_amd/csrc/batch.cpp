@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -645,7 +646,13 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
     FC3_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t reusable = B->ws_bytes - regions;   // bytes past the snapshot regions
     const size_t reserve = size_t(2) << 30;
-    const size_t budget = free_b + reusable > reserve ? free_b + reusable - reserve : 0;
+    size_t budget = free_b + reusable > reserve ? free_b + reusable - reserve : 0;
+    // FC3_BATCH_LANE_BYTES caps the lanes' bytes (to leave memory to the caller, or to
+    // force fewer lanes)
+    if (const char* e = std::getenv("FC3_BATCH_LANE_BYTES")) {
+        const double cap = std::atof(e);
+        if (cap >= 0.0 && cap < (double)budget) budget = (size_t)cap;
+    }
     const LanePlan L = plan_lanes(P, snaps, count, precision, budget);
     FC3_TRY(grow_batch_ws(B, regions + L.bytes, st, &moved));
     base = static_cast<char*>(B->ws);

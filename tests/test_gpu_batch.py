@@ -233,3 +233,34 @@ def test_batch_lanes_with_shared_outputs(ctx):
     for bl in bufs.values():
         for b in bl:
             b.free()
+
+
+def test_batch_lane_budget_shrinks_lanes(ctx, monkeypatch):
+    """The lane planner under a memory cap (FC3_BATCH_LANE_BYTES): with room for every
+    lane the batch of four distinct-output snapshots per size runs in 4 + 4 lanes; with
+    a cap of 0 bytes the lanes shrink to one per size class and then to a single lane.
+    The K3 of every plan is the same bit for bit (each snapshot's kernels do not depend
+    on the lane it runs in)."""
+    spec = [((2, 2, 2), 90 + j) for j in range(4)] + [((3, 3, 3), 95 + j) for j in range(4)]
+    snaps = _snapshots(ctx, spec)
+    args = [s[:3] for s in snaps]
+    results = {}
+    for cap in (None, "0"):
+        if cap is None:
+            monkeypatch.delenv("FC3_BATCH_LANE_BYTES", raising=False)
+        else:
+            monkeypatch.setenv("FC3_BATCH_LANE_BYTES", cap)
+        fc3.batch_third_order_mcc3(args, LJ, CALC, precision="bf16x3", ctx=ctx)
+        ctx.synchronize()
+        lanes = fc3.batch_graph_lanes(ctx)
+        print(f"lane cap {cap}: {lanes} lanes", flush=True)
+        results[cap] = (lanes, [s[2].to_host() for s in snaps])
+    monkeypatch.delenv("FC3_BATCH_LANE_BYTES", raising=False)
+    assert results[None][0] == 8 and results["0"][0] == 1
+    for a, b in zip(results[None][1], results["0"][1]):
+        assert np.array_equal(a, b)
+    for (sysd, _, k3, phi32), got in zip(snaps, results[None][1]):
+        assert _rel(got, _ref(sysd, phi32)) <= TOL["bf16x3"]
+    for s in snaps:
+        s[1].free()
+        s[2].free()

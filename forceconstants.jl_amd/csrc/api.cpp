@@ -36,6 +36,7 @@ int ensure_workspace(fc3_context* h, size_t bytes)
     hipError_t e = hipMalloc(&h->ws, want);
     if (e != hipSuccess) {
         h->ws = nullptr;
+        (void)hipGetLastError();   // a caller may retry smaller: leave no error behind
         return set_error(FC3_ERR_OUT_OF_MEMORY,
                          "workspace allocation of " + std::to_string(want) + " bytes failed: " +
                              hipGetErrorString(e));
@@ -389,12 +390,20 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
     auto u_bytes = [&](int cap) {
         return Carve::need(sizeof(uint16_t) * 2 * 3 * (size_t)n * mcc3_gather_k(cap) * N);
     };
-    Prepared P;
-    FC3_TRY(prepare(h, positions, N, box, r_cut, masses, [&](int cap) {
+    // with room for the FP32 intermediate (mode 3 without a split pass) when the device
+    // has it, else with the bounded split scratch (the same K3)
+    auto need = [&](int cap, bool direct) {
         return Carve::need(sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * (27 + 9))) +
                Carve::need(sizeof(float) * (size_t)n * n) + u_bytes(cap) +
-               mcc3_from_gathered_bytes((int)n, cap) + 4096;
-    }, P));
+               mcc3_from_gathered_bytes((int)n, cap, direct) + 4096;
+    };
+    Prepared P;
+    int rc = prepare(h, positions, N, box, r_cut, masses,
+                     [&](int cap) { return need(cap, true); }, P);
+    if (rc == FC3_ERR_OUT_OF_MEMORY)
+        rc = prepare(h, positions, N, box, r_cut, masses,
+                     [&](int cap) { return need(cap, false); }, P);
+    FC3_TRY(rc);
     if (lj_mode1_sparse_lds(P.nl.cap) > 64 * 1024)
         return set_error(FC3_ERR_UNSUPPORTED, "fused mode 1: too many neighbours per atom");
     const int Kc = mcc3_gather_k(P.nl.cap);

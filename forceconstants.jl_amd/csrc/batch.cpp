@@ -162,7 +162,7 @@ size_t fused_bytes(long n, int cap)
 {
     return align256(sizeof(float) * (size_t)n * n) +
            align256(sizeof(uint16_t) * 2 * 3 * (size_t)n * mcc3_gather_k(cap) * (n / 3)) +
-           mcc3_from_gathered_bytes((int)n, cap);
+           mcc3_from_gathered_bytes((int)n, cap, true);
 }
 
 // Kernels of one snapshot on stream s (no synchronisation, no allocation).

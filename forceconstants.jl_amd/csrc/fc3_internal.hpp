@@ -219,7 +219,9 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
 // columns (gathered phi rows), mode 3 in place; K3 FP32 in k3 (n % 4 == 0, n >= 16).
 // phiT: phi transposed (phiT[q + n i] = phi[i + n q]), for the row gather.
 int mcc3_gather_k(int cap);
-size_t mcc3_from_gathered_bytes(int n, int cap);
+// direct: room for the FP32 intermediate that mode 3 reads without a split pass (n^3
+// floats more); mcc3_from_gathered takes that path when ws_bytes allows it.
+size_t mcc3_from_gathered_bytes(int n, int cap, bool direct);
 int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, const float* phiT, int n,
                        const int* count, const int* nbr, int cap, float* k3, void* ws,
                        size_t ws_bytes);

@@ -745,6 +745,212 @@ __device__ __forceinline__ void gemm3_tile(const G3& g, const int m0, const int 
         }
 }
 
+// One tile with an FP32 A operand split in registers (k_gemm3_af32):
+// A(m, k) = Af[m + k a_KS], FP32 and column-major, so rows are contiguous for each k.
+// The rows k < K must all exist and be finite, since the B rows past the contraction
+// length are zero.
+// Copies: a K block's 16 k-rows of the 256-row tile are 16 contiguous KB runs. Wave 0
+// copies k-rows 0-7 and wave 1 k-rows 8-15, into the bytes that gemm3_tile's bf16 hi and
+// lo planes take: k-row k at float 256 k of the stage. B is copied as in gemm3_tile.
+// A fragments: each lane's fragment of block i is row wm0 + 32 i + lane % 32, k-rows
+// 8 (lane / 32) .. + 7.
+//   - It is read as 8 floats during the step before the one that uses it.
+//   - It is split into bf16 hi / lo (split_bf16, as the split-transpose kernels write
+//     them) at the top of its own step, after the barrier that retired the reads.
+// The MFMAs therefore see the operands of the pre-split path, and C is the same bit for
+// bit.
+template <int SHAPE>
+__device__ __forceinline__ void gemm3_tile_af32(const G3& g, const int m0, const int n0, __bf16* s0,
+                                                __bf16* s1, __bf16* s2, __bf16* s3)
+{
+    using SH = Shape<SHAPE>;
+    constexpr int TBM = SH::WM * SH::TM * 32, TBN = SH::WN * SH::TN * 32;
+    static_assert(TBM == BM, "A tile is 256 rows");
+    constexpr int AE = TBM * BK, BE = TBN * BK;        // bf16 elements per plane
+    constexpr int CA = TBM / 32, CB = TBN / 32;        // copies per wave: A k-rows, B chunks
+    static_assert(CA == 8, "8 k-rows per A wave");
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm0 = (wave % SH::WM) * (SH::TM * 32), wn0 = (wave / SH::WM) * (SH::TN * 32);
+    constexpr int CMAX = CA > CB ? CA : CB;
+    const bool is_a = __builtin_amdgcn_readfirstlane(wave) < 2;
+    const int ncopy = is_a ? CA : CB;
+    const __bf16* src[CMAX];
+#pragma unroll
+    for (int q = 0; q < CMAX; ++q) {
+        if (is_a) {
+            int m = m0 + 4 * lane;
+            if (m > g.M - 4) m = g.M - 4;   // rows past M feed only discarded rows of C
+            src[q] = reinterpret_cast<const __bf16*>(reinterpret_cast<const float*>(g.A) + m +
+                                                     (long)(8 * wave + q) * g.a_KS);
+        } else {
+            const int P = 64 * q + lane;
+            const int r = 16 * (P >> 5) + (P & 15);
+            const int c = (P >> 4) & 1;
+            src[q] = g.B + (wave == 3 ? g.b_lo : 0) + 16L * (n0 + (q < CB ? r : 0)) + 8 * c;
+        }
+    }
+    // bf16 elements per K block: A 16 k-rows of floats, B as in gemm3_tile
+    const long kstride = is_a ? 32L * g.a_KS : g.b_KS;
+    const int nk = g.K / BK;
+    const int wofs = wave == 0 ? 0 : wave == 1 ? AE : wave == 2 ? 2 * AE : 2 * AE + BE;
+    auto issue = [&](__bf16* st, int k) {
+        const long kk = (long)(k < nk ? k : nk - 1) * kstride;
+#pragma unroll
+        for (int q = 0; q < CMAX; ++q)
+            if (q < ncopy) lds_copy16(src[q] + kk, st + wofs + 512 * q);
+    };
+
+    f32x16 acc[SH::TM][SH::TN];
+#pragma unroll
+    for (int i = 0; i < SH::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < SH::TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    int offb[SH::TN];
+#pragma unroll
+    for (int j = 0; j < SH::TN; ++j) offb[j] = 8 * lds_chunk(wn0 + j * 32 + (lane & 31), lane >> 5);
+    // float offset of this lane's A fragment row in a stage (k-row 8 (lane / 32) + u at
+    // + 256 u)
+    const int offa = 256 * 8 * (lane >> 5) + wm0 + (lane & 31);
+
+    float xa[SH::TM][8];   // raw A fragments of the next stage
+    bf16x8 ah[SH::TM], al[SH::TM];
+    bf16x8 bh0[SH::TN], bl0[SH::TN], bh1[SH::TN], bl1[SH::TN];
+#define FC3_MMA3(X, Y)                                                                      \
+    _Pragma("unroll") for (int i = 0; i < SH::TM; ++i)                                      \
+        _Pragma("unroll") for (int j = 0; j < SH::TN; ++j)                                  \
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[i], Y[j], acc[i][j], 0, 0, 0);
+#define FC3_RING_BARRIER(D)                                                                 \
+    {                                                                                       \
+        if constexpr (CA == CB) ring_barrier<(D) * CA>();                                   \
+        else if (is_a) ring_barrier<(D) * CA>();                                            \
+        else ring_barrier<(D) * CB>();                                                      \
+    }
+#define FC3_READ_NEXT(ST, BH, BL)                                                           \
+    {                                                                                       \
+        const float* sf_ = reinterpret_cast<const float*>(ST) + offa;                       \
+        _Pragma("unroll") for (int i = 0; i < SH::TM; ++i)                                  \
+            _Pragma("unroll") for (int u = 0; u < 8; ++u) xa[i][u] = sf_[256 * u + 32 * i]; \
+        _Pragma("unroll") for (int j = 0; j < SH::TN; ++j) {                                \
+            BH[j] = *reinterpret_cast<const bf16x8*>(ST + 2 * AE + offb[j]);                \
+            BL[j] = *reinterpret_cast<const bf16x8*>(ST + 2 * AE + BE + offb[j]);           \
+        }                                                                                   \
+    }
+    // split_bf16 on pairs (one packed conversion per 2 values; round to nearest even as
+    // the scalar conversion)
+#define FC3_SPLIT()                                                                         \
+    {                                                                                       \
+        typedef float f2v_ __attribute__((ext_vector_type(2)));                             \
+        typedef __bf16 b2v_ __attribute__((ext_vector_type(2)));                            \
+        _Pragma("unroll") for (int i = 0; i < SH::TM; ++i)                                  \
+            _Pragma("unroll") for (int u = 0; u < 8; u += 2) {                              \
+                const f2v_ x_ = {xa[i][u], xa[i][u + 1]};                                   \
+                const b2v_ h_ = __builtin_convertvector(x_, b2v_);                          \
+                const f2v_ hf_ = __builtin_convertvector(h_, f2v_);                         \
+                const f2v_ d_ = {x_[0] - hf_[0], x_[1] - hf_[1]};                           \
+                const b2v_ l_ = __builtin_convertvector(d_, b2v_);                          \
+                ah[i][u] = h_[0];                                                           \
+                ah[i][u + 1] = h_[1];                                                       \
+                al[i][u] = l_[0];                                                           \
+                al[i][u + 1] = l_[1];                                                       \
+            }                                                                               \
+    }
+    // Scheduling hint for one step: block 0's split first, then each block's first four
+    // MFMAs (lo x B hi) with the next block's split, the reads of the next stage two per
+    // MFMA, the copies one per two MFMAs, then the rest.
+#define FC3_ORDER_AF32()                                                                    \
+    {                                                                                       \
+        constexpr int NV = 24, NMMA = 3 * SH::TM * SH::TN;                                  \
+        constexpr int NRD = SH::TM * 4 + 2 * SH::TN;                                        \
+        constexpr int NCP = CA == CB ? CA : 0;                                              \
+        __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);                                 \
+        _Pragma("unroll") for (int i = 1; i < SH::TM; ++i) {                                \
+            __builtin_amdgcn_sched_group_barrier(0x008, SH::TN, 0);                         \
+            __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);                             \
+        }                                                                                   \
+        __builtin_amdgcn_sched_group_barrier(0x008, SH::TN, 0);                             \
+        _Pragma("unroll") for (int r = 0; r < NRD / 2; ++r) {                               \
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                              \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                              \
+        }                                                                                   \
+        _Pragma("unroll") for (int r = 0; r < NCP; ++r) {                                   \
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                              \
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);                              \
+        }                                                                                   \
+        __builtin_amdgcn_sched_group_barrier(0x008, NMMA - SH::TM * SH::TN - NRD / 2 - 2 * NCP, 0); \
+    }
+    // Step K (stage K % 4 read into xa / B set during step K - 1): split it, read stage
+    // K + 1, multiply, refill stage K % 4 with step K + 4; the barrier leaves the copies
+    // of steps K + 3 and K + 4 in flight.
+#define FC3_STEP4(K, SNX, SCUR, BH, BL, NBH, NBL)                                           \
+    {                                                                                       \
+        FC3_SPLIT()                                                                         \
+        FC3_READ_NEXT(SNX, NBH, NBL)                                                        \
+        FC3_MMA3(al, BH)                                                                    \
+        issue(SCUR, (K) + 4);                                                               \
+        FC3_MMA3(ah, BL)                                                                    \
+        FC3_MMA3(ah, BH)                                                                    \
+        FC3_ORDER_AF32()                                                                    \
+        FC3_RING_BARRIER(2)                                                                 \
+    }
+#define FC3_EVEN(K, SNX, SCUR) FC3_STEP4(K, SNX, SCUR, bh0, bl0, bh1, bl1)
+#define FC3_ODD(K, SNX, SCUR) FC3_STEP4(K, SNX, SCUR, bh1, bl1, bh0, bl0)
+    {
+        issue(s0, 0);
+        issue(s1, 1);
+        issue(s2, 2);
+        issue(s3, 3);
+        FC3_RING_BARRIER(3)
+        FC3_READ_NEXT(s0, bh0, bl0)
+        FC3_RING_BARRIER(2)
+        int k = 0;
+        for (; k + 4 <= nk; k += 4) {
+            FC3_EVEN(k, s1, s0)
+            FC3_ODD(k + 1, s2, s1)
+            FC3_EVEN(k + 2, s3, s2)
+            FC3_ODD(k + 3, s0, s3)
+        }
+        if (k < nk) FC3_EVEN(k, s1, s0)
+        if (k + 1 < nk) FC3_ODD(k + 1, s2, s1)
+        if (k + 2 < nk) FC3_EVEN(k + 2, s3, s2)
+#undef FC3_EVEN
+#undef FC3_ODD
+#undef FC3_STEP4
+#undef FC3_SPLIT
+#undef FC3_ORDER_AF32
+#undef FC3_READ_NEXT
+#undef FC3_RING_BARRIER
+#undef FC3_MMA3
+        wait_vmcnt0();   // no copy may still be landing when the workgroup's LDS is released
+    }
+    if constexpr (SHAPE == 0) {
+        if (g.epi != 0) {
+            staged_epilogue<false>(g, m0, n0, wm0, wn0, 0, acc,
+                                   reinterpret_cast<float*>(wave == 0   ? s0
+                                                            : wave == 1 ? s1
+                                                            : wave == 2 ? s2
+                                                                        : s3));
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < SH::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < SH::TN; ++j) {
+            const int col = n0 + wn0 + j * 32 + (lane & 31);
+            if (col >= g.N) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = m0 + wm0 + i * 32 + q * 8 + (lane >> 5) * 4;
+                if (row >= g.M) continue;
+                store4<false>(g, 0, row, col, acc[i][j][4 * q], acc[i][j][4 * q + 1],
+                              acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]);
+            }
+        }
+}
+
 // Elements of one LDS stage of the widest (256 x 256) tile: A hi, A lo, B hi, B lo.
 constexpr int kStageE = 2 * BM * BK + 2 * 256 * BK;
 
@@ -829,6 +1035,38 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3(G3 g)
         }
         gemm3_tile<SPLIT_OUT, 0>(g, m0, n0, s0, s1, s2, s3);
     }
+}
+
+// C = A B with A FP32 column-major (A(m, k) = Af[m + k a_KS]) split in registers; FP32
+// C; unsplit K, no batch.  Tile order, stagger and narrow last tile column as k_gemm3.
+template <int NARROW>
+__global__ void __launch_bounds__(NT, 1) k_gemm3_af32(G3 g)
+{
+    const long wg = blockIdx.x;
+    const long tile = (wg % kXcd) * g.per_xcd + wg / kXcd;
+    if (g.stagger > 0 && wg < 256) {
+        const long wait = ((wg / kXcd) % kStaggerGroups) * g.stagger;   // bounded
+        const long t0 = wall_clock64();
+        while (wall_clock64() - t0 < wait) __builtin_amdgcn_s_sleep(2);
+    }
+    if (tile >= (long)g.tiles_m * g.tiles_n) return;
+    const long grp = tile / (kGM * (long)g.tiles_n), first = grp * kGM;
+    const int gsz = (int)((g.tiles_m - first) < kGM ? (g.tiles_m - first) : kGM);
+    const long w = tile - grp * kGM * (long)g.tiles_n;
+    const int tm = (int)(first + w % gsz);
+    const int tn = (int)(w / gsz);
+    const int m0 = tm * BM, n0 = tn * 256;
+    __shared__ __attribute__((aligned(16))) __bf16 s0[kStageE];
+    __shared__ __attribute__((aligned(16))) __bf16 s1[kStageE];
+    __shared__ __attribute__((aligned(16))) __bf16 s2[kStageE];
+    __shared__ __attribute__((aligned(16))) __bf16 s3[kStageE];
+    if constexpr (NARROW != 0) {
+        if (tn == g.tiles_n - 1) {
+            gemm3_tile_af32<NARROW>(g, m0, n0, s0, s1, s2, s3);
+            return;
+        }
+    }
+    gemm3_tile_af32<0>(g, m0, n0, s0, s1, s2, s3);
 }
 
 // Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K) -> the K-blocked A
@@ -1398,6 +1636,55 @@ int launch3_cols(hipStream_t s, int kind, const v3::G3& g)
     if (shape == 2) return launch3<SPLIT, 2, SPK>(s, kind, g);
     if (shape == 3) return launch3<SPLIT, 3, SPK>(s, kind, g);
     return launch3<SPLIT, 0, SPK>(s, kind, g);
+}
+
+template <int NARROW>
+int launch3_af32(hipStream_t s, int kind, v3::G3 g)
+{
+    const int rem = g.N % 256;
+    g.tiles_m = (int)((g.M + v3::BM - 1) / v3::BM);
+    g.tiles_n = NARROW ? (g.N - rem) / 256 + 1 : (g.N + 255) / 256;
+    const long tiles = (long)g.tiles_m * g.tiles_n;
+    if (tiles == 0) return FC3_OK;
+    g.per_xcd = (tiles + kXcd - 1) / kXcd;
+    const long grid = g.per_xcd * kXcd;
+    if (grid > 0x7fffffffL) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: grid too large");
+    g.stagger = (g.K / v3::BK <= 32 && grid >= 4096) ? v3::kStaggerTicks : 0;
+    ProfScope prof(kind, s);
+    hipLaunchKernelGGL((v3::k_gemm3_af32<NARROW>), dim3((unsigned)grid), dim3(v3::NT), 0, s, g);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+// C(m, c) = sum_k A(m, k) phi(k, c) for an FP32 column-major A(m, k) = A[m + k lda] with
+// K rows (zero B rows past the contraction length; the A rows there finite), split into
+// bf16 hi / lo inside the GEMM: C equals, bit for bit, split-transposing A and running
+// gemm3.  FP32 C through o; M % 4 == 0, M >= 4, lda % 4 == 0, A 16-byte aligned.
+int gemm3_af32(hipStream_t s, int kind, const float* A, long lda, const Split3& B,
+               const OutMap& o, long M, int N, int K)
+{
+    if (M >= (1L << 31) || o.R >= (1L << 31))
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: GEMM M too large");
+    const bool ok = K % v3::BK == 0 && K > 0 && M % 4 == 0 && M >= 4 && lda % 4 == 0 &&
+                    lda >= M && B.lo % 8 == 0 && B.ks % 8 == 0 && o.R % 4 == 0 &&
+                    o.SS % 4 == 0 && o.KS % 4 == 0 && o.CS % 4 == 0 && aligned16(A) &&
+                    aligned16(B.b) && aligned16(o.C);
+    if (!ok) return set_error(FC3_ERR_UNSUPPORTED, "mcc3: FP32-operand GEMM layout");
+    v3::G3 g{};
+    g.A = reinterpret_cast<const __bf16*>(A);
+    g.a_KS = lda;
+    g.B = B.b; g.b_lo = B.lo; g.b_KS = B.ks;
+    g.C = o.C; g.c_SS = o.SS; g.c_KS = o.KS; g.c_CS = o.CS; g.c_lo = o.lo; g.c_R = (int)o.R;
+    g.M = (int)M; g.N = N; g.K = K;
+    g.zk = 1; g.kc = K / v3::BK; g.nz = 1;
+    g.a_R2 = 1 << 30;
+    g.epi = o.KS == 16 ? 1 : o.CS == 16 ? 2 : 0;
+    const int rem = N % 256;
+    const int shape = rem == 0 ? 0 : rem <= 32 ? 1 : rem <= 64 ? 2 : rem <= 128 ? 3 : 0;
+    if (shape == 1) return launch3_af32<1>(s, kind, g);
+    if (shape == 2) return launch3_af32<2>(s, kind, g);
+    if (shape == 3) return launch3_af32<3>(s, kind, g);
+    return launch3_af32<0>(s, kind, g);
 }
 
 // C(m, c) = sum_k A(m, k) phi(k, c); A(m, k) = A[(m / a_R) a_SS + (k / 16) a_KS + 16 (m % a_R)
@@ -1974,12 +2261,22 @@ __global__ void __launch_bounds__(256) k_gather_phi_nbr(const float* __restrict_
 
 int mcc3_gather_k(int cap) { return round_up(3 * (cap + 1), 16); }
 
-size_t mcc3_from_gathered_bytes(int n, int cap)
+size_t mcc3_from_gathered_bytes(int n, int cap, bool direct)
 {
     const int N = n / 3, Kc = mcc3_gather_k(cap), Npad = round_up(n, v3::BN);
+    const size_t base = phi3_bytes(n) + Carve::need(2 * sizeof(__bf16) * (size_t)Kc * Npad * N) +
+                        4096;
+    if (direct)   // the n^2 x Kp FP32 intermediate, read by mode 3 directly
+        return base + Carve::need(sizeof(float) * (size_t)n * n * padk(n));
     const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
-    return phi3_bytes(n) + Carve::need(2 * sizeof(__bf16) * (size_t)Kc * Npad * N) +
-           Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n)) + 4096;
+    return base + Carve::need(2 * sizeof(__bf16) * (size_t)R * padk(n));
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, long cnt, float v)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < cnt;
+         i += (long)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, const float* phiT, int n,
@@ -1989,13 +2286,21 @@ int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, const f
     if (!v3_applies(n, FC3_PREC_BF16X3) || !aligned16(U) || !aligned16(k3) || n % 3 != 0)
         return set_error(FC3_ERR_UNSUPPORTED, "fused mcc3: needs n % 4 == 0, n >= 16");
     const int N = n / 3, Kc = mcc3_gather_k(cap), Npad = round_up(n, v3::BN);
+    const int Kp = padk(n);
+    // direct: mode 2 writes the intermediate to an n^2 x Kp FP32 scratch that mode 3 reads
+    // as its FP32 A operand (no split pass); else (a smaller workspace) mode 2 writes it
+    // into k3 and mode 3 runs in place by row blocks, each split into X first.  Both give
+    // the same K3 bit for bit.
+    const bool direct = ws_bytes >= mcc3_from_gathered_bytes(n, cap, true) &&
+                        !std::getenv("FC3_FUSED_SPLIT");
     Carve cv(ws, ws_bytes);
     Split3 B;
     FC3_TRY(prep_phi3(s, phi, n, cv, B));
     const long n2 = (long)n * n;
     const long R = mode3_rows(n, kScratchRowsInplace, kTilesPerLaunch);
     __bf16* Bg = cv.take<__bf16>((size_t)2 * Kc * Npad * N);
-    __bf16* X = cv.take<__bf16>((size_t)2 * R * padk(n));
+    float* V = direct ? cv.take<float>((size_t)n2 * Kp) : nullptr;
+    __bf16* X = direct ? nullptr : cv.take<__bf16>((size_t)2 * R * Kp);
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "fused mcc3 workspace too small");
     hipLaunchKernelGGL(k_gather_phi_nbr, dim3((unsigned)((Npad + 255) / 256), (unsigned)(Kc / 16),
                                               (unsigned)N),
@@ -2004,9 +2309,21 @@ int mcc3_from_gathered(hipStream_t s, const __bf16* U, const float* phi, const f
     // mode 2, one product per atom K over its 3 slabs: C(g n + p, q) = K3[p + n q + n^2 (3K + g)]
     // = sum_t U_K(g n + p, t) B_K(t, q)
     const Split3 BK{Bg, (long)Kc * Npad, 16L * Npad};
-    const OutMap oT{k3, n, n2, 16, n, 0};
+    const OutMap oT{direct ? V : k3, n, n2, 16, n, 0};
     FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE12, U, 3L * n * Kc, 3L * n, 0, 16L * 3 * n, BK, oT, false,
                   3L * n, n, Kc, 0, 3 * n2, N, 2L * 3 * n * Kc, 2L * Kc * Npad));
+    if (direct) {
+        // mode 3 from the FP32 intermediate: K3(m, l) = sum_k V[m + n^2 k] phi(k, l); the
+        // K padding slabs of V are zero (phi's padded rows are zero too)
+        if (Kp > n) {
+            const long pad = n2 * (Kp - n);
+            hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)std::min<long>((pad + 255) / 256, 65536)),
+                               dim3(256), 0, s, V + n2 * n, pad, 0.0f);
+            FC3_HIP(hipGetLastError());
+        }
+        const OutMap oK{k3, n2, 0, 16, n2, 0};
+        return gemm3_af32(s, FC3_KERNEL_GEMM_MODE3, V, n2, B, oK, n2, n, Kp);
+    }
     // mode 3 in place by row blocks (each block split into X before the GEMM overwrites it)
     for (long r0 = 0; r0 < n2; r0 += R) {
         const long rows = (n2 - r0) < R ? (n2 - r0) : R;

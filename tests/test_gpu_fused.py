@@ -55,6 +55,23 @@ def test_fused_matches_fp64_oracle(ctx, cells, seed, su, tol, weighted):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("cells,seed", [((3, 3, 3), 46), ((4, 4, 4), 47), ((5, 5, 5), 48)])
+def test_fused_direct_mode3_equals_split_path(ctx, cells, seed, monkeypatch):
+    """Mode 3 of the fused path reads the FP32 intermediate directly (the bf16 hi/lo split
+    inside the GEMM) when the workspace holds it; FC3_FUSED_SPLIT=1 forces the split pass
+    into a bf16 scratch first.  Both multiply the same operands in the same order: K3 must
+    agree bit for bit (324 and 1500 DoF have K padding, 768 none)."""
+    sysd, calc, phi32, dphi = _case(ctx, cells, seed, 0.05, 1e-12)
+    k_direct = fc3.third_order_mcc3(sysd, LJ, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    monkeypatch.setenv("FC3_FUSED_SPLIT", "1")
+    k_split = fc3.third_order_mcc3(sysd, LJ, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    monkeypatch.delenv("FC3_FUSED_SPLIT")
+    diff = int(np.count_nonzero(k_direct.view(np.uint32) != k_split.view(np.uint32)))
+    print(f"fused {cells}: direct vs split mode 3: {diff} entries differ", flush=True)
+    assert diff == 0
+    dphi.free()
+
+
 @pytest.mark.parametrize("cells,seed", [((3, 3, 3), 44), ((4, 4, 4), 45)])
 def test_fused_matches_dense_path(ctx, cells, seed):
     sysd, calc, phi32, dphi = _case(ctx, cells, seed, 0.05, 1e-12)

@@ -275,6 +275,9 @@ static int get_blas(fc3_context* h, rocblas_handle* out)
         rocblas_handle b = nullptr;
         FC3_BLAS(rocblas_create_handle(&b));
         h->blas = b;
+        // no kernels that sum with atomics (rocBLAS's default, stated here): the FP32
+        // pullback and the eigensolver return the same bits on every run
+        FC3_BLAS(rocblas_set_atomics_mode(b, rocblas_atomics_not_allowed));
     }
     rocblas_handle b = static_cast<rocblas_handle>(h->blas);
     FC3_BLAS(rocblas_set_stream(b, h->stream));

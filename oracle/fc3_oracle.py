@@ -24,7 +24,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# FC3_ORACLE_LIB names another build of fc3_oracle.c (the sanitizer test's ASan/UBSan one)
+_LIB_PATH = os.environ.get("FC3_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -1224,22 +1224,30 @@ __global__ void k_fill_u8(unsigned char* __restrict__ p, long cnt, unsigned char
 // ---- gathered mode 1 (n % 16 == 0) ---------------------------------------------------
 // Psi's (i, j) pattern, Psi viewed as an n^2 x n matrix (rows r = i + n j contiguous):
 // pat[r] = 1 when some Psi[r + n^2 k] != 0.  Thread per 4 consecutive rows, k looped: every
-// load of a wave is 1 KB contiguous, and Psi is read once.  n^2 % 4 == 0.
+// load of a wave is 1 KB contiguous, and Psi is read once.  Eight loads go out before the
+// first is tested, so that each wave keeps 8 KB in flight.  n^2 % 4 == 0.
 __global__ void __launch_bounds__(256) k_psi_pattern(const float* __restrict__ psi, long n2, int n,
                                                      unsigned char* __restrict__ pat)
 {
     const long r = 4 * ((long)blockIdx.x * 256 + threadIdx.x);
     if (r >= n2) return;
-    const float4* p = reinterpret_cast<const float4*>(psi + r);
+    // the bit patterns: a value is nonzero when its bits without the sign are (-0.0f == 0)
+    using v2::u32x4;
+    const u32x4* p = reinterpret_cast<const u32x4*>(psi + r);
     const long step = n2 / 4;
-    bool a = false, b = false, c = false, d = false;
-    for (int k = 0; k < n; ++k) {
-        const float4 v = p[step * k];
-        a |= v.x != 0.0f;
-        b |= v.y != 0.0f;
-        c |= v.z != 0.0f;
-        d |= v.w != 0.0f;
+    u32x4 any = {0u, 0u, 0u, 0u};
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+        u32x4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(p + step * (k + u));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) any |= v[u];
     }
+    for (; k < n; ++k) any |= p[step * k];
+    const unsigned mag = 0x7fffffffu;
+    const bool a = (any.x & mag) != 0, b = (any.y & mag) != 0, c = (any.z & mag) != 0,
+               d = (any.w & mag) != 0;
     uchar4 o;
     o.x = a; o.y = b; o.z = c; o.w = d;
     *reinterpret_cast<uchar4*>(pat + r) = o;

@@ -1398,23 +1398,45 @@ __global__ void __launch_bounds__(256) k_split_slabs(const float* __restrict__ s
 
 // rows x K matrix with K contiguous (src[e + K r]) -> the K-blocked operand
 // Y[(e / 16) 16 rows + 16 r + e % 16] (hi), lo at + rows K.  K % 16 == 0.
+// Block per tile of 128 rows x 64 e (4 K blocks), staged through LDS: the loads are float4
+// runs along each row, and each K block's 128 rows go out as contiguous 16-byte pieces
+// (a wave stores 1 KB runs), where a thread per 4 values would scatter 8-byte stores
+// 16 rows apart.
+constexpr int kSrRows = 128, kSrE = 64;
 __global__ void __launch_bounds__(256) k_split_rows(const float* __restrict__ src, long K,
                                                      long rows, __bf16* __restrict__ Y)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // group of 4 values
-    const long per = K / 4;
-    if (q >= per * rows) return;
-    const long r = q / per, e = 4 * (q - r * per);
-    const float4 x = *reinterpret_cast<const float4*>(src + e + K * r);
-    bf16x4 h, lo;
-    __bf16 a, b;
-    split_bf16(x.x, a, b); h[0] = a; lo[0] = b;
-    split_bf16(x.y, a, b); h[1] = a; lo[1] = b;
-    split_bf16(x.z, a, b); h[2] = a; lo[2] = b;
-    split_bf16(x.w, a, b); h[3] = a; lo[3] = b;
-    const long off = (e >> 4) * 16 * rows + 16 * r + (e & 15);
-    *reinterpret_cast<bf16x4*>(Y + off) = h;
-    *reinterpret_cast<bf16x4*>(Y + off + rows * K) = lo;
+    __shared__ float t[kSrRows][kSrE + 4];
+    const long e0 = (long)blockIdx.x * kSrE, r0 = (long)blockIdx.y * kSrRows;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < kSrRows * kSrE / 4 / 256; ++it) {
+        const int idx = it * 256 + tid, rr = idx / (kSrE / 4), c4 = idx % (kSrE / 4);
+        const long r = r0 + rr, e = e0 + 4 * c4;
+        if (r < rows && e < K)   // K % 16 == 0: the 4 values are all inside
+            *reinterpret_cast<float4*>(&t[rr][4 * c4]) =
+                *reinterpret_cast<const float4*>(src + e + K * r);
+    }
+    __syncthreads();
+    // item = (K block kb of the tile, row rr, half h): 8 values -> 16 bytes per plane
+#pragma unroll
+    for (int it = 0; it < kSrRows * kSrE / 8 / 256; ++it) {
+        const int idx = it * 256 + tid, h = idx & 1, rr = (idx >> 1) % kSrRows,
+                  kb = (idx >> 1) / kSrRows;
+        const long r = r0 + rr, e = e0 + 16 * kb;
+        if (r >= rows || e >= K) continue;
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            __bf16 a, b;
+            split_bf16(t[rr][16 * kb + 8 * h + u], a, b);
+            hi[u] = a;
+            lo[u] = b;
+        }
+        const long off = (e >> 4) * 16 * rows + 16 * r + 8 * h;
+        *reinterpret_cast<bf16x8*>(Y + off) = hi;
+        *reinterpret_cast<bf16x8*>(Y + off + rows * K) = lo;
+    }
 }
 
 // Zeroes the last K block (block_elems bf16 at block_off) of both planes of `slabs` slabs
@@ -2371,9 +2393,9 @@ int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n
             hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, src, (long)n, (int)n2,
                                (int)n2, 0L, (long)n, dst, (unsigned char*)nullptr, 0);
         } else {
-            const long q = n3 / 4;
-            hipLaunchKernelGGL(v3::k_split_rows, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
-                               src, n2, (long)n, dst);
+            dim3 gt((unsigned)((n2 + v3::kSrE - 1) / v3::kSrE),
+                    (unsigned)((n + v3::kSrRows - 1) / v3::kSrRows));
+            hipLaunchKernelGGL(v3::k_split_rows, gt, dim3(256), 0, s, src, n2, (long)n, dst);
         }
         FC3_HIP(hipGetLastError());
     }

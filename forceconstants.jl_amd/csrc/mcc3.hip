@@ -1370,52 +1370,32 @@ __global__ void __launch_bounds__(256) k_split_gather(const float* __restrict__ 
     dl[1] = l1;
 }
 
-// cnt slabs T_l (n x n, T_l[i + n j] at src[l n^2 + i + n j]) -> the K-blocked A operand
-// of the mode-1 product A(l n + j, i) = T_l[i + n j]: hi at
-// Y[l 2P + (i / 16) 16 n + 16 j + i % 16], lo at + P, with P = n Kp (Kp = n rounded up
-// to 16; the padding k >= n is not written).  n % 4 == 0.
-__global__ void __launch_bounds__(256) k_split_slabs(const float* __restrict__ src, int n, int Kp,
-                                                      long cnt, __bf16* __restrict__ Y)
-{
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // group of 4 values
-    const long n2 = (long)n * n;
-    const long per = n2 / 4;
-    if (q >= per * cnt) return;
-    const long l = q / per, e = 4 * (q - l * per);
-    const int j = (int)(e / n), i = (int)(e - (long)j * n);
-    const float4 x = *reinterpret_cast<const float4*>(src + l * n2 + e);
-    bf16x4 h, lo;
-    __bf16 a, b;
-    split_bf16(x.x, a, b); h[0] = a; lo[0] = b;
-    split_bf16(x.y, a, b); h[1] = a; lo[1] = b;
-    split_bf16(x.z, a, b); h[2] = a; lo[2] = b;
-    split_bf16(x.w, a, b); h[3] = a; lo[3] = b;
-    const long P = (long)n * Kp;
-    const long off = l * 2 * P + (long)(i >> 4) * 16 * n + 16L * j + (i & 15);
-    *reinterpret_cast<bf16x4*>(Y + off) = h;
-    *reinterpret_cast<bf16x4*>(Y + off + P) = lo;
-}
-
-// rows x K matrix with K contiguous (src[e + K r]) -> the K-blocked operand
-// Y[(e / 16) 16 rows + 16 r + e % 16] (hi), lo at + rows K.  K % 16 == 0.
+// Per slab z = blockIdx.z: a rows x K matrix with K contiguous, src[z src_zs + e + ld r]
+// (K % 4 == 0, ld % 4 == 0) -> the K-blocked operand Y[z y_zs + (e / 16) 16 rows + 16 r +
+// e % 16] (hi), lo at + lo_off; past K nothing is written (a last partial K block keeps
+// whatever its padding holds).
 // Block per tile of 128 rows x 64 e (4 K blocks), staged through LDS: the loads are float4
 // runs along each row, and each K block's 128 rows go out as contiguous 16-byte pieces
 // (a wave stores 1 KB runs), where a thread per 4 values would scatter 8-byte stores
 // 16 rows apart.
 constexpr int kSrRows = 128, kSrE = 64;
-__global__ void __launch_bounds__(256) k_split_rows(const float* __restrict__ src, long K,
-                                                     long rows, __bf16* __restrict__ Y)
+__global__ void __launch_bounds__(256) k_split_staged(const float* __restrict__ src, long src_zs,
+                                                       long ld, long K, long rows,
+                                                       __bf16* __restrict__ Y, long y_zs,
+                                                       long lo_off)
 {
     __shared__ float t[kSrRows][kSrE + 4];
     const long e0 = (long)blockIdx.x * kSrE, r0 = (long)blockIdx.y * kSrRows;
+    src += (long)blockIdx.z * src_zs;
+    Y += (long)blockIdx.z * y_zs;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int it = 0; it < kSrRows * kSrE / 4 / 256; ++it) {
         const int idx = it * 256 + tid, rr = idx / (kSrE / 4), c4 = idx % (kSrE / 4);
         const long r = r0 + rr, e = e0 + 4 * c4;
-        if (r < rows && e < K)   // K % 16 == 0: the 4 values are all inside
+        if (r < rows && e < K)   // K % 4 == 0: the 4 values are all inside
             *reinterpret_cast<float4*>(&t[rr][4 * c4]) =
-                *reinterpret_cast<const float4*>(src + e + K * r);
+                *reinterpret_cast<const float4*>(src + e + ld * r);
     }
     __syncthreads();
     // item = (K block kb of the tile, row rr, half h): 8 values -> 16 bytes per plane
@@ -1423,7 +1403,7 @@ __global__ void __launch_bounds__(256) k_split_rows(const float* __restrict__ sr
     for (int it = 0; it < kSrRows * kSrE / 8 / 256; ++it) {
         const int idx = it * 256 + tid, h = idx & 1, rr = (idx >> 1) % kSrRows,
                   kb = (idx >> 1) / kSrRows;
-        const long r = r0 + rr, e = e0 + 16 * kb;
+        const long r = r0 + rr, e = e0 + 16 * kb + 8 * h;
         if (r >= rows || e >= K) continue;
         bf16x8 hi, lo;
 #pragma unroll
@@ -1433,9 +1413,18 @@ __global__ void __launch_bounds__(256) k_split_rows(const float* __restrict__ sr
             hi[u] = a;
             lo[u] = b;
         }
-        const long off = (e >> 4) * 16 * rows + 16 * r + 8 * h;
-        *reinterpret_cast<bf16x8*>(Y + off) = hi;
-        *reinterpret_cast<bf16x8*>(Y + off + rows * K) = lo;
+        const long off = (e >> 4) * 16 * rows + 16 * r + (e & 15);
+        if (e + 8 <= K) {
+            *reinterpret_cast<bf16x8*>(Y + off) = hi;
+            *reinterpret_cast<bf16x8*>(Y + off + lo_off) = lo;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e + u < K) {
+                    Y[off + u] = hi[u];
+                    Y[off + lo_off + u] = lo[u];
+                }
+        }
     }
 }
 
@@ -1624,6 +1613,31 @@ int zero_kpad(hipStream_t s, __bf16* base, int n, long slabs)
                        (long)(Kp / 16 - 1) * blk, blk);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
+}
+
+// k_split_staged over `slabs` slabs (see there); K % 4 == 0, ld % 4 == 0, src 16-byte aligned.
+int split_staged(hipStream_t s, const float* src, long src_zs, long ld, long K, long rows,
+                 long slabs, __bf16* Y, long y_zs, long lo_off)
+{
+    if (slabs < 1 || slabs > 65535 || (rows + v3::kSrRows - 1) / v3::kSrRows > 65535 ||
+        (K + v3::kSrE - 1) / v3::kSrE > 0x7fffffffL)
+        return set_error(FC3_ERR_UNSUPPORTED, "mcc3: split grid out of range");
+    const dim3 grid((unsigned)((K + v3::kSrE - 1) / v3::kSrE),
+                    (unsigned)((rows + v3::kSrRows - 1) / v3::kSrRows), (unsigned)slabs);
+    hipLaunchKernelGGL(v3::k_split_staged, grid, dim3(256), 0, s, src, src_zs, ld, K, rows, Y, y_zs,
+                       lo_off);
+    FC3_HIP(hipGetLastError());
+    return FC3_OK;
+}
+
+// cnt slabs T_l (n x n, T_l[i + n j] at src[l n^2 + i + n j]) -> the K-blocked A operand
+// of the mode-1 product A(l n + j, i) = T_l[i + n j]: hi at
+// Y[l 2P + (i / 16) 16 n + 16 j + i % 16], lo at + P, with P = n Kp (Kp = n rounded up
+// to 16; the padding k >= n is not written).  n % 4 == 0.
+int split_slabs(hipStream_t s, const float* src, int n, int cnt, __bf16* Y)
+{
+    const long n2 = (long)n * n, P = (long)n * padk(n);
+    return split_staged(s, src, n2, n, n, n, cnt, Y, 2 * P, P);
 }
 
 // Output addressing of v3::G3 (see there); lo plane at + lo for split outputs.
@@ -2111,10 +2125,7 @@ int mcc3_v3_inplace(hipStream_t s, float* psi, const float* phi, int n, void* ws
     // modes 1 and 2 on the slabs of Psi (they commute with mode 3)
     for (int l0 = 0; l0 < n; l0 += nb) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        const long q = n2 / 4 * cnt;
-        hipLaunchKernelGGL(v3::k_split_slabs, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
-                           psi + n2 * l0, n, Kp, (long)cnt, Y);
-        FC3_HIP(hipGetLastError());
+        FC3_TRY(split_slabs(s, psi + n2 * l0, n, cnt, Y));
         FC3_TRY(modes12_v3(s, Y, n, cnt, B, S, psi + n2 * l0));
     }
     // mode 3 by row blocks: each block is copied (split) to X before the GEMM overwrites it
@@ -2246,10 +2257,7 @@ int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y,
     FC3_TRY(zero_kpad(s, S, n, nb));
     for (int l0 = 0; l0 < n; l0 += nb) {
         const int cnt = (n - l0) < nb ? (n - l0) : nb;
-        const long q = n2 / 4 * cnt;
-        hipLaunchKernelGGL(v3::k_split_slabs, dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s,
-                           X + n2 * l0, n, padk(n), (long)cnt, Ys);
-        FC3_HIP(hipGetLastError());
+        FC3_TRY(split_slabs(s, X + n2 * l0, n, cnt, Ys));
         FC3_TRY(modes12_v3(s, Ys, n, cnt, B, S, Y + n2 * l0));
     }
     return FC3_OK;
@@ -2393,9 +2401,8 @@ int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n
             hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, src, (long)n, (int)n2,
                                (int)n2, 0L, (long)n, dst, (unsigned char*)nullptr, 0);
         } else {
-            dim3 gt((unsigned)((n2 + v3::kSrE - 1) / v3::kSrE),
-                    (unsigned)((n + v3::kSrRows - 1) / v3::kSrRows));
-            hipLaunchKernelGGL(v3::k_split_rows, gt, dim3(256), 0, s, src, n2, (long)n, dst);
+            // rows r < n, K index e < n^2 contiguous: src[e + n^2 r]
+            FC3_TRY(split_staged(s, src, 0, n2, n2, n, 1, dst, 0, (long)n * n2));
         }
         FC3_HIP(hipGetLastError());
     }

@@ -1071,21 +1071,25 @@ __global__ void __launch_bounds__(NT, 1) k_gemm3_af32(G3 g)
 
 // Psi rows [r0, r0 + rows) (M-contiguous: Psi[r + n2 k], k < K) -> the K-blocked A
 // operand with K padded to Kp (a multiple of 16, zeros for k >= K):
-// X[(k / 16) 16 rows + 16 (r - r0) + k % 16] (hi), lo at + rows Kp.  Block (K block
-// blockIdx.x, 256 rows blockIdx.y): thread r loads its row's 16 values (each load 1 KB
-// contiguous over the block) and stores them as two 32-byte runs per plane, so the
+// X[(k / 16) 16 rows + 16 (r - r0) + k % 16] (hi), lo at + rows Kp.  Block per (K block,
+// 256 rows) on a 1-D grid, rows fastest: thread r loads its row's 16 values (each load
+// 1 KB contiguous over the block) and stores them as two 32-byte runs per plane, so the
 // block writes 8 KB contiguous per plane and needs no LDS.
 // With kmask, the block also records whether any of its values is nonzero, at
-// kmask[blockIdx.y nkb + blockIdx.x] (the mode-3 GEMM's row tile and K block), and an
-// all-zero block is not written at all: the GEMM skips it.
+// kmask[tile nkb + kb] (the mode-3 GEMM's row tile and K block), and an all-zero block is
+// not written at all: the GEMM skips it.
 __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict__ psi, long n2,
                                                           int K, int Kp, long r0, long rows,
                                                           __bf16* __restrict__ X,
                                                           unsigned char* __restrict__ kmask,
                                                           int nkb)
 {
-    const int kb = blockIdx.x;
-    const long r = (long)blockIdx.y * 256 + threadIdx.x;
+    // 1-D grid, row tile fastest: neighbouring blocks read neighbouring KB of the same
+    // k rows and write neighbouring 8 KB of the same K block
+    const long ntiles = (rows + 255) / 256;
+    const long tile = blockIdx.x % ntiles;
+    const int kb = (int)(blockIdx.x / ntiles);
+    const long r = tile * 256 + threadIdx.x;
     const bool valid = r < rows;
     const int k0 = 16 * kb;
     bf16x8 h0, h1, l0, l1;
@@ -1101,7 +1105,7 @@ __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict
     }
     if (kmask) {   // every thread of the block reaches this barrier
         const int any = __syncthreads_or(nz ? 1 : 0);
-        if (threadIdx.x == 0) kmask[(long)blockIdx.y * nkb + kb] = any ? 1 : 0;
+        if (threadIdx.x == 0) kmask[tile * nkb + kb] = any ? 1 : 0;
         if (!any) return;
     }
     if (!valid) return;
@@ -1118,17 +1122,19 @@ __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict
 // u) (r0 = n j0), as X rows m = (i / 16) 16 u + 16 (j - j0) + i % 16 for i < Kp (rows
 // i >= n, the K padding of U, are zero), split and K-blocked over k like
 // k_split_transpose, with rows = u Kp.  A 256-row GEMM tile is then 16 i x 16 j (for
-// u >= 16): its output column is one contiguous run of U.  Block (K block blockIdx.x, row
-// tile blockIdx.y); kmask and the skipped all-zero blocks as k_split_transpose.
+// u >= 16): its output column is one contiguous run of U.  Block (K block, row tile) on a
+// 1-D grid, tiles fastest; kmask and the skipped all-zero blocks as k_split_transpose.
 __global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restrict__ psi, long n2,
                                                             int n, int Kp, long r0, int u,
                                                             __bf16* __restrict__ X,
                                                             unsigned char* __restrict__ kmask,
                                                             int nkb)
 {
-    const int kb = blockIdx.x;
     const long rows = (long)u * Kp;
-    const long m = (long)blockIdx.y * 256 + threadIdx.x;
+    const long ntiles = (rows + 255) / 256;   // 1-D grid, row tile fastest
+    const long tile = blockIdx.x % ntiles;
+    const int kb = (int)(blockIdx.x / ntiles);
+    const long m = tile * 256 + threadIdx.x;
     const long ib = m / (16L * u), rem = m - ib * 16L * u;
     const int jj = (int)(rem >> 4), i = (int)(ib * 16 + (rem & 15));
     const bool inb = m < rows, valid = inb && i < n;
@@ -1146,7 +1152,7 @@ __global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restri
     }
     if (kmask) {   // every thread of the block reaches this barrier
         const int any = __syncthreads_or(nz ? 1 : 0);
-        if (threadIdx.x == 0) kmask[(long)blockIdx.y * nkb + kb] = any ? 1 : 0;
+        if (threadIdx.x == 0) kmask[tile * nkb + kb] = any ? 1 : 0;
         if (!any) return;
     }
     if (!inb) return;
@@ -1162,7 +1168,7 @@ __global__ void __launch_bounds__(256) k_split_transpose_u(const float* __restri
 // k_split_transpose_u over two neighbouring i blocks per workgroup (512 threads: 32 i x
 // 16 j), so that each load of a k covers whole 128-byte lines of Psi (32 consecutive i)
 // instead of half lines that another workgroup fetches again.  Same X and kmask as
-// k_split_transpose_u.  Block (K block blockIdx.x, (i-block pair, 16-j chunk) blockIdx.y);
+// k_split_transpose_u.  Block (K block, (i-block pair, 16-j chunk)) on a 1-D grid;
 // u % 16 == 0.
 __global__ void __launch_bounds__(512) k_split_transpose_u2(const float* __restrict__ psi, long n2,
                                                              int n, int Kp, long r0, int u,
@@ -1171,10 +1177,12 @@ __global__ void __launch_bounds__(512) k_split_transpose_u2(const float* __restr
                                                              int nkb)
 {
     __shared__ int any2[2];
-    const int kb = blockIdx.x;
     const long rows = (long)u * Kp;
     const int chunks = u / 16;
-    const int ib2 = blockIdx.y / chunks, jc = blockIdx.y - ib2 * chunks;
+    const long nblk = (long)((Kp / 16 + 1) / 2) * chunks;   // 1-D grid, row block fastest
+    const int by = (int)(blockIdx.x % nblk);
+    const int kb = (int)(blockIdx.x / nblk);
+    const int ib2 = by / chunks, jc = by - ib2 * chunks;
     const int il = threadIdx.x & 31, jl = threadIdx.x >> 5, half = il >> 4;
     const int i = 32 * ib2 + il, jj = 16 * jc + jl;
     const long ib = 2L * ib2 + half;
@@ -1326,15 +1334,16 @@ __global__ void __launch_bounds__(256) k_gather_phi_list(const float* __restrict
 // k like k_split_transpose_u (rows = 16 G n).  A 256-row tile is one (jb, tb): a tile past
 // the group's rows only records an empty mask (its GEMM tile then writes nothing, and mode 1
 // never reads those rows).  kmask[tile nkb + kb] = any nonzero in the tile's K block kb.
-// Block (K block blockIdx.x, tile blockIdx.y); thread (jj, t % 16).
+// Block (K block, tile) on a 1-D grid, tiles fastest; thread (jj, t % 16).
 __global__ void __launch_bounds__(256) k_split_gather(const float* __restrict__ psi, long n2, int n,
                                                       int jb0, const int* __restrict__ list,
                                                       const int* __restrict__ cnt,
                                                       __bf16* __restrict__ X, long rows,
                                                       unsigned char* __restrict__ kmask, int nkb)
 {
-    const int kb = blockIdx.x;
-    const long tile = blockIdx.y;
+    const long ntiles = rows / 256;                // 1-D grid, tile fastest
+    const long tile = blockIdx.x % ntiles;
+    const int kb = (int)(blockIdx.x / ntiles);
     const int tpg = n / 16;                        // tiles per group
     const int jb = jb0 + (int)(tile / tpg), tb = (int)(tile % tpg);
     const int L = cnt[jb];
@@ -1833,7 +1842,7 @@ int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, cons
     const long n2 = (long)n * n;
     const int Kp = padk(n);
     if (kmask && (!split_out || Kp / 16 > v3::kMaxKBlocks)) kmask = nullptr;
-    dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
+    const dim3 gt((unsigned)((Kp / 16) * ((rows + 255) / 256)));
     hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows, X,
                        kmask, Kp / 16);
     FC3_HIP(hipGetLastError());
@@ -1857,11 +1866,11 @@ int mode3_block_u(hipStream_t s, const float* psi, int n, long j0, int u, const 
     const long rows = (long)u * Kp;
     if (u % 16 == 0) {   // whole 128-byte lines of Psi per load
         const long pairs = (Kp / 16 + 1) / 2;
-        dim3 gt((unsigned)(Kp / 16), (unsigned)(pairs * (u / 16)));
+        const dim3 gt((unsigned)((Kp / 16) * pairs * (u / 16)));
         hipLaunchKernelGGL(v3::k_split_transpose_u2, gt, dim3(512), 0, s, psi, n2, n, Kp, n * j0,
                            u, X, kmask, Kp / 16);
     } else {
-        dim3 gt((unsigned)(Kp / 16), (unsigned)((rows + 255) / 256));
+        const dim3 gt((unsigned)((Kp / 16) * ((rows + 255) / 256)));
         hipLaunchKernelGGL(v3::k_split_transpose_u, gt, dim3(256), 0, s, psi, n2, n, Kp, n * j0,
                            u, X, kmask, Kp / 16);
     }
@@ -2058,7 +2067,7 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
         for (int jb0 = 0; jb0 < NG; jb0 += G) {
             const int g = std::min(G, NG - jb0);
             const long rows = 16L * g * n;
-            hipLaunchKernelGGL(v3::k_split_gather, dim3((unsigned)(Kp / 16), (unsigned)(rows / 256)),
+            hipLaunchKernelGGL(v3::k_split_gather, dim3((unsigned)((Kp / 16) * (rows / 256))),
                                dim3(256), 0, s, psi, n2, n, jb0, glist, gcnt, X, rows, mask,
                                Kp / 16);
             FC3_HIP(hipGetLastError());
@@ -2397,7 +2406,7 @@ int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n
         __bf16* dst = t == 0 ? SA : SB;
         if (form == 0) {
             // rows r < n, K index e < n^2 at stride n: src[r + n e]
-            dim3 gt((unsigned)(n2 / 16), (unsigned)((n + 255) / 256));
+            const dim3 gt((unsigned)((n2 / 16) * ((n + 255) / 256)));
             hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, src, (long)n, (int)n2,
                                (int)n2, 0L, (long)n, dst, (unsigned char*)nullptr, 0);
         } else {

@@ -316,6 +316,7 @@ int fc3_second_order_lj(fc3_handle h, const double* positions, int n_atoms, cons
                         double sigma, double epsilon, double pot_r_cut, double r_cut, double tol,
                         double* ifc2)
 {
+    FC3_TRACE_RANGE();
     (void)pot_r_cut;
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
@@ -340,6 +341,7 @@ int fc3_third_order_lj(fc3_handle h, const double* positions, int n_atoms, const
                        double sigma, double epsilon, double pot_r_cut, double r_cut, double tol,
                        const double* masses, double* psi64, float* psi32)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
     for (int c = 0; c < 3; ++c)
@@ -372,6 +374,7 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
                             double r_cut, double tol, const double* masses, const float* phi,
                             float* k3)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
     for (int c = 0; c < 3; ++c)
@@ -429,6 +432,7 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
 int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int dim,
                                 const double* masses)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     // @assert D ∈ [1,2,3] (third_order_analytical.jl:190)
     if (dim < 1 || dim > 3) return set_error(FC3_ERR_ARGUMENT, "D ∈ [1, 2, 3]");
@@ -447,6 +451,7 @@ int fc3_mass_weight_third_order(fc3_handle h, double* psi64, int n_atoms, int di
 int fc3_second_order_sw(fc3_handle h, const double* positions, int n_atoms, const double box[3],
                         const double* sw_params, double r_cut, double tol, double* ifc2)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
     if (!sw_params || !ifc2) return set_error(FC3_ERR_ARGUMENT, "sw_params, ifc2 must be non-null");
@@ -466,6 +471,7 @@ int fc3_modes_lj(fc3_handle h, const double* positions, int n_atoms, const doubl
                  const double* masses, int num_rigid_translation, double* freqs_sq,
                  double* phi64, float* phi32)
 {
+    FC3_TRACE_RANGE();
     (void)pot_r_cut;
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
@@ -515,6 +521,7 @@ int fc3_modes_sw(fc3_handle h, const double* positions, int n_atoms, const doubl
                  const double* sw_params, double r_cut, double tol, const double* masses,
                  int num_rigid_translation, double* freqs_sq, double* phi64, float* phi32)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
     if (!sw_params || !masses || !freqs_sq)
@@ -563,6 +570,7 @@ int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms, const
                        const double* sw_params, double r_cut, double tol, const double* masses,
                        double* psi64, float* psi32)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
     if (!sw_params) return set_error(FC3_ERR_ARGUMENT, "sw_params must be non-null");
@@ -584,6 +592,7 @@ int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms, const
 
 int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precision, float* k3)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     if (!psi || !phi || !k3) return set_error(FC3_ERR_ARGUMENT, "psi, phi, k3 must be non-null");
     if (n < 1) return set_error(FC3_ERR_ARGUMENT, "n must be >= 1");
@@ -600,6 +609,7 @@ int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precis
 
 int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int precision)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     if (!psi || !phi) return set_error(FC3_ERR_ARGUMENT, "psi, phi must be non-null");
     if (n < 1) return set_error(FC3_ERR_ARGUMENT, "n must be >= 1");
@@ -615,6 +625,7 @@ int fc3_mcc3_inplace(fc3_handle h, float* psi, const float* phi, int n, int prec
 int fc3_mcc3_unique(fc3_handle h, const float* psi, const float* phi, int n, int precision,
                     float* k3u)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     if (!psi || !phi || !k3u) return set_error(FC3_ERR_ARGUMENT, "psi, phi, k3u must be non-null");
     if (n < 1) return set_error(FC3_ERR_ARGUMENT, "n must be >= 1");
@@ -634,6 +645,7 @@ int fc3_mcc3_unique(fc3_handle h, const float* psi, const float* phi, int n, int
 int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n, int block_size,
                      int precision, float* k3_host)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     if (!psi || !phi || !k3_host)
         return set_error(FC3_ERR_ARGUMENT, "psi, phi, k3_host must be non-null");
@@ -700,6 +712,7 @@ int fc3_mcc3_blocked(fc3_handle h, const float* psi, const float* phi, int n, in
 int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const float* dk3, int n,
                       int precision, float* dpsi, float* dphi)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     if (!phi || !dk3) return set_error(FC3_ERR_ARGUMENT, "phi, dk3 must be non-null");
     if (!dpsi && !dphi) return set_error(FC3_ERR_ARGUMENT, "dpsi and dphi are both null");
@@ -718,6 +731,7 @@ int fc3_mcc3_backward(fc3_handle h, const float* psi, const float* phi, const fl
 int fc3_weighted_square_loss(fc3_handle h, const float* k3, long long count,
                              unsigned long long seed, float* dk3, double* loss)
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     if (!k3 || count < 0) return set_error(FC3_ERR_ARGUMENT, "k3 must be non-null, count >= 0");
     if (!dk3 && !loss) return set_error(FC3_ERR_ARGUMENT, "dk3 and loss are both null");
@@ -748,6 +762,7 @@ int fc3_third_order_lj_param_backward(fc3_handle h, const double* positions, int
                                       const double* masses, const float* dpsi32,
                                       const double* dpsi64, double grad[2])
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
     for (int c = 0; c < 3; ++c)
@@ -787,6 +802,7 @@ int fc3_third_order_sw_param_backward(fc3_handle h, const double* positions, int
                                       double tol, const double* masses, const float* dpsi32,
                                       const double* dpsi64, double grad[4])
 {
+    FC3_TRACE_RANGE();
     FC3_TRY(check_handle(h));
     FC3_TRY(check_system(positions, n_atoms, box));
     if (!sw_params || !grad) return set_error(FC3_ERR_ARGUMENT, "sw_params, grad must be non-null");

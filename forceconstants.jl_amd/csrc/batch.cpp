@@ -497,6 +497,7 @@ int fc3_batch_third_order_mcc3_lj(fc3_handle h, const fc3_snapshot* snaps, int c
                                   double sigma, double epsilon, double pot_r_cut, double r_cut,
                                   double tol, int precision, int flags)
 {
+    FC3_TRACE_RANGE();
     if (!h) return set_error(FC3_ERR_ARGUMENT, "null handle");
     FC3_HIP(hipSetDevice(h->device));
     set_current(h);

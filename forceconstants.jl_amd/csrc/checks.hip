@@ -110,6 +110,7 @@ extern "C" {
 int fc3_asr_satisfied(fc3_handle h, const double* ifc, int order, int n_atoms, int dim, double tol,
                       long long* n_violations, long long* first_violation, double* first_sum)
 {
+    FC3_TRACE_RANGE();
     if (!h) return set_error(FC3_ERR_ARGUMENT, "null fc3 handle");
     FC3_HIP(hipSetDevice(h->device));
     set_current(h);
@@ -139,6 +140,7 @@ int fc3_asr_satisfied(fc3_handle h, const double* ifc, int order, int n_atoms, i
 
 int fc3_issymmetric2(fc3_handle h, const double* ifc2, int n, long long* n_asymmetric)
 {
+    FC3_TRACE_RANGE();
     if (!h) return set_error(FC3_ERR_ARGUMENT, "null fc3 handle");
     FC3_HIP(hipSetDevice(h->device));
     set_current(h);

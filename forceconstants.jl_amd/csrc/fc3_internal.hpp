@@ -64,6 +64,9 @@ namespace fc3 {
 
 // The handle whose API call is running on this thread (set by the C-ABI layer).
 void set_current(fc3_context* h);
+// A launch site of one kernel kind: a roctx range named for the kind around the enqueue
+// (host side, seen by rocprofv3 --marker-trace), and, when the handle's profiling is on,
+// HIP events around its kernels on the launch stream.
 struct ProfScope {
     fc3_context* h = nullptr;
     hipStream_t s = nullptr;
@@ -71,6 +74,14 @@ struct ProfScope {
     ProfScope(int kind, hipStream_t stream);
     ~ProfScope();
 };
+// A roctx range around one C-ABI call (its enqueue on the host), named for the function.
+struct TraceRange {
+    explicit TraceRange(const char* name);
+    ~TraceRange();
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
+#define FC3_TRACE_RANGE() ::fc3::TraceRange fc3_trace_range_(__func__)
 
 // Allocation-free entry points (noalloc.hip): size of the static device scratch, and the
 // once-per-device lookup of its address (fc3_create calls it with the device current).

@@ -338,6 +338,7 @@ int fc3_format_float(double x, char* buf, int len)
 int fc3_save_second_order_txt(fc3_handle h, const double* ifc2, int n_atoms, int dim,
                               const char* units, const char* path)
 {
+    FC3_TRACE_RANGE();
     if (!h) return set_error(FC3_ERR_ARGUMENT, "null fc3 handle");
     FC3_HIP(hipSetDevice(h->device));
     set_current(h);
@@ -350,6 +351,7 @@ int fc3_save_second_order_txt(fc3_handle h, const double* ifc2, int n_atoms, int
 int fc3_save_third_order_txt(fc3_handle h, const double* psi, int n_atoms, int dim,
                              const char* units, const char* path)
 {
+    FC3_TRACE_RANGE();
     if (!h) return set_error(FC3_ERR_ARGUMENT, "null fc3 handle");
     FC3_HIP(hipSetDevice(h->device));
     set_current(h);

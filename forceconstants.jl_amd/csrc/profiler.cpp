@@ -1,4 +1,7 @@
-// Per-kernel event timing for measurements (fc3_profile_enable / fc3_profile_query).
+// Per-kernel event timing for measurements (fc3_profile_enable / fc3_profile_query), and
+// roctx ranges around the C-ABI calls and each kernel kind's launches.
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "fc3_internal.hpp"
 
 namespace fc3 {
@@ -21,8 +24,25 @@ hipEvent_t take_event(Profiler& p)
 
 void set_current(fc3_context* h) { tl_current = h; }
 
+const char* kind_range_name(int kind)
+{
+    switch (kind) {
+    case FC3_KERNEL_IFC3_WRITE: return "fc3 IFC3 writer";
+    case FC3_KERNEL_GEMM_MODE3: return "fc3 mcc3 mode 3";
+    case FC3_KERNEL_GEMM_MODE12: return "fc3 mcc3 mode 1";
+    case FC3_KERNEL_PAIR_BLOCKS: return "fc3 pair blocks";
+    case FC3_KERNEL_NEIGHBORS: return "fc3 neighbour lists";
+    case FC3_KERNEL_GEMM_MODE2: return "fc3 mcc3 mode 2";
+    default: return "fc3 kernel";
+    }
+}
+
+TraceRange::TraceRange(const char* name) { roctxRangePushA(name); }
+TraceRange::~TraceRange() { roctxRangePop(); }
+
 ProfScope::ProfScope(int kind, hipStream_t stream)
 {
+    roctxRangePushA(kind_range_name(kind));
     if (!tl_current || !tl_current->prof.enabled) return;
     h = tl_current;
     s = stream;
@@ -36,6 +56,7 @@ ProfScope::ProfScope(int kind, hipStream_t stream)
 ProfScope::~ProfScope()
 {
     if (h && rec >= 0) hipEventRecord(h->prof.records[rec].stop, s);
+    roctxRangePop();
 }
 
 }  // namespace fc3

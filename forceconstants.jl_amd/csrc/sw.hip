@@ -88,27 +88,36 @@ __device__ __forceinline__ u64 key3of(int A, int B, int C, int N)
     return ((u64)C * N + B) * N + A;
 }
 
+// One thread per (centre i, slot s, slot u) with u >= s.  Centre i's records lie at
+// off3[i] (off2[i]) in slot order: slot s's pair term (6 IFC3 / 2 IFC2 records, when nb[s]
+// > i), then its triplets (s, tt) for tt = s + 1 .. d - 1 (24 / 6 records each).  Thread
+// (i, s, s) writes the pair term and thread (i, s, u > s) the triplet (s, u), each at its
+// place in that order, so the records are those of a sequential loop over the slots.
 __global__ void __launch_bounds__(64) k_sw_emit(EmitCtx c)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= c.N * c.cap) return;
-    const int i = t / c.cap, s = t - i * c.cap;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)c.N * c.cap * c.cap) return;
+    const int i = (int)(t / ((long)c.cap * c.cap));
+    const int su = (int)(t - (long)i * c.cap * c.cap);
+    const int s = su / c.cap, u = su - s * c.cap;
     const int d = c.count[i];
-    if (s >= d) return;
+    if (s >= d || u >= d || u < s) return;
     const int* nb = c.nbr + (size_t)i * c.cap;
     const int N = c.N;
     const int f = first_above(nb, d, i);
-    const int before_pairs = s > f ? s - f : 0;
-    const int before_trips = s * (d - 1) - s * (s - 1) / 2;
-    long r3 = c.off3 ? c.off3[i] + 6L * before_pairs + 24L * before_trips : 0;
-    long r2 = c.off2 ? c.off2[i] + 2L * before_pairs + 6L * before_trips : 0;
     const int j = nb[s];
+    // records before this thread's: the pair terms of slots < s (and of s itself for a
+    // triplet thread), the triplets of slots < s, and slot s's triplets before u
+    const int pairs_before = (s > f ? s - f : 0) + (u > s && j > i ? 1 : 0);
+    const int trips_before = s * (d - 1) - s * (s - 1) / 2 + (u > s ? u - s - 1 : 0);
+    long r3 = c.off3 ? c.off3[i] + 6L * pairs_before + 24L * trips_before : 0;
+    long r2 = c.off2 ? c.off2[i] + 2L * pairs_before + 6L * trips_before : 0;
     const double ri[3] = {c.pos[3 * i], c.pos[3 * i + 1], c.pos[3 * i + 2]};
     double rij[3] = {ri[0] - c.pos[3 * j], ri[1] - c.pos[3 * j + 1], ri[2] - c.pos[3 * j + 2]};
     geom::mirror(rij, c.L);
 
     double blk[27];
-    if (j > i) {
+    if (u == s && j > i) {
         // pair term: (i,i,j),(i,j,i),(j,i,i) get -H3, (j,j,i),(j,i,j),(i,j,j) get +H3;
         // IFC2 (i,j) and (j,i) get -H2 (third_order_AD.jl:103-111, second_order_AD.jl:66-70)
         double h2[9];
@@ -131,8 +140,8 @@ __global__ void __launch_bounds__(64) k_sw_emit(EmitCtx c)
             ++r2;
         }
     }
-    for (int tt = s + 1; tt < d; ++tt) {
-        const int k = nb[tt];
+    if (u > s) {
+        const int k = nb[u];
         double rik[3] = {ri[0] - c.pos[3 * k], ri[1] - c.pos[3 * k + 1], ri[2] - c.pos[3 * k + 2]};
         geom::mirror(rik, c.L);
         // the reference evaluates its blocks at r_arr = [r_i; r_i - r_ij; r_i - r_ik]
@@ -817,7 +826,8 @@ int sw_assemble(hipStream_t s, const double* d_pos, const double box[3], const d
     c.key3 = key3; c.val3 = val3; c.key2 = key2; c.val2 = val2;
     {
         ProfScope prof(FC3_KERNEL_PAIR_BLOCKS, s);
-        hipLaunchKernelGGL(k_sw_emit, dim3(blocks((long)N * nl.cap, 64)), dim3(64), 0, s, c);
+        hipLaunchKernelGGL(k_sw_emit, dim3(blocks((long)N * nl.cap * nl.cap, 64)), dim3(64), 0, s,
+                           c);
         hipLaunchKernelGGL(k_sw_self, dim3(blocks(N, 256)), dim3(256), 0, s, nl.count, N, R3,
                            key3, val3, R2, key2, val2);
         FC3_HIP(hipGetLastError());

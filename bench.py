@@ -241,6 +241,65 @@ def sw_assembly(ctx, steps=3):
     return res
 
 
+def sw_fused(ctx, steps=3, cells=4):
+    """Stillinger-Weber third_order + mcc3 on diamond 4x4x4 (1536 DoF, sigma_u = 0.03 A,
+    seed 1536), phi from the library's dynamical matrix and host eigensolver: the fused
+    call (mode 1 over the reduced block table's nonzeros, modes 2 and 3 BF16x3, no n^3
+    Psi) against the dense route (FP32 mass-weighted Psi, then the BF16x3 mcc3), with the
+    relative difference of the two K3 on 4 sampled slabs."""
+    import ctypes
+    import fc3
+    from fc3 import systems as S
+    from fc3._lib import check
+    pot = S.StillingerWeberSilicon()
+    calc = S.AutoDiffCalculator(1e-12, pot.r_cut)
+    sysd = S.diamond(S.SI_A, (cells, cells, cells), S.SI_MASS, 0.03, seed=3 * 8 * cells ** 3)
+    n = sysd.n_dof
+    _, phi = fc3.get_modes(fc3.dynamical_matrix(sysd, pot, calc, ctx=ctx))
+    dphi = fc3.DeviceArray.from_host(np.asfortranarray(phi.astype(np.float32)), ctx=ctx)
+    k3f = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    k3d = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    d32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+
+    def dense():
+        fc3.third_order_device(sysd, pot, calc, psi32=d32, masses=sysd.masses, ctx=ctx)
+        fc3.mcc3(d32, dphi, precision="bf16x3", out=k3d)
+
+    def fused():
+        fc3.third_order_mcc3(sysd, pot, calc, dphi, masses=sysd.masses, out=k3f, ctx=ctx)
+
+    res = {"config": f"SW silicon diamond {cells}x{cells}x{cells} ({n} DoF)"}
+    for name, fn in (("dense", dense), ("fused", fused)):
+        fn()
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(steps):
+            fn()
+        res[f"{name}_ms_per_step"] = ctx.timer_stop() / steps
+    d32.free()
+    num = den = 0.0
+    a = np.empty((n, n), dtype=np.float32, order="F")
+    b = np.empty_like(a)
+    for l in np.linspace(3, n - 1, 4).astype(int):
+        off = 4 * n * n * int(l)
+        check(ctx.lib.fc3_memcpy_d2h(ctx.h, a.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.c_void_p(k3f.ptr.value + off), 4 * n * n))
+        check(ctx.lib.fc3_memcpy_d2h(ctx.h, b.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.c_void_p(k3d.ptr.value + off), 4 * n * n))
+        num += float(np.sum((a.astype(np.float64) - b) ** 2))
+        den += float(np.sum(b.astype(np.float64) ** 2))
+    k3f.free()
+    k3d.free()
+    dphi.free()
+    ctx.release_workspace()
+    res["speedup_vs_dense"] = res["dense_ms_per_step"] / res["fused_ms_per_step"]
+    res["rel_fro_diff_vs_dense"] = float(np.sqrt(num / den))
+    res["note"] = ("dense = FP32 mass-weighted Psi (the assembly without the FP64 copy) + "
+                   "block-sparse BF16x3 mcc3; fused = the same K3 with no n^3 Psi")
+    log(f"sw fused {res}")
+    return res
+
+
 def batched_snapshots(ctx, per_size=32, replays=3, modes_timing=False):
     """The batched configuration: 3 x `per_size` LJ argon snapshots of FCC 3x3x3,
     4x4x4 and 5x5x5 (324, 768, 1500 DoF), sigma_u ~ U[0.02, 0.10] A per snapshot
@@ -747,6 +806,7 @@ def main():
         out["adjoint"] = adjoint_timing(ctx)
     if not a.no_sw:
         out["sw_assembly"] = sw_assembly(ctx)
+        out["sw_fused"] = sw_fused(ctx)
     if not a.no_cpu_baseline:
         log("cpu baselines")
         out["cpu_baseline"] = cpu_baseline_mcc3(coo32, phi32, n)

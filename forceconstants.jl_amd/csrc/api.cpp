@@ -590,6 +590,67 @@ int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms, const
                        nullptr);
 }
 
+int fc3_third_order_mcc3_sw(fc3_handle h, const double* positions, int n_atoms,
+                            const double box[3], const double* sw_params, double r_cut,
+                            double tol, const double* masses, const float* phi, float* k3)
+{
+    FC3_TRACE_RANGE();
+    FC3_TRY(check_handle(h));
+    FC3_TRY(check_system(positions, n_atoms, box));
+    if (!sw_params) return set_error(FC3_ERR_ARGUMENT, "sw_params must be non-null");
+    if (!phi || !k3) return set_error(FC3_ERR_ARGUMENT, "phi, k3 must be non-null");
+    if (!(r_cut <= sw_params[2] * sw_params[1]))
+        return set_error(FC3_ERR_ARGUMENT,
+                         "For AutoDiff SW silicon force constant cutoff must be less than potential cutoff");
+    if (masses)
+        for (int i = 0; i < n_atoms; ++i)
+            if (!(masses[i] > 0.0)) return set_error(FC3_ERR_ARGUMENT, "masses must be positive");
+    const int N = n_atoms;
+    const long n = 3L * N;
+    if (n % 4 != 0 || n < 16 || n * n >= (1L << 31) || (uintptr_t)k3 % 16 != 0)
+        return set_error(FC3_ERR_UNSUPPORTED,
+                         "fused third_order + mcc3: needs 3N % 4 == 0, 3N >= 16, aligned k3");
+    // A block joins atoms of one triplet, so a slab atom's column atoms lie within two
+    // neighbour hops: at most cap + cap (cap - 1) = cap^2 of them besides itself.
+    auto cap2_bound = [&](int cap) { return (int)std::min<long>((long)N - 1 > 0 ? N - 1 : 1,
+                                                                (long)cap * cap); };
+    auto need = [&](int cap, bool direct) {
+        const int c2 = cap2_bound(cap);
+        return Carve::need(sw_workspace_bytes(N, cap, true)) +
+               Carve::need(sizeof(float) * (size_t)n * n) +
+               Carve::need(sizeof(uint16_t) * 2 * 3 * (size_t)n * mcc3_gather_k(c2) * N) +
+               Carve::need(sw_fused_list_bytes(N, c2)) +
+               mcc3_from_gathered_bytes((int)n, c2, direct) + 4096;
+    };
+    Prepared P;
+    int rc = prepare(h, positions, N, box, r_cut, masses,
+                     [&](int cap) { return need(cap, true); }, P);
+    if (rc == FC3_ERR_OUT_OF_MEMORY)
+        rc = prepare(h, positions, N, box, r_cut, masses,
+                     [&](int cap) { return need(cap, false); }, P);
+    FC3_TRY(rc);
+    const int c2 = cap2_bound(P.nl.cap);
+    Carve cv(P.extra, P.extra_bytes);
+    char* sws = cv.take<char>(sw_workspace_bytes(N, P.nl.cap, true));
+    float* phiT = cv.take<float>((size_t)n * n);
+    __bf16* U = cv.take<__bf16>(2 * 3 * (size_t)n * mcc3_gather_k(c2) * N);
+    char* lists = cv.take<char>(sw_fused_list_bytes(N, c2));
+    char* rest = cv.take<char>(0);
+    if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "fused SW third_order + mcc3 workspace");
+    SwBlockTable T;
+    FC3_TRY(sw_assemble(h->stream, P.pos, box, sw_params, sw_params[10], sw_params[5], P.nl, sws,
+                        sw_workspace_bytes(N, P.nl.cap, true), tol, masses ? P.masses : nullptr,
+                        nullptr, nullptr, nullptr, &T));
+    SwColumns cols;
+    FC3_TRY(sw_fused_columns(h->stream, T, N, c2, lists, cols));
+    const int Kc = mcc3_gather_k(cols.cap2);
+    FC3_TRY(transpose_square(h->stream, phi, (int)n, phiT));
+    FC3_TRY(sw_mode1_gathered(h->stream, T, N, cols, tol, masses ? P.masses : nullptr, phiT, Kc,
+                              U));
+    return mcc3_from_gathered(h->stream, U, phi, phiT, (int)n, cols.colcnt, cols.collist,
+                              cols.cap2, k3, rest, cv.cap - cv.off);
+}
+
 int fc3_mcc3(fc3_handle h, const float* psi, const float* phi, int n, int precision, float* k3)
 {
     FC3_TRACE_RANGE();

@@ -183,6 +183,28 @@ int sw_param_grad(hipStream_t s, const double* d_pos, const double box[3], const
                   const SwBlockTable* fwd, const double* d_masses, const float* dpsi32,
                   const double* dpsi64, void* ws, double* d_out);
 
+// ---- fused SW third_order + mcc3 (sw_fused.hip) --------------------------------------
+// Columns of each slab atom K from an SW block table, in the LJ fused path's convention:
+// slot 0 = K, slots 1 .. colcnt[K] = the other atoms J with a block (., J, K), ascending, at
+// collist[K cap2 + s - 1]; seg[K 2 (cap2 + 1) + 2 s .. + 1] = slot s's block range.
+struct SwColumns {
+    int* colcnt = nullptr;
+    int* collist = nullptr;
+    int* seg = nullptr;
+    int cap2 = 0;     // most column atoms J != K of any slab
+    int maxblk = 0;   // most blocks of any slab
+};
+// Workspace of sw_fused_columns for at most cap2_ub column atoms per slab.
+size_t sw_fused_list_bytes(int N, int cap2_ub);
+// Builds the columns (one host read-back of the maxima); FC3_ERR_UNSUPPORTED past cap2_ub.
+int sw_fused_columns(hipStream_t s, const SwBlockTable& T, int N, int cap2_ub, void* ws,
+                     SwColumns& out);
+// The mode-1 product over the table's nonzeros (the FP32 writer's values: tolerance, mass
+// weighting), in mcc3_from_gathered's layout with Kc = mcc3_gather_k(cols.cap2).
+size_t sw_mode1_lds(int maxblk);
+int sw_mode1_gathered(hipStream_t s, const SwBlockTable& T, int N, const SwColumns& cols,
+                      double tol, const double* d_masses, const float* phiT, int Kc, __bf16* U);
+
 // ---- mcc3 (mcc3.hip) --------------------------------------------------------------
 // Kernel family and launch sizing of one mcc3: variant 2 = FP32-operand kernels (any n),
 // 3 = pre-split BF16x3 kernels (BF16X3 with n % 4 == 0, n >= 16), 0 = default (3 when it

@@ -80,6 +80,8 @@ SIGNATURES = {
                               _dp, _vp, _vp]),
     "fc3_third_order_mcc3_lj": (_c_int, [_handle, _dp, _c_int, _dp, _c_double, _c_double,
                                          _c_double, _c_double, _c_double, _dp, _vp, _vp]),
+    "fc3_third_order_mcc3_sw": (_c_int, [_handle, _dp, _c_int, _dp, _dp, _c_double, _c_double,
+                                         _dp, _vp, _vp]),
     "fc3_mass_weight_third_order": (_c_int, [_handle, _vp, _c_int, _c_int, _dp]),
     "fc3_asr_satisfied": (_c_int, [_handle, _vp, _c_int, _c_int, _c_int, _c_double,
                                    ctypes.POINTER(ctypes.c_longlong),

@@ -311,12 +311,15 @@ def third_order_device(sys: SuperCellSystem, pot, calc, psi64: Optional[DeviceAr
 def third_order_mcc3(sys: SuperCellSystem, pot, calc, phi: DeviceArray, masses=None,
                      out: Optional[DeviceArray] = None, ctx: Optional[Context] = None
                      ) -> DeviceArray:
-    """K3 = mcc3(mass-weighted FP32 third_order(sys, LJ, calc), phi) without forming Psi:
-    the mode-1 product over Psi's nonzeros straight from the pair blocks, modes 2 and 3
-    in BF16x3 (fc3_third_order_mcc3_lj).  LJ with the analytical calculator only;
-    3N % 4 == 0."""
+    """K3 = mcc3(mass-weighted FP32 third_order(sys, pot, calc), phi) without forming Psi:
+    the mode-1 product over Psi's nonzeros (LJ: straight from the pair blocks,
+    fc3_third_order_mcc3_lj; SW: from the assembly's reduced block table,
+    fc3_third_order_mcc3_sw), modes 2 and 3 in BF16x3.  LJ with the analytical calculator
+    or SW with the autodiff one; 3N % 4 == 0."""
     ctx = ctx or Context.default()
-    if not (isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)):
+    is_lj = isinstance(pot, LJ) and isinstance(calc, AnalyticalCalculator)
+    is_sw = isinstance(pot, StillingerWeberSilicon) and isinstance(calc, AutoDiffCalculator)
+    if not (is_lj or is_sw):
         raise NotImplementedError(f"third_order_mcc3 for {type(pot).__name__} with {type(calc).__name__}")
     n = sys.n_dof
     _check_buf(phi, (n, n), np.float32, "third_order_mcc3: phi", ctx)
@@ -325,9 +328,15 @@ def third_order_mcc3(sys: SuperCellSystem, pot, calc, phi: DeviceArray, masses=N
     k3 = out if out is not None else DeviceArray((n, n, n), np.float32, ctx)
     pos, box = _sys_args(sys)
     m = None if masses is None else _check_masses(masses, sys.n_atoms)
-    check(ctx.lib.fc3_third_order_mcc3_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
-                                          pot.epsilon, pot.r_cut, calc.r_cut, calc.tol,
-                                          _ptr(m) if m is not None else None, phi.ptr, k3.ptr))
+    mp = _ptr(m) if m is not None else None
+    if is_lj:
+        check(ctx.lib.fc3_third_order_mcc3_lj(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), pot.sigma,
+                                              pot.epsilon, pot.r_cut, calc.r_cut, calc.tol, mp,
+                                              phi.ptr, k3.ptr))
+    else:
+        prm = pot.params.as_abi_array()
+        check(ctx.lib.fc3_third_order_mcc3_sw(ctx.h, _ptr(pos), sys.n_atoms, _ptr(box), _ptr(prm),
+                                              calc.r_cut, calc.tol, mp, phi.ptr, k3.ptr))
     return k3
 
 

@@ -186,6 +186,20 @@ int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms,
                        const double box[3], const double* sw_params, double r_cut,
                        double tol, const double* masses, double* psi64, float* psi32);
 
+/* Fused third_order + mass weighting + mcc3 for SW silicon without the dense tensor: the
+ * K3 = mcc3(Psi32, phi) that fc3_third_order_sw (psi32, masses) followed by fc3_mcc3
+ * (FC3_PREC_BF16X3) gives, with the mode-1 product taken over the nonzero 3x3x3 blocks of
+ * the assembly's reduced block table (FP32; the values are exactly those psi32 would
+ * hold), mode 2 over each atom's column atoms (those sharing a triplet with it) and
+ * mode 3 on the BF16x3 kernels.  No counterpart in the reference; it follows the sparse
+ * mode-1 intent of src/todo/old_mcc.jl:17-65, as fc3_third_order_mcc3_lj does for LJ.
+ * Neither n^3 Psi is formed.  Arguments as fc3_third_order_sw, with phi (device n x n
+ * FP32) and k3 (device n^3 FP32, 16-byte aligned) as fc3_third_order_mcc3_lj.
+ * Needs 3N % 4 == 0 and 3N >= 16 (FC3_ERR_UNSUPPORTED otherwise).                  */
+int fc3_third_order_mcc3_sw(fc3_handle h, const double* positions, int n_atoms,
+                            const double box[3], const double* sw_params, double r_cut,
+                            double tol, const double* masses, const float* phi, float* k3);
+
 /* mass_weight_third_order!(Psi, masses) replaces
  * src/analytical/third_order_analytical.jl:185-214: psi64 (device, n^3, n = dim *
  * n_atoms) is divided in place by sqrt(m_i m_j m_k) where nonzero.              */

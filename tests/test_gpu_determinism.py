@@ -142,6 +142,25 @@ def test_fused_is_deterministic(ctx, lj324):
     _same(runs[0], runs[1], f"fused third_order + mcc3 n={n}")
 
 
+@pytest.mark.parametrize("tol", [1e-12, 0.1])
+def test_fused_sw_is_deterministic(ctx, tol):
+    sysd = S.diamond(S.SI_A, (3, 3, 3), S.SI_MASS, 0.03, seed=648)
+    calc = S.AutoDiffCalculator(tol, SW.r_cut)
+    n = sysd.n_dof
+    phi = fc3.DeviceArray.from_host(
+        np.asfortranarray(np.random.default_rng(648).standard_normal((n, n)).astype(np.float32)),
+        ctx=ctx)
+    runs = []
+    for r in range(2):
+        out = _nan((n, n, n), np.float32, ctx)
+        fc3.third_order_mcc3(sysd, SW, calc, phi, masses=sysd.masses, out=out, ctx=ctx)
+        runs.append(out.to_host())
+        out.free()
+        _other_work(ctx)
+    phi.free()
+    _same(runs[0], runs[1], f"fused SW third_order + mcc3 n={n} tol {tol}")
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 def test_adjoints_are_deterministic(ctx, lj324, precision):
     sysd, calc, phi = lj324

@@ -1,4 +1,5 @@
-"""Fused LJ third_order + mcc3 without the dense tensor (fc3_third_order_mcc3_lj) on the GPU.
+"""Fused third_order + mcc3 without the dense tensor on the GPU: LJ
+(fc3_third_order_mcc3_lj) and Stillinger-Weber (fc3_third_order_mcc3_sw).
 
 Pins:
   * against the FP64 oracle: K3 = mcc3 of the oracle's mass-weighted Psi (the reference
@@ -9,6 +10,8 @@ Pins:
     is evaluated (over the nonzeros in FP32, or densely in BF16x3).
 Sizes cover K padding (n = 324 -> 336) and none (n = 96, 768), with and without
 masses, and a tolerance that zeroes entries (1e-8, as the reference's "LJ Values" test).
+The SW cases hold the same two bars on silicon diamond 2x2x2 and 3x3x3 (192 and 648 DoF),
+with the tolerance 1e-12 and 0.1 (the reference's SW test value, which zeroes entries).
 """
 import numpy as np
 import pytest
@@ -21,6 +24,7 @@ from oracle import fc3_oracle as O
 pytestmark = pytest.mark.gpu
 
 LJ = S.ARGON_LJ
+SW = S.StillingerWeberSilicon()
 
 
 def _rel(a, b):
@@ -92,3 +96,64 @@ def test_fused_rejects_unsupported_size(ctx):
     dphi = fc3.DeviceArray.from_host(np.eye(93, dtype=np.float32, order="F"), ctx=ctx)
     with pytest.raises(FC3Error):
         fc3.third_order_mcc3(sysd, LJ, calc, dphi, ctx=ctx)
+
+
+def _sw_case(ctx, cells, seed, su, tol):
+    sysd = S.diamond(S.SI_A, cells, S.SI_MASS, su, seed=seed)
+    calc = S.AutoDiffCalculator(tol, SW.r_cut)
+    f2 = O.sw_second_order(sysd, SW, S.AutoDiffCalculator(1e-12, SW.r_cut))
+    _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    return sysd, calc, phi32, fc3.DeviceArray.from_host(phi32, ctx=ctx)
+
+
+@pytest.mark.parametrize("cells,seed,su,tol,masses", [
+    ((2, 2, 2), 51, 0.03, 1e-12, "si"),
+    ((2, 2, 2), 52, 0.03, 0.1, None),
+    ((2, 2, 2), 53, 0.0, 1e-12, "random"),
+    ((3, 3, 3), 54, 0.03, 1e-12, "si"),
+    ((3, 3, 3), 55, 0.03, 0.1, "si"),
+])
+def test_fused_sw_matches_fp64_oracle(ctx, cells, seed, su, tol, masses):
+    sysd, calc, phi32, dphi = _sw_case(ctx, cells, seed, su, tol)
+    if masses == "si":
+        m = sysd.masses
+    elif masses == "random":
+        m = np.random.default_rng(seed).uniform(20.0, 40.0, size=sysd.n_atoms)
+    else:
+        m = None
+    k3 = fc3.third_order_mcc3(sysd, SW, calc, dphi, masses=m, ctx=ctx).to_host()
+    psi = O.sw_third_order(sysd, SW, calc)
+    if m is not None:
+        O.mass_weight_third_order(psi, m)
+    ref = O.mcc3_fp64(psi.astype(np.float32), phi32)
+    err = _rel(k3, ref)
+    print(f"SW {cells} seed {seed} tol {tol} masses {masses}: rel fro err vs FP64 {err:.3e}",
+          flush=True)
+    assert err <= 1e-5
+    dphi.free()
+
+
+@pytest.mark.parametrize("cells,seed,tol", [((2, 2, 2), 56, 1e-12), ((3, 3, 3), 57, 1e-12),
+                                            ((3, 3, 3), 58, 0.1)])
+def test_fused_sw_matches_dense_path(ctx, cells, seed, tol):
+    sysd, calc, phi32, dphi = _sw_case(ctx, cells, seed, 0.03, tol)
+    n = sysd.n_dof
+    k3f = fc3.third_order_mcc3(sysd, SW, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, SW, calc, psi32=psi32, masses=sysd.masses, ctx=ctx)
+    k3d = fc3.mcc3(psi32, dphi, precision="bf16x3").to_host()
+    psi32.free()
+    err = _rel(k3f, k3d.astype(np.float64))
+    print(f"SW {cells} (n={n}) tol {tol}: fused vs dense BF16x3 rel {err:.3e}", flush=True)
+    assert err <= 1e-5
+    dphi.free()
+
+
+def test_fused_sw_rejects_bad_cutoff(ctx):
+    sysd = S.diamond(S.SI_A, (2, 2, 2), S.SI_MASS)
+    n = sysd.n_dof
+    dphi = fc3.DeviceArray.from_host(np.eye(n, dtype=np.float32, order="F"), ctx=ctx)
+    with pytest.raises(AssertionError):
+        fc3.third_order_mcc3(sysd, SW, S.AutoDiffCalculator(1e-12, SW.r_cut + 0.1), dphi, ctx=ctx)
+    dphi.free()

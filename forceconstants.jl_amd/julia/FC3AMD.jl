@@ -34,7 +34,7 @@ import ForceConstants
 
 export DeviceArray, second_order, third_order, mass_weight_third_order!, mcc3, mcc3!,
        mcc3_unique, asr_satisfied, save_second_order, save_third_order, set_precision!,
-       get_modes_device, weighted_square_loss, FP32, BF16X3
+       get_modes_device, weighted_square_loss, third_order_mcc3, FP32, BF16X3
 
 const libfc3 = get(ENV, "FC3_LIBRARY", joinpath(@__DIR__, "..", "fc3", "libfc3.so"))
 
@@ -328,6 +328,45 @@ function weighted_square_loss(K3::DeviceArray{Float32}, seed::Integer; grad::Boo
                 grad ? dK3.ptr : Ptr{Float32}(C_NULL), L))
     return L[], dK3
 end
+
+"""
+    third_order_mcc3(sys, pot, calc, phi; masses = nothing) -> DeviceArray{Float32,3}
+
+K3 = mcc3(Float32.(mass_weight_third_order!(third_order(sys, pot, calc), masses)), phi)
+without forming the n^3 Psi (the sparse mode-1 intent of src/todo/old_mcc.jl:17-65): mode 1
+runs over Psi's nonzero 3x3x3 blocks as the assembly produces them, modes 2 and 3 in
+BF16X3 whatever `set_precision!` says.  `masses = nothing` leaves Psi unweighted.  Needs
+3N % 4 == 0; LJ with AnalyticalCalculator or StillingerWeberSilicon with AutoDiffCalculator.
+"""
+function third_order_mcc3(sys::SuperCellSystem{3}, pot::LJ, calc::AnalyticalCalculator,
+                          phi::DeviceArray{Float32, 2}; masses = nothing)
+    lu, eu = length_unit(pot), energy_unit(pot)
+    n = 3 * n_atoms(sys)
+    K3 = DeviceArray{Float32}(phi.h, n, n, n)
+    m = masses === nothing ? C_NULL : mass_vector(masses)
+    check(ccall((:fc3_third_order_mcc3_lj, libfc3), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Float64}, Float64, Float64, Float64,
+                 Float64, Float64, Ptr{Float64}, Ptr{Float32}, Ptr{Float32}),
+                phi.h.ptr, positions_matrix(sys, lu), n_atoms(sys), box3(sys, lu),
+                ustrip(lu, pot.σ), ustrip(eu, pot.ϵ), ustrip(lu, pot.r_cut),
+                ustrip(lu, calc.r_cut), calc.tol, m, phi.ptr, K3.ptr))
+    return K3
+end
+function third_order_mcc3(sys::SuperCellSystem{3}, pot::StillingerWeberSilicon,
+                          calc::AutoDiffCalculator, phi::DeviceArray{Float32, 2};
+                          masses = nothing)
+    lu = length_unit(pot)
+    n = 3 * n_atoms(sys)
+    K3 = DeviceArray{Float32}(phi.h, n, n, n)
+    m = masses === nothing ? C_NULL : mass_vector(masses)
+    check(ccall((:fc3_third_order_mcc3_sw, libfc3), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Float64, Float64,
+                 Ptr{Float64}, Ptr{Float32}, Ptr{Float32}),
+                phi.h.ptr, positions_matrix(sys, lu), n_atoms(sys), box3(sys, lu),
+                sw_abi_params(pot), ustrip(lu, calc.r_cut), calc.tol, m, phi.ptr, K3.ptr))
+    return K3
+end
+third_order_mcc3(sys, pot, calc, phi; masses = nothing) = unsupported(pot, calc)
 
 # get_modes(dynamical_matrix(sys, pot, calc)) (src/dynmat.jl:5-33,123-131) on the device,
 # phi left there in FP32 for mcc3

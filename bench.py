@@ -243,7 +243,8 @@ def sw_assembly(ctx, steps=3):
 
 def sw_fused(ctx, steps=3, cells=4):
     """Stillinger-Weber third_order + mcc3 on diamond 4x4x4 (1536 DoF, sigma_u = 0.03 A,
-    seed 1536), phi from the library's dynamical matrix and host eigensolver: the fused
+    seed 1536; cells=3: 648 DoF, seed 648), phi from the library's dynamical matrix and
+    host eigensolver: the fused
     call (mode 1 over the reduced block table's nonzeros, modes 2 and 3 BF16x3, no n^3
     Psi) against the dense route (FP32 mass-weighted Psi, then the BF16x3 mcc3), with the
     relative difference of the two K3 on 4 sampled slabs."""
@@ -807,6 +808,7 @@ def main():
     if not a.no_sw:
         out["sw_assembly"] = sw_assembly(ctx)
         out["sw_fused"] = sw_fused(ctx)
+        out["sw_fused_648"] = sw_fused(ctx, steps=10, cells=3)
     if not a.no_cpu_baseline:
         log("cpu baselines")
         out["cpu_baseline"] = cpu_baseline_mcc3(coo32, phi32, n)

@@ -29,60 +29,91 @@ __device__ __forceinline__ int lower_bound_key(const u64* a, int n, u64 key)
     return lo;
 }
 
-// Thread per slab atom K: the number of column atoms J != K and of blocks of slab K, with
-// their maxima over K in maxes[0], maxes[1] (integer atomics: the result does not depend
-// on the order).
-__global__ void k_sw_col_counts(const u64* __restrict__ ukey, int U, int N,
-                                int* __restrict__ colcnt, int* __restrict__ maxes)
+__device__ __forceinline__ int key_j(const u64* ukey, int p, int N)
 {
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K >= N) return;
-    const u64 NN = (u64)N * N;
-    const int lo = lower_bound_key(ukey, U, (u64)K * NN);
-    const int hi = lower_bound_key(ukey, U, (u64)(K + 1) * NN);
-    int cnt = 0, last = -1;
-    for (int p = lo; p < hi; ++p) {
-        const int J = (int)((ukey[p] / (u64)N) % (u64)N);
-        if (J != last) {
-            if (J != K) ++cnt;
-            last = J;
-        }
-    }
-    colcnt[K] = cnt;
-    atomicMax(&maxes[0], cnt);
-    atomicMax(&maxes[1], hi - lo);
+    return (int)((ukey[p] / (u64)N) % (u64)N);
 }
 
-// Thread per slab atom K: the column list (stride cap2) and the per-slot block ranges
-// (2 (cap2 + 1) ints per K; slots past colcnt[K] are empty).
-__global__ void k_sw_col_lists(const u64* __restrict__ ukey, int U, int N, int cap2,
-                               int* __restrict__ collist, int* __restrict__ seg)
+// Wave per slab atom K (4 per workgroup): the number of column atoms J != K and of blocks
+// of slab K, with their maxima over K in maxes[0], maxes[1] (integer atomics: the result
+// does not depend on the order).  The wave walks the slab's blocks 64 at a time; a block
+// opens a column atom when its J differs from the previous block's.
+__global__ void __launch_bounds__(256) k_sw_col_counts(const u64* __restrict__ ukey, int U, int N,
+                                                       int* __restrict__ colcnt,
+                                                       int* __restrict__ maxes)
 {
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K >= N) return;
+    const int K = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (K >= N) return;   // wave-uniform
     const u64 NN = (u64)N * N;
     const int lo = lower_bound_key(ukey, U, (u64)K * NN);
     const int hi = lower_bound_key(ukey, U, (u64)(K + 1) * NN);
+    int cnt = 0;
+    for (int base = lo; base < hi; base += 64) {
+        const int p = base + lane;
+        bool open = false;
+        if (p < hi) {
+            const int J = key_j(ukey, p, N);
+            open = J != K && (p == lo || key_j(ukey, p - 1, N) != J);
+        }
+        cnt += __popcll(__ballot(open));
+    }
+    if (lane == 0) {
+        colcnt[K] = cnt;
+        atomicMax(&maxes[0], cnt);
+        atomicMax(&maxes[1], hi - lo);
+    }
+}
+
+// Wave per slab atom K: the column list (stride cap2) and the per-slot block ranges
+// (2 (cap2 + 1) ints per K; slot 0 = J = K, slots past colcnt[K] are empty [lo, lo)).
+// Every int of the slab's range table is written exactly once.
+__global__ void __launch_bounds__(256) k_sw_col_lists(const u64* __restrict__ ukey, int U, int N,
+                                                      int cap2, const int* __restrict__ colcnt,
+                                                      int* __restrict__ collist,
+                                                      int* __restrict__ seg)
+{
+    const int K = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (K >= N) return;   // wave-uniform
+    const u64 NN = (u64)N * N;
+    const int lo = lower_bound_key(ukey, U, (u64)K * NN);
+    const int hi = lower_bound_key(ukey, U, (u64)(K + 1) * NN);
+    const int cnt = colcnt[K];
     int* sg = seg + (size_t)K * 2 * (cap2 + 1);
-    for (int s = 0; s <= cap2; ++s) {
+    for (int s = cnt + 1 + lane; s <= cap2; s += 64) {   // empty slots
         sg[2 * s] = lo;
         sg[2 * s + 1] = lo;
     }
-    int s = 0, last = -1;
-    for (int p = lo; p < hi; ++p) {
-        const int J = (int)((ukey[p] / (u64)N) % (u64)N);
-        if (J == last) continue;
-        last = J;
-        int q = lower_bound_key(ukey + p, hi - p, ((u64)K * N + J + 1) * (u64)N) + p;
-        if (J == K) {
-            sg[0] = p;
-            sg[1] = q;
-        } else {
-            ++s;
-            collist[(size_t)K * cap2 + s - 1] = J;
-            sg[2 * s] = p;
-            sg[2 * s + 1] = q;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int slots = 0;
+    bool self = false;
+    for (int base = lo; base < hi; base += 64) {
+        const int p = base + lane;
+        int J = -1;
+        bool open = false;
+        if (p < hi) {
+            J = key_j(ukey, p, N);
+            open = p == lo || key_j(ukey, p - 1, N) != J;
         }
+        const unsigned long long other = __ballot(open && J != K);
+        self = self || __ballot(open && J == K) != 0;
+        if (open) {
+            // the column atom's blocks end where the next J's begin
+            const int q = lower_bound_key(ukey + p, hi - p, ((u64)K * N + J + 1) * (u64)N) + p;
+            if (J == K) {
+                sg[0] = p;
+                sg[1] = q;
+            } else {
+                const int s = slots + __popcll(other & below) + 1;
+                collist[(size_t)K * cap2 + s - 1] = J;
+                sg[2 * s] = p;
+                sg[2 * s + 1] = q;
+            }
+        }
+        slots += __popcll(other);
+    }
+    if (!self && lane == 0) {   // an atom without blocks of its own
+        sg[0] = lo;
+        sg[1] = lo;
     }
 }
 
@@ -210,7 +241,7 @@ int sw_fused_columns(hipStream_t s, const SwBlockTable& T, int N, int cap2_ub, v
     int* maxes = cv.take<int>(2);
     if (!cv.ok()) return set_error(FC3_ERR_OUT_OF_MEMORY, "fused SW columns workspace");
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, maxes, 2);
-    hipLaunchKernelGGL(k_sw_col_counts, dim3(nblocks(N, 256)), dim3(256), 0, s, T.ukey, T.U, N,
+    hipLaunchKernelGGL(k_sw_col_counts, dim3(nblocks(N, 4)), dim3(256), 0, s, T.ukey, T.U, N,
                        out.colcnt, maxes);
     FC3_HIP(hipGetLastError());
     int hm[2] = {0, 0};
@@ -222,8 +253,8 @@ int sw_fused_columns(hipStream_t s, const SwBlockTable& T, int N, int cap2_ub, v
     out.maxblk = hm[1];
     out.collist = list_space;
     out.seg = seg_space;
-    hipLaunchKernelGGL(k_sw_col_lists, dim3(nblocks(N, 256)), dim3(256), 0, s, T.ukey, T.U, N,
-                       out.cap2, out.collist, out.seg);
+    hipLaunchKernelGGL(k_sw_col_lists, dim3(nblocks(N, 4)), dim3(256), 0, s, T.ukey, T.U, N,
+                       out.cap2, out.colcnt, out.collist, out.seg);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }

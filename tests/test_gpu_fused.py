@@ -113,6 +113,8 @@ def _sw_case(ctx, cells, seed, su, tol):
     ((2, 2, 2), 53, 0.0, 1e-12, "random"),
     ((3, 3, 3), 54, 0.03, 1e-12, "si"),
     ((3, 3, 3), 55, 0.03, 0.1, "si"),
+    ((1, 1, 1), 59, 0.03, 1e-12, "si"),      # 24 DoF: every atom's second shell wraps
+    ((2, 2, 2), 60, 0.2, 1e-12, "random"),   # up to 14 neighbours, 484 blocks per slab
 ])
 def test_fused_sw_matches_fp64_oracle(ctx, cells, seed, su, tol, masses):
     sysd, calc, phi32, dphi = _sw_case(ctx, cells, seed, su, tol)

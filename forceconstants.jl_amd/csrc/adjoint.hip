@@ -425,11 +425,11 @@ int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const floa
             FC3_BLAS(rocblas_sgemm(b, N_, T_, n, n, (int)n2, &one, psi, n, T2, n, &zero, dphi, n));
         }
         // Y = H13 with its first two indices swapped (Y_k = H13_k^T):
-        // dphi += sum_k Psi_k^T Y_k^T, i.e. the first form with both slab-transposed
+        // dphi(a, b) += sum_{i,k} Psi(i, a, k) Y(b, i, k), the first form with both
+        // slab-transposed; the contraction's split reads Psi in that order directly
         FC3_TRY(mcc3_modes31_t(s, G, phiT, n, T2, rest, rest_bytes));
         if (cv3) {
-            FC3_TRY(transpose_slabs(s, psi, n, T1));
-            FC3_TRY(mcc3_contract(s, 0, T1, T2, n, T1, &zc, rest, rest_bytes));
+            FC3_TRY(mcc3_contract(s, 2, psi, T2, n, T1, &zc, rest, rest_bytes));
         } else {
             FC3_BLAS(rocblas_sgemm_strided_batched(b, T_, T_, n, n, n, &one, psi, n, n2, T2, n,
                                                    n2, &zero, T1, n, n2, n));

@@ -268,6 +268,12 @@ int mcc3_blocked_slabs(hipStream_t s, const float* psi, const float* phi, int n,
                        int l0, int cnt, float* out, void* ws, size_t ws_bytes);
 // out[j + n i] = in[i + n j] (device, n x n)
 int transpose_square(hipStream_t s, const float* in, int n, float* out);
+// P_z = partial sums over K chunks z < *zc of the n x n x n^2 contraction
+// sum_e A(a, e) B(b, e) (BF16X3, split K), A from X and B from H, with the K index e:
+//   form 0: A(a, e) = X[a + n e], B(b, e) = H[b + n e]   (e over the last two indices)
+//   form 1: A(a, e) = X[e + n^2 a], B likewise from H    (e over the first two)
+//   form 2: A(a, e) = X[i + n a + n^2 k] for e = i + n k (e over the first and last),
+//           B as form 0
 bool mcc3_contract_applies(int n);
 size_t mcc3_contract_bytes(int n);
 int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n, float* P,

@@ -1382,16 +1382,20 @@ __global__ void __launch_bounds__(256) k_split_gather(const float* __restrict__ 
 // Per slab z = blockIdx.z: a rows x K matrix with K contiguous, src[z src_zs + e + ld r]
 // (K % 4 == 0, ld % 4 == 0) -> the K-blocked operand Y[z y_zs + (e / 16) 16 rows + 16 r +
 // e % 16] (hi), lo at + lo_off; past K nothing is written (a last partial K block keeps
-// whatever its padding holds).
+// whatever its padding holds).  FOLD: K runs over kn-long segments kfs apart, element
+// (r, e) at src[z src_zs + e % kn + kfs (e / kn) + ld r] (kn % 4 == 0, so no float4
+// crosses a segment); for a tensor X[i + n r + n^2 k] with kn = n, kfs = n^2 the K index
+// is e = i + n k.
 // Block per tile of 128 rows x 64 e (4 K blocks), staged through LDS: the loads are float4
 // runs along each row, and each K block's 128 rows go out as contiguous 16-byte pieces
 // (a wave stores 1 KB runs), where a thread per 4 values would scatter 8-byte stores
 // 16 rows apart.
 constexpr int kSrRows = 128, kSrE = 64;
+template <bool FOLD>
 __global__ void __launch_bounds__(256) k_split_staged(const float* __restrict__ src, long src_zs,
                                                        long ld, long K, long rows,
                                                        __bf16* __restrict__ Y, long y_zs,
-                                                       long lo_off)
+                                                       long lo_off, long kn, long kfs)
 {
     __shared__ float t[kSrRows][kSrE + 4];
     const long e0 = (long)blockIdx.x * kSrE, r0 = (long)blockIdx.y * kSrRows;
@@ -1402,9 +1406,11 @@ __global__ void __launch_bounds__(256) k_split_staged(const float* __restrict__ 
     for (int it = 0; it < kSrRows * kSrE / 4 / 256; ++it) {
         const int idx = it * 256 + tid, rr = idx / (kSrE / 4), c4 = idx % (kSrE / 4);
         const long r = r0 + rr, e = e0 + 4 * c4;
-        if (r < rows && e < K)   // K % 4 == 0: the 4 values are all inside
+        if (r < rows && e < K) {   // K % 4 == 0: the 4 values are all inside
+            const long ea = FOLD ? e % kn + kfs * (e / kn) : e;
             *reinterpret_cast<float4*>(&t[rr][4 * c4]) =
-                *reinterpret_cast<const float4*>(src + e + ld * r);
+                *reinterpret_cast<const float4*>(src + ea + ld * r);
+        }
     }
     __syncthreads();
     // item = (K block kb of the tile, row rr, half h): 8 values -> 16 bytes per plane
@@ -1625,16 +1631,23 @@ int zero_kpad(hipStream_t s, __bf16* base, int n, long slabs)
 }
 
 // k_split_staged over `slabs` slabs (see there); K % 4 == 0, ld % 4 == 0, src 16-byte aligned.
+// kn > 0: K folded over kn-long segments kfs apart (k_split_staged<true>)
 int split_staged(hipStream_t s, const float* src, long src_zs, long ld, long K, long rows,
-                 long slabs, __bf16* Y, long y_zs, long lo_off)
+                 long slabs, __bf16* Y, long y_zs, long lo_off, long kn = 0, long kfs = 0)
 {
     if (slabs < 1 || slabs > 65535 || (rows + v3::kSrRows - 1) / v3::kSrRows > 65535 ||
         (K + v3::kSrE - 1) / v3::kSrE > 0x7fffffffL)
         return set_error(FC3_ERR_UNSUPPORTED, "mcc3: split grid out of range");
+    if (kn != 0 && (kn < 0 || kn % 4 != 0))
+        return set_error(FC3_ERR_ARGUMENT, "mcc3: folded split segment must be a multiple of 4");
     const dim3 grid((unsigned)((K + v3::kSrE - 1) / v3::kSrE),
                     (unsigned)((rows + v3::kSrRows - 1) / v3::kSrRows), (unsigned)slabs);
-    hipLaunchKernelGGL(v3::k_split_staged, grid, dim3(256), 0, s, src, src_zs, ld, K, rows, Y, y_zs,
-                       lo_off);
+    if (kn)
+        hipLaunchKernelGGL(v3::k_split_staged<true>, grid, dim3(256), 0, s, src, src_zs, ld, K,
+                           rows, Y, y_zs, lo_off, kn, kfs);
+    else
+        hipLaunchKernelGGL(v3::k_split_staged<false>, grid, dim3(256), 0, s, src, src_zs, ld, K,
+                           rows, Y, y_zs, lo_off, K, 0L);
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }
@@ -2404,11 +2417,15 @@ int mcc3_contract(hipStream_t s, int form, const float* X, const float* H, int n
     for (int t = 0; t < 2; ++t) {
         const float* src = t == 0 ? X : H;
         __bf16* dst = t == 0 ? SA : SB;
-        if (form == 0) {
+        if (form == 0 || (form == 2 && t == 1)) {
             // rows r < n, K index e < n^2 at stride n: src[r + n e]
             const dim3 gt((unsigned)((n2 / 16) * ((n + 255) / 256)));
             hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, src, (long)n, (int)n2,
                                (int)n2, 0L, (long)n, dst, (unsigned char*)nullptr, 0);
+        } else if (form == 2) {
+            // rows r < n, K index e = i + n k: src[i + n r + n^2 k] (form 0 of the tensor
+            // with its first two indices swapped, without forming it)
+            FC3_TRY(split_staged(s, src, 0, n, n2, n, 1, dst, 0, (long)n * n2, n, n2));
         } else {
             // rows r < n, K index e < n^2 contiguous: src[e + n^2 r]
             FC3_TRY(split_staged(s, src, 0, n2, n2, n, 1, dst, 0, (long)n * n2));

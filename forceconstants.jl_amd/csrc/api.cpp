@@ -645,8 +645,10 @@ int fc3_third_order_mcc3_sw(fc3_handle h, const double* positions, int n_atoms,
     FC3_TRY(sw_fused_columns(h->stream, T, N, c2, lists, cols));
     const int Kc = mcc3_gather_k(cols.cap2);
     FC3_TRY(transpose_square(h->stream, phi, (int)n, phiT));
+    // `rest` is free until mcc3_from_gathered: it holds mode 1's coefficients when a slab's
+    // do not fit in LDS
     FC3_TRY(sw_mode1_gathered(h->stream, T, N, cols, tol, masses ? P.masses : nullptr, phiT, Kc,
-                              U));
+                              U, rest, cv.cap - cv.off));
     return mcc3_from_gathered(h->stream, U, phi, phiT, (int)n, cols.colcnt, cols.collist,
                               cols.cap2, k3, rest, cv.cap - cv.off);
 }

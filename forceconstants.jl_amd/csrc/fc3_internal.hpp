@@ -200,10 +200,14 @@ size_t sw_fused_list_bytes(int N, int cap2_ub);
 int sw_fused_columns(hipStream_t s, const SwBlockTable& T, int N, int cap2_ub, void* ws,
                      SwColumns& out);
 // The mode-1 product over the table's nonzeros (the FP32 writer's values: tolerance, mass
-// weighting), in mcc3_from_gathered's layout with Kc = mcc3_gather_k(cols.cap2).
+// weighting), in mcc3_from_gathered's layout with Kc = mcc3_gather_k(cols.cap2).  A slab
+// whose coefficients exceed 64 KB of LDS (sw_mode1_lds(cols.maxblk)) takes them from
+// global scratch instead: sw_mode1_global_bytes(T.U) of `scratch`, free again on return.
 size_t sw_mode1_lds(int maxblk);
+size_t sw_mode1_global_bytes(int U);
 int sw_mode1_gathered(hipStream_t s, const SwBlockTable& T, int N, const SwColumns& cols,
-                      double tol, const double* d_masses, const float* phiT, int Kc, __bf16* U);
+                      double tol, const double* d_masses, const float* phiT, int Kc, __bf16* U,
+                      void* scratch, size_t scratch_bytes);
 
 // ---- mcc3 (mcc3.hip) --------------------------------------------------------------
 // Kernel family and launch sizing of one mcc3: variant 2 = FP32-operand kernels (any n),

@@ -11,6 +11,8 @@
 // 1 .. colcnt[K] the other atoms J with a block (., J, K), ascending, at
 // collist[K cap2 + s - 1]; column t = 3 s + b.  seg[K][2 s], seg[K][2 s + 1] bound slot s's
 // blocks in the table (rows I ascending).
+#include <cstdlib>
+
 #include "bf16_split.hpp"
 #include "fc3_internal.hpp"
 
@@ -117,19 +119,56 @@ __global__ void __launch_bounds__(256) k_sw_col_lists(const u64* __restrict__ uk
     }
 }
 
+// Block p's coefficient e (= a + 3b + 9g) as the FP32 writer stores it: the tolerance,
+// then mass weighting v / sqrt((m_I m_J) m_K), rounded to FP32; *I = its row atom.
+template <bool MW>
+__device__ __forceinline__ float sw_coeff(const u64* __restrict__ ukey,
+                                          const double* __restrict__ uval, int p, int e, int N,
+                                          double tol, const double* __restrict__ masses, int* I)
+{
+    const u64 key = ukey[p];
+    *I = (int)(key % (u64)N);
+    double v = uval[(size_t)p * 27 + e];
+    if (fabs(v) < tol) v = 0.0;
+    float f = (float)v;
+    if (MW && v != 0.0) {
+        const int J = (int)((key / (u64)N) % (u64)N), K = (int)(key / ((u64)N * N));
+        f = (float)(v / sqrt(masses[*I] * masses[J] * masses[K]));
+    }
+    return f;
+}
+
+// Every block's coefficients into global memory (gcf[27 p + e], grow3[p] = 3 I), for slabs
+// whose blocks do not fit in LDS.  Thread per coefficient.
+template <bool MW>
+__global__ void __launch_bounds__(256) k_sw_coeffs(const u64* __restrict__ ukey,
+                                                   const double* __restrict__ uval, int U, int N,
+                                                   double tol, const double* __restrict__ masses,
+                                                   float* __restrict__ gcf,
+                                                   int* __restrict__ grow3)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 27L * U) return;
+    const int p = (int)(q / 27), e = (int)(q - 27L * p);
+    int I;
+    gcf[q] = sw_coeff<MW>(ukey, uval, p, e, N, tol, masses, &I);
+    if (e == 0) grow3[p] = 3 * I;
+}
+
 // U_K(g n + p, t) = sum over the blocks (I, J_s, K) of slot s = t / 3 (b = t % 3), rows
 // I ascending, of sum_a phiT[p + n (3I + a)] Psi32[3I + a, 3J_s + b, 3K + g], an fmaf chain
-// from zero, with Psi32 the value the FP32 writer stores (tolerance, then mass weighting
-// v / sqrt((m_I m_J) m_K), rounded to FP32).  Split into bf16 hi / lo, K-blocked over t:
-// hi at U[K 2 M Kc + (t / 16) 16 M + 16 (g n + p) + t % 16] with M = 3n, lo at + M Kc;
-// columns past the count are zero.  Workgroup per (256 rows p, atom K = blockIdx.y): the
-// slab's coefficients go to LDS once (dynamic: maxblk (27 floats + 1 int)).
-template <bool MW>
+// from zero, with Psi32 the value the FP32 writer stores (sw_coeff).  Split into bf16 hi /
+// lo, K-blocked over t: hi at U[K 2 M Kc + (t / 16) 16 M + 16 (g n + p) + t % 16] with
+// M = 3n, lo at + M Kc; columns past the count are zero.  Workgroup per (256 rows p, atom
+// K = blockIdx.y).  LDS: the slab's coefficients go to LDS once (dynamic: maxblk (27
+// floats + 1 int)); else they are read from k_sw_coeffs's gcf / grow3, the same values.
+template <bool MW, bool LDS>
 __global__ void __launch_bounds__(256)
 k_sw_mode1_gathered(const u64* __restrict__ ukey, const double* __restrict__ uval, int U, int N,
                     const int* __restrict__ colcnt, const int* __restrict__ seg, int cap2,
                     double tol, const double* __restrict__ masses, const float* __restrict__ phiT,
-                    int Kc, __bf16* __restrict__ Ug)
+                    int Kc, const float* __restrict__ gcf, const int* __restrict__ grow3,
+                    __bf16* __restrict__ Ug)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
     const int n = 3 * N;
@@ -143,22 +182,25 @@ k_sw_mode1_gathered(const u64* __restrict__ ukey, const double* __restrict__ uva
         lo = min(lo, sg[2 * s]);
         hi = max(hi, sg[2 * s + 1]);
     }
-    const int nb = hi - lo;
-    float* cf = reinterpret_cast<float*>(dyn);
-    int* row3 = reinterpret_cast<int*>(cf + 27 * (size_t)nb);
-    const double mK = MW ? masses[K] : 0.0;
-    for (int q = threadIdx.x; q < 27 * nb; q += blockDim.x) {
-        const int blk = q / 27, e = q - 27 * blk;
-        const u64 key = ukey[lo + blk];
-        const int I = (int)(key % (u64)N), J = (int)((key / (u64)N) % (u64)N);
-        double v = uval[(size_t)(lo + blk) * 27 + e];
-        if (fabs(v) < tol) v = 0.0;
-        float f = (float)v;
-        if (MW && v != 0.0) f = (float)(v / sqrt(masses[I] * masses[J] * mK));
-        cf[q] = f;
-        if (e == 0) row3[blk] = 3 * I;
+    const float* cf;
+    const int* row3;
+    if constexpr (LDS) {
+        const int nb = hi - lo;
+        float* lcf = reinterpret_cast<float*>(dyn);
+        int* lrow3 = reinterpret_cast<int*>(lcf + 27 * (size_t)nb);
+        for (int q = threadIdx.x; q < 27 * nb; q += blockDim.x) {
+            const int blk = q / 27, e = q - 27 * blk;
+            int I;
+            lcf[q] = sw_coeff<MW>(ukey, uval, lo + blk, e, N, tol, masses, &I);
+            if (e == 0) lrow3[blk] = 3 * I;
+        }
+        __syncthreads();
+        cf = lcf;
+        row3 = lrow3;
+    } else {
+        cf = gcf + 27 * (size_t)lo;
+        row3 = grow3 + lo;
     }
-    __syncthreads();
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;   // no barrier follows
     const float* ph = phiT + p;
@@ -261,23 +303,53 @@ int sw_fused_columns(hipStream_t s, const SwBlockTable& T, int N, int cap2_ub, v
 
 size_t sw_mode1_lds(int maxblk) { return (size_t)maxblk * (27 * sizeof(float) + sizeof(int)); }
 
+size_t sw_mode1_global_bytes(int U)
+{
+    return Carve::need(sizeof(float) * 27 * (size_t)U) + Carve::need(sizeof(int) * (size_t)U);
+}
+
 int sw_mode1_gathered(hipStream_t s, const SwBlockTable& T, int N, const SwColumns& cols,
-                      double tol, const double* d_masses, const float* phiT, int Kc, __bf16* U)
+                      double tol, const double* d_masses, const float* phiT, int Kc, __bf16* U,
+                      void* scratch, size_t scratch_bytes)
 {
     const int n = 3 * N;
-    const size_t lds = sw_mode1_lds(cols.maxblk);
-    if (lds > 64 * 1024)
-        return set_error(FC3_ERR_UNSUPPORTED, "fused SW mode 1: too many blocks per slab atom");
     if (Kc % 16 != 0 || Kc < 3 * (cols.cap2 + 1))
         return set_error(FC3_ERR_ARGUMENT, "fused SW mode 1: gathered K too small");
+    // FC3_SW_FUSED_GLOBAL=1 takes the global-coefficient kernel for any slab size (the two
+    // kernels multiply the same values in the same order)
+    const size_t lds = sw_mode1_lds(cols.maxblk);
+    const bool use_lds = lds <= 64 * 1024 && !std::getenv("FC3_SW_FUSED_GLOBAL");
+    float* gcf = nullptr;
+    int* grow3 = nullptr;
+    if (!use_lds) {
+        Carve cv(scratch, scratch_bytes);
+        gcf = cv.take<float>(27 * (size_t)T.U);
+        grow3 = cv.take<int>((size_t)T.U);
+        if (!cv.ok())
+            return set_error(FC3_ERR_OUT_OF_MEMORY, "fused SW mode 1: coefficient scratch");
+    }
     dim3 grid(nblocks(n, 256), (unsigned)N);
     ProfScope prof(FC3_KERNEL_IFC3_WRITE, s);
-    if (d_masses)
-        hipLaunchKernelGGL(k_sw_mode1_gathered<true>, grid, dim3(256), lds, s, T.ukey, T.uval, T.U,
-                           N, cols.colcnt, cols.seg, cols.cap2, tol, d_masses, phiT, Kc, U);
-    else
-        hipLaunchKernelGGL(k_sw_mode1_gathered<false>, grid, dim3(256), lds, s, T.ukey, T.uval,
-                           T.U, N, cols.colcnt, cols.seg, cols.cap2, tol, d_masses, phiT, Kc, U);
+    if (!use_lds && T.U > 0) {
+        const unsigned gq = nblocks(27L * T.U, 256);
+        if (d_masses)
+            hipLaunchKernelGGL(k_sw_coeffs<true>, dim3(gq), dim3(256), 0, s, T.ukey, T.uval, T.U,
+                               N, tol, d_masses, gcf, grow3);
+        else
+            hipLaunchKernelGGL(k_sw_coeffs<false>, dim3(gq), dim3(256), 0, s, T.ukey, T.uval, T.U,
+                               N, tol, d_masses, gcf, grow3);
+        FC3_HIP(hipGetLastError());
+    }
+#define FC3_SW_M1(MW, L)                                                                       \
+    hipLaunchKernelGGL((k_sw_mode1_gathered<MW, L>), grid, dim3(256), L ? lds : 0, s, T.ukey,   \
+                       T.uval, T.U, N, cols.colcnt, cols.seg, cols.cap2, tol, d_masses, phiT,   \
+                       Kc, gcf, grow3, U)
+    if (use_lds) {
+        if (d_masses) FC3_SW_M1(true, true); else FC3_SW_M1(false, true);
+    } else {
+        if (d_masses) FC3_SW_M1(true, false); else FC3_SW_M1(false, false);
+    }
+#undef FC3_SW_M1
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }

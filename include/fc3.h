@@ -195,9 +195,7 @@ int fc3_third_order_sw(fc3_handle h, const double* positions, int n_atoms,
  * mode-1 intent of src/todo/old_mcc.jl:17-65, as fc3_third_order_mcc3_lj does for LJ.
  * Neither n^3 Psi is formed.  Arguments as fc3_third_order_sw, with phi (device n x n
  * FP32) and k3 (device n^3 FP32, 16-byte aligned) as fc3_third_order_mcc3_lj.
- * Needs 3N % 4 == 0 and 3N >= 16, and at most 585 blocks (I, J, K) per atom K, whose
- * coefficients fill 64 KB of LDS (diamond: 212 at sigma_u = 0.03 A, 484 at 0.2 A);
- * FC3_ERR_UNSUPPORTED otherwise.                                                     */
+ * Needs 3N % 4 == 0 and 3N >= 16 (FC3_ERR_UNSUPPORTED otherwise).                  */
 int fc3_third_order_mcc3_sw(fc3_handle h, const double* positions, int n_atoms,
                             const double box[3], const double* sw_params, double r_cut,
                             double tol, const double* masses, const float* phi, float* k3);

@@ -152,6 +152,51 @@ def test_fused_sw_matches_dense_path(ctx, cells, seed, tol):
     dphi.free()
 
 
+def test_fused_sw_global_coefficients_bitwise(ctx, monkeypatch):
+    """Slabs whose block coefficients do not fit in LDS read them from global scratch
+    (FC3_SW_FUSED_GLOBAL=1 forces that kernel): the same values multiplied in the same
+    order, so K3 must agree bit for bit."""
+    sysd, calc, phi32, dphi = _sw_case(ctx, (3, 3, 3), 61, 0.03, 1e-12)
+    k_lds = fc3.third_order_mcc3(sysd, SW, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    monkeypatch.setenv("FC3_SW_FUSED_GLOBAL", "1")
+    k_glb = fc3.third_order_mcc3(sysd, SW, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    monkeypatch.delenv("FC3_SW_FUSED_GLOBAL")
+    diff = int(np.count_nonzero(k_lds.view(np.uint32) != k_glb.view(np.uint32)))
+    print(f"SW fused LDS vs global coefficients: {diff} entries differ", flush=True)
+    assert diff == 0
+    dphi.free()
+
+
+@pytest.mark.parametrize("a,masses", [(4.8, "si"), (4.6, None)])
+def test_fused_sw_compressed_diamond(ctx, a, masses):
+    """Diamond compressed to a = 4.8 / 4.6 A: 16 / 20 neighbours and 727 / 973 blocks per
+    slab atom, past what LDS holds (585), so mode 1 reads its coefficients from global
+    scratch.  Against the FP64 oracle and the dense route."""
+    sysd = S.diamond(a, (2, 2, 2), S.SI_MASS, 0.03, seed=62)
+    calc = S.AutoDiffCalculator(1e-12, SW.r_cut)
+    f2 = O.sw_second_order(sysd, SW, calc)
+    _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    dphi = fc3.DeviceArray.from_host(phi32, ctx=ctx)
+    m = sysd.masses if masses == "si" else None
+    k3 = fc3.third_order_mcc3(sysd, SW, calc, dphi, masses=m, ctx=ctx).to_host()
+    psi = O.sw_third_order(sysd, SW, calc)
+    if m is not None:
+        O.mass_weight_third_order(psi, m)
+    ref = O.mcc3_fp64(psi.astype(np.float32), phi32)
+    err = _rel(k3, ref)
+    n = sysd.n_dof
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, SW, calc, psi32=psi32, masses=m, ctx=ctx)
+    k3d = fc3.mcc3(psi32, dphi, precision="bf16x3").to_host()
+    psi32.free()
+    errd = _rel(k3, k3d.astype(np.float64))
+    print(f"SW compressed diamond a = {a}: rel fro err vs FP64 {err:.3e}, vs dense {errd:.3e}",
+          flush=True)
+    assert err <= 1e-5 and errd <= 1e-5
+    dphi.free()
+
+
 def test_fused_sw_rejects_bad_cutoff(ctx):
     sysd = S.diamond(S.SI_A, (2, 2, 2), S.SI_MASS)
     n = sysd.n_dof

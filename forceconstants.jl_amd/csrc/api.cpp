@@ -395,10 +395,15 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
     };
     // with room for the FP32 intermediate (mode 3 without a split pass) when the device
     // has it, else with the bounded split scratch (the same K3)
+    // (neighbourhoods past 64 KB of LDS: room for their coefficients in global memory,
+    // which mcc3_from_gathered's scratch holds until it starts)
     auto need = [&](int cap, bool direct) {
         return Carve::need(sizeof(double) * ((size_t)N * cap * (27 + 9) + (size_t)N * (27 + 9))) +
                Carve::need(sizeof(float) * (size_t)n * n) + u_bytes(cap) +
-               mcc3_from_gathered_bytes((int)n, cap, direct) + 4096;
+               std::max(mcc3_from_gathered_bytes((int)n, cap, direct),
+                        lj_mode1_sparse_lds(cap) > 64 * 1024 ? lj_mode1_global_bytes(N, cap)
+                                                             : size_t(0)) +
+               4096;
     };
     Prepared P;
     int rc = prepare(h, positions, N, box, r_cut, masses,
@@ -407,8 +412,6 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
         rc = prepare(h, positions, N, box, r_cut, masses,
                      [&](int cap) { return need(cap, false); }, P);
     FC3_TRY(rc);
-    if (lj_mode1_sparse_lds(P.nl.cap) > 64 * 1024)
-        return set_error(FC3_ERR_UNSUPPORTED, "fused mode 1: too many neighbours per atom");
     const int Kc = mcc3_gather_k(P.nl.cap);
     Carve cv(P.extra, P.extra_bytes);
     const size_t slots = (size_t)N * P.nl.cap;
@@ -424,7 +427,8 @@ int fc3_third_order_mcc3_lj(fc3_handle h, const double* positions, int n_atoms,
     b.E = E3;
     b.S = S3;
     FC3_TRY(transpose_square(h->stream, phi, (int)n, phiT));
-    FC3_TRY(lj_mode1_gathered(h->stream, P.nl, b, tol, masses ? P.masses : nullptr, phiT, Kc, U));
+    FC3_TRY(lj_mode1_gathered(h->stream, P.nl, b, tol, masses ? P.masses : nullptr, phiT, Kc, U,
+                              rest, cv.cap - cv.off));
     return mcc3_from_gathered(h->stream, U, phi, phiT, (int)n, P.nl.count, P.nl.nbr, P.nl.cap,
                               k3, rest, cv.cap - cv.off);
 }

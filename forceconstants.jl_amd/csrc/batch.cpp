@@ -196,7 +196,7 @@ int snapshot_work(hipStream_t s, char* base, const SnapPlan& P, const fc3_snapsh
         const size_t used = (size_t)(w - static_cast<char*>(mws));
         if (used > mws_bytes) return set_error(FC3_ERR_OUT_OF_MEMORY, "batch fused scratch");
         FC3_TRY(transpose_square(s, sn.phi, n, phiT));
-        FC3_TRY(lj_mode1_gathered(s, nl, b, prm.tol, m, phiT, Kc, U));
+        FC3_TRY(lj_mode1_gathered(s, nl, b, prm.tol, m, phiT, Kc, U, nullptr, 0));
         return mcc3_from_gathered(s, U, sn.phi, phiT, n, nl.count, nl.nbr, P.cap, sn.k3, w,
                                   mws_bytes - used);
     }

@@ -151,10 +151,14 @@ int mass_weight3(hipStream_t s, double* psi, int N, int D, const double* d_masse
 // only columns of those slabs that hold nonzeros; Psi32 exactly the FP32 writer's
 // values.  Split (bf16 hi/lo), K-blocked over t with Kc = mcc3_gather_k(cap) (zero
 // past the count): hi at U[K 2 (3n) Kc + (t / 16) 16 (3n) + 16 (g n + p) + t % 16], lo at
-// + 3n Kc.  Dynamic LDS of lj_mode1_sparse_lds(cap) bytes (at most 64 KB).
+// + 3n Kc.  Dynamic LDS of lj_mode1_sparse_lds(cap) bytes when that is at most 64 KB;
+// else the coefficients go to global memory first, lj_mode1_global_bytes(N, cap) of
+// `scratch` (free again on return; FC3_ERR_UNSUPPORTED without it).
 size_t lj_mode1_sparse_lds(int cap);
+size_t lj_mode1_global_bytes(int N, int cap);
 int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
-                      const double* d_masses, const float* phiT, int Kc, __bf16* U);
+                      const double* d_masses, const float* phiT, int Kc, __bf16* U,
+                      void* scratch, size_t scratch_bytes);
 
 // ---- Stillinger-Weber (sw.hip) ------------------------------------------------------
 // Workspace for one SW assembly with neighbour capacity `cap` (upper bound).

@@ -16,6 +16,7 @@
 // ncols*n range of Psi), first resolves each column's nonzero rows into LDS, then
 // streams the whole range with 16-byte stores (8-byte for FP32), zeros included:
 // HBM sees exactly 8 n^3 (+ 4 n^3) bytes of writes and nothing else.
+#include <cstdlib>
 #include "bf16_split.hpp"
 #include "fc3_internal.hpp"
 
@@ -234,26 +235,26 @@ k_mass_weight3(double* __restrict__ psi, int N, int D, const double* __restrict_
 // and FP32 rounding applied) go to LDS once per workgroup; a neighbour's three phi
 // values then serve its 3 columns x 3 slabs, and every U row is written once, as 32-byte
 // runs per (16 columns, slab) that are contiguous across the workgroup's lanes.
-// Dynamic LDS: lj_mode1_sparse_lds(cap) bytes.
+// Dynamic LDS: lj_mode1_sparse_lds(cap) bytes; with LDS = false the same coefficients
+// come from k_lj_mode1_coeffs's global copy instead (for neighbourhoods past 64 KB).
+//
+// Atom K's coefficients (per slot s' < count[K]: cJ[27], cK[27], cS[27] (floats), then
+// cSelf[27], then the slots' first rows 3 J (ints)) into coef / cSelf / jrow, by threads
+// tid, tid + stride, ...
 template <bool MW>
-__global__ void __launch_bounds__(256)
-k_lj_mode1_gathered(const int* __restrict__ count, const int* __restrict__ nbr,
-                    const int* __restrict__ rev, int N, int cap, const double* __restrict__ E,
-                    const double* __restrict__ S, double tol, const double* __restrict__ masses,
-                    const float* __restrict__ phiT, int Kc, __bf16* __restrict__ U)
+__device__ __forceinline__ void lj_stage_coeffs(int K, int tid, int stride,
+                                                const int* __restrict__ count,
+                                                const int* __restrict__ nbr,
+                                                const int* __restrict__ rev, int cap,
+                                                const double* __restrict__ E,
+                                                const double* __restrict__ S, double tol,
+                                                const double* __restrict__ masses, float* coef,
+                                                float* cSelf, int* jrow)
 {
-    extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
-    const int n = 3 * N;
-    const int K = blockIdx.y;
     const int cntK = count[K];
     const int* nbrK = nbr + (size_t)K * cap;
-    // LDS: per slot s' < cntK: cJ[27], cK[27], cS[27] (floats), then cSelf[27], then the
-    // slots' first rows 3 J (ints)
-    float* coef = reinterpret_cast<float*>(dyn);
-    float* cSelf = coef + 81 * cap;
-    int* jrow = reinterpret_cast<int*>(cSelf + 27);
     const double mK = MW ? masses[K] : 0.0;
-    for (int q = threadIdx.x; q < 27 * cntK + 27; q += blockDim.x) {
+    for (int q = tid; q < 27 * cntK + 27; q += stride) {
         if (q >= 27 * cntK) {
             const int e = q - 27 * cntK;
             const double v = tol_cut(S[(size_t)K * 27 + e], tol);
@@ -280,7 +281,52 @@ k_lj_mode1_gathered(const int* __restrict__ count, const int* __restrict__ nbr,
         coef[81 * sl + 54 + e] = fs;
         if (e == 0) jrow[sl] = 3 * J;
     }
-    __syncthreads();
+}
+
+// Floats of one atom's coefficient record in the global copy: 81 cap + 27, then cap ints
+// (the LDS layout), padded to 4 bytes x 4.
+__host__ __device__ inline long lj_coeff_stride(int cap) { return (81L * cap + 27 + cap + 3) / 4 * 4; }
+
+// Workgroup per atom K: its coefficients into gc + K lj_coeff_stride(cap).
+template <bool MW>
+__global__ void __launch_bounds__(256)
+k_lj_mode1_coeffs(const int* __restrict__ count, const int* __restrict__ nbr,
+                  const int* __restrict__ rev, int cap, const double* __restrict__ E,
+                  const double* __restrict__ S, double tol, const double* __restrict__ masses,
+                  float* __restrict__ gc)
+{
+    const int K = blockIdx.x;
+    float* coef = gc + (size_t)K * lj_coeff_stride(cap);
+    lj_stage_coeffs<MW>(K, threadIdx.x, blockDim.x, count, nbr, rev, cap, E, S, tol, masses,
+                        coef, coef + 81 * cap, reinterpret_cast<int*>(coef + 81 * cap + 27));
+}
+
+template <bool MW, bool LDS>
+__global__ void __launch_bounds__(256)
+k_lj_mode1_gathered(const int* __restrict__ count, const int* __restrict__ nbr,
+                    const int* __restrict__ rev, int N, int cap, const double* __restrict__ E,
+                    const double* __restrict__ S, double tol, const double* __restrict__ masses,
+                    const float* __restrict__ phiT, int Kc, const float* __restrict__ gc,
+                    __bf16* __restrict__ U)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
+    const int n = 3 * N;
+    const int K = blockIdx.y;
+    const int cntK = count[K];
+    const float* coef;
+    const float* cSelf;
+    const int* jrow;
+    if constexpr (LDS) {
+        float* lc = reinterpret_cast<float*>(dyn);
+        lj_stage_coeffs<MW>(K, threadIdx.x, blockDim.x, count, nbr, rev, cap, E, S, tol, masses,
+                            lc, lc + 81 * cap, reinterpret_cast<int*>(lc + 81 * cap + 27));
+        __syncthreads();
+        coef = lc;
+    } else {
+        coef = gc + (size_t)K * lj_coeff_stride(cap);
+    }
+    cSelf = coef + 81 * cap;
+    jrow = reinterpret_cast<const int*>(coef + 81 * cap + 27);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;   // no barrier follows
     const float* ph = phiT + p;
@@ -387,23 +433,50 @@ size_t lj_mode1_sparse_lds(int cap)
     return (size_t)(81 * cap + 27) * sizeof(float) + (size_t)cap * sizeof(int);
 }
 
+size_t lj_mode1_global_bytes(int N, int cap)
+{
+    return Carve::need(sizeof(float) * (size_t)N * lj_coeff_stride(cap));
+}
+
 int lj_mode1_gathered(hipStream_t s, const NeighborList& nl, const Ifc3Blocks& b, double tol,
-                      const double* d_masses, const float* phiT, int Kc, __bf16* U)
+                      const double* d_masses, const float* phiT, int Kc, __bf16* U,
+                      void* scratch, size_t scratch_bytes)
 {
     const int n = 3 * nl.n_atoms;
-    const size_t lds = lj_mode1_sparse_lds(nl.cap);
-    if (lds > 64 * 1024)
-        return set_error(FC3_ERR_UNSUPPORTED, "fused mode 1: too many neighbours per atom");
     if (Kc % 16 != 0 || Kc < 3 * (nl.cap + 1))
         return set_error(FC3_ERR_ARGUMENT, "fused mode 1: gathered K too small");
+    // FC3_LJ_FUSED_GLOBAL=1 takes the global-coefficient kernel for any neighbourhood (the
+    // two kernels multiply the same values in the same order)
+    const size_t lds = lj_mode1_sparse_lds(nl.cap);
+    const bool use_lds = lds <= 64 * 1024 && !(scratch && std::getenv("FC3_LJ_FUSED_GLOBAL"));
+    float* gc = nullptr;
+    if (!use_lds) {
+        if (scratch_bytes < lj_mode1_global_bytes(nl.n_atoms, nl.cap))
+            return set_error(scratch ? FC3_ERR_OUT_OF_MEMORY : FC3_ERR_UNSUPPORTED,
+                             "fused mode 1: no scratch for the neighbourhoods' coefficients");
+        gc = static_cast<float*>(scratch);
+    }
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)nl.n_atoms);
     ProfScope prof(FC3_KERNEL_IFC3_WRITE, s);
-    if (d_masses)
-        hipLaunchKernelGGL(k_lj_mode1_gathered<true>, grid, dim3(256), lds, s, nl.count, nl.nbr,
-                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kc, U);
-    else
-        hipLaunchKernelGGL(k_lj_mode1_gathered<false>, grid, dim3(256), lds, s, nl.count, nl.nbr,
-                           nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kc, U);
+    if (!use_lds) {
+        if (d_masses)
+            hipLaunchKernelGGL(k_lj_mode1_coeffs<true>, dim3((unsigned)nl.n_atoms), dim3(256), 0,
+                               s, nl.count, nl.nbr, nl.rev, nl.cap, b.E, b.S, tol, d_masses, gc);
+        else
+            hipLaunchKernelGGL(k_lj_mode1_coeffs<false>, dim3((unsigned)nl.n_atoms), dim3(256), 0,
+                               s, nl.count, nl.nbr, nl.rev, nl.cap, b.E, b.S, tol, d_masses, gc);
+        FC3_HIP(hipGetLastError());
+    }
+#define FC3_LJ_M1(MW, L)                                                                       \
+    hipLaunchKernelGGL((k_lj_mode1_gathered<MW, L>), grid, dim3(256), L ? lds : 0, s, nl.count, \
+                       nl.nbr, nl.rev, nl.n_atoms, nl.cap, b.E, b.S, tol, d_masses, phiT, Kc,   \
+                       gc, U)
+    if (use_lds) {
+        if (d_masses) FC3_LJ_M1(true, true); else FC3_LJ_M1(false, true);
+    } else {
+        if (d_masses) FC3_LJ_M1(true, false); else FC3_LJ_M1(false, false);
+    }
+#undef FC3_LJ_M1
     FC3_HIP(hipGetLastError());
     return FC3_OK;
 }

@@ -89,6 +89,54 @@ def test_fused_matches_dense_path(ctx, cells, seed):
     assert err <= 1e-5
 
 
+def test_fused_global_coefficients_bitwise(ctx, monkeypatch):
+    """Neighbourhoods whose coefficients do not fit in LDS read them from global scratch
+    (FC3_LJ_FUSED_GLOBAL=1 forces that kernel): the same values multiplied in the same
+    order, so K3 must agree bit for bit."""
+    sysd, calc, phi32, dphi = _case(ctx, (3, 3, 3), 49, 0.05, 1e-12)
+    k_lds = fc3.third_order_mcc3(sysd, LJ, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    monkeypatch.setenv("FC3_LJ_FUSED_GLOBAL", "1")
+    k_glb = fc3.third_order_mcc3(sysd, LJ, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    monkeypatch.delenv("FC3_LJ_FUSED_GLOBAL")
+    diff = int(np.count_nonzero(k_lds.view(np.uint32) != k_glb.view(np.uint32)))
+    print(f"LJ fused LDS vs global coefficients: {diff} entries differ", flush=True)
+    assert diff == 0
+    dphi.free()
+
+
+def test_fused_long_cutoff(ctx):
+    """r_cut = 12.5 A on FCC 4x4x4 with a = 4.4 A (768 DoF): 236 neighbours per atom, past
+    the 199 whose coefficients fit in LDS, so mode 1 reads them from global scratch.
+    Against the FP64 oracle on sampled slabs and against the dense route on the whole K3."""
+    pot = S.LJ(sigma=3.4, epsilon=0.24037, r_cut=12.5)
+    sysd = S.fcc(4.4, (4, 4, 4), S.ARGON_MASS, 0.05, seed=50)
+    calc = S.AnalyticalCalculator(1e-12, pot.r_cut)
+    n = sysd.n_dof
+    f2 = O.lj_second_order(sysd, pot, calc)
+    _, phi = O.get_modes(O.dynamical_matrix(f2, sysd.masses))
+    phi32 = np.asfortranarray(phi.astype(np.float32))
+    dphi = fc3.DeviceArray.from_host(phi32, ctx=ctx)
+    k3 = fc3.third_order_mcc3(sysd, pot, calc, dphi, masses=sysd.masses, ctx=ctx).to_host()
+    coo = O.lj_third_order_coo(sysd, pot, calc)
+    v32 = O.coo_mass_weight(coo, sysd.masses).astype(np.float32).astype(np.float64)
+    coo32 = (coo[0], coo[1], coo[2], v32)
+    num = den = 0.0
+    for l in (5, 200, 401, 767):
+        ref = O.mcc3_fp64_slab(coo32, phi32.astype(np.float64), l)
+        num += float(np.sum((k3[:, :, l].astype(np.float64) - ref) ** 2))
+        den += float(np.sum(ref ** 2))
+    err = np.sqrt(num / den)
+    psi32 = fc3.DeviceArray((n, n, n), np.float32, ctx)
+    fc3.third_order_device(sysd, pot, calc, psi32=psi32, masses=sysd.masses, ctx=ctx)
+    k3d = fc3.mcc3(psi32, dphi, precision="bf16x3").to_host()
+    psi32.free()
+    errd = _rel(k3, k3d.astype(np.float64))
+    print(f"LJ r_cut 12.5 (n={n}): rel fro err vs FP64 slabs {err:.3e}, vs dense {errd:.3e}",
+          flush=True)
+    assert err <= 1e-5 and errd <= 1e-5
+    dphi.free()
+
+
 def test_fused_rejects_unsupported_size(ctx):
     full = S.fcc(S.ARGON_A, (2, 2, 2), S.ARGON_MASS, 0.0, seed=1)
     sysd = S.SuperCellSystem(full.positions[:31], full.masses[:31], full.box_sizes)   # 3N = 93

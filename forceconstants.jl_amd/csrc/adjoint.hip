@@ -56,67 +56,6 @@ __global__ void k_transpose(const float* __restrict__ in, int n, float* __restri
     }
 }
 
-// Slab-wise transpose of gridDim.z slabs: out[j + n i + n^2 z] = in[i + n j + n^2 z], for
-// n % 4 == 0 in 64 x 64 tiles: each thread loads and stores float4 runs (a wave moves four
-// 256-byte row pieces per access), and the tile turns in LDS.
-__global__ void __launch_bounds__(256) k_transpose_slabs4(const float* __restrict__ in, int n,
-                                                          float* __restrict__ out)
-{
-    __shared__ float t[64][65];
-    const long z = (long)blockIdx.z * n * n;
-    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
-    const int c4 = threadIdx.x & 15, y0 = threadIdx.x >> 4;   // float4 column, row 0..15
-#pragma unroll
-    for (int y = y0; y < 64; y += 16) {
-        const int i = i0 + 4 * c4, j = j0 + y;
-        if (i < n && j < n) {   // n % 4 == 0: the 4 values are all inside
-            const float4 v = *reinterpret_cast<const float4*>(in + z + i + (long)n * j);
-            t[y][4 * c4] = v.x;
-            t[y][4 * c4 + 1] = v.y;
-            t[y][4 * c4 + 2] = v.z;
-            t[y][4 * c4 + 3] = v.w;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int y = y0; y < 64; y += 16) {
-        const int j = j0 + 4 * c4, i = i0 + y;
-        if (i < n && j < n)
-            *reinterpret_cast<float4*>(out + z + j + (long)n * i) =
-                make_float4(t[4 * c4][y], t[4 * c4 + 1][y], t[4 * c4 + 2][y], t[4 * c4 + 3][y]);
-    }
-}
-
-// The same for any n, 32 x 32 tiles of single values.
-__global__ void k_transpose_slabs(const float* __restrict__ in, int n, float* __restrict__ out)
-{
-    __shared__ float t[32][33];
-    const long z = (long)blockIdx.z * n * n;
-    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-    for (int y = threadIdx.y; y < 32; y += 8) {
-        const int i = i0 + threadIdx.x, j = j0 + y;
-        if (i < n && j < n) t[y][threadIdx.x] = in[z + i + (long)n * j];
-    }
-    __syncthreads();
-    for (int y = threadIdx.y; y < 32; y += 8) {
-        const int j = j0 + threadIdx.x, i = i0 + y;
-        if (i < n && j < n) out[z + j + (long)n * i] = t[threadIdx.x][y];
-    }
-}
-
-// All n slabs of an n x n x n tensor transposed (out may not alias in).
-int transpose_slabs(hipStream_t s, const float* in, int n, float* out)
-{
-    if (n % 4 == 0 && ((uintptr_t)in | (uintptr_t)out) % 16 == 0)
-        hipLaunchKernelGGL(k_transpose_slabs4, dim3((n + 63) / 64, (n + 63) / 64, n), dim3(256), 0,
-                           s, in, n, out);
-    else
-        hipLaunchKernelGGL(k_transpose_slabs, dim3((n + 31) / 32, (n + 31) / 32, n), dim3(32, 8), 0,
-                           s, in, n, out);
-    FC3_HIP(hipGetLastError());
-    return FC3_OK;
-}
-
 // out[e] (+)= sum_k T[e + n2 k], k ascending, for e < n2 (accumulate or overwrite)
 __global__ void k_sum_slabs(const float* __restrict__ T, long n2, int cnt, float* __restrict__ out,
                             int accumulate = 1)
@@ -413,9 +352,9 @@ int mcc3_backward(fc3_context* h, const float* psi, const float* phi, const floa
         FC3_HIP(hipGetLastError());
         int zc = 0;
         // H23 = G x2 phi x3 phi: modes31_t of the slab-transposed G (G's first two
-        // indices swapped) writes H23 in its natural layout
-        FC3_TRY(transpose_slabs(s, G, n, T1));
-        FC3_TRY(mcc3_modes31_t(s, T1, phiT, n, T2, rest, rest_bytes));
+        // indices swapped) writes H23 in its natural layout; its mode-3 split reads G in
+        // that order directly
+        FC3_TRY(mcc3_modes31_t(s, G, phiT, n, T2, rest, rest_bytes, true));
         // dphi = Psi_(1) * H23_(1)^T
         if (cv3) {
             FC3_TRY(mcc3_contract(s, 0, psi, T2, n, T1, &zc, rest, rest_bytes));

@@ -238,7 +238,8 @@ int mcc3_run_inplace(hipStream_t s, float* psi, const float* phi, int n, int pre
 // M is n x n column-major, M(k, c) = M[k + n c]; X, Y are n^3 (Y must not alias X).
 //   mode3:     Y[i + n j + n^2 c] = sum_k X[i + n j + n^2 k] M(k, c)
 //   modes31_t: Y[j + n b + n^2 c] = sum_{i,k} X[i + n j + n^2 k] M(i, b) M(k, c)
-//              (the (b, j) slab is stored transposed)
+//              (the (b, j) slab is stored transposed); x_swapped: X holds that tensor
+//              with its first two indices swapped, X[j + n i + n^2 k]
 //   modes12:   Y[b + n c + n^2 l] = sum_{i,j} X[i + n j + n^2 l] M(i, b) M(j, c)
 // They apply when mcc3_adjoint_v3_applies (n % 4 == 0, n >= 16, 16-byte aligned tensors).
 bool mcc3_adjoint_v3_applies(int n, const void* a, const void* b, const void* c);
@@ -246,7 +247,7 @@ size_t mcc3_adjoint_v3_bytes(int n);
 int mcc3_mode3(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
                size_t ws_bytes);
 int mcc3_modes31_t(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
-                   size_t ws_bytes);
+                   size_t ws_bytes, bool x_swapped = false);
 int mcc3_modes12(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
                  size_t ws_bytes);
 // The adjoint's n x n^2 x n contractions on the BF16X3 kernel with split K: partials

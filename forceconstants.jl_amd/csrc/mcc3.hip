@@ -1118,6 +1118,65 @@ __global__ void __launch_bounds__(256) k_split_transpose(const float* __restrict
     dl[1] = l1;
 }
 
+// The mode-3 operand of a tensor T held with its first two indices swapped: rows
+// r = i + n j, j in [j0, j0 + u), of T[i + n j + n^2 k] = G[j + n i + n^2 k], split and
+// K-blocked over k exactly as k_split_transpose on T (rows = u n, r0 = n j0: X[(k / 16)
+// 16 rows + 16 r + k % 16], lo at + rows Kp, zeros for k >= n), without forming T.
+// Block per (32 j, 16 i, K block) on a 1-D grid, j tiles fastest (neighbouring blocks
+// read neighbouring 128-byte runs of the same rows of G), through an LDS tile
+// [16 k][16 i][32 j]; each j's 16 rows go out as 512 contiguous bytes per plane.
+__global__ void __launch_bounds__(256) k_split_transpose_swapped(const float* __restrict__ G,
+                                                                  long n2, int n, int Kp, long j0,
+                                                                  long u,
+                                                                  __bf16* __restrict__ X)
+{
+    __shared__ float t[16][16][33];
+    const long ni = (n + 15) / 16, nj = (u + 31) / 32;
+    long b = blockIdx.x;
+    const long jt = b % nj;
+    b /= nj;
+    const int it = (int)(b % ni);
+    const int kb = (int)(b / ni);
+    const int i0 = 16 * it;
+    const long jl0 = 32 * jt;
+    const int lj = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    // all 32 loads of a thread in flight before the first LDS store
+    float x[32];
+    const bool jin = jl0 + lj < u;
+    const float* g0 = G + (j0 + jl0 + lj) + n2 * (16L * kb);
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const int q = grp + 8 * c, kk = q >> 4, ii = q & 15;
+        const bool in = jin && 16 * kb + kk < n && i0 + ii < n;
+        x[c] = in ? g0[(long)n * (i0 + ii) + n2 * kk] : 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const int q = grp + 8 * c;
+        t[q >> 4][q & 15][lj] = x[c];
+    }
+    __syncthreads();
+    const long rows = u * n;
+#pragma unroll
+    for (int q = threadIdx.x; q < 1024; q += 256) {
+        const int h = q & 1, ii = (q >> 1) & 15, jj = q >> 5;
+        const int i = i0 + ii;
+        const long jl = jl0 + jj;
+        if (i >= n || jl >= u) continue;
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            __bf16 a, c;
+            split_bf16(t[8 * h + v][ii][jj], a, c);
+            hi[v] = a;
+            lo[v] = c;
+        }
+        const long off = (long)kb * 16 * rows + 16 * (i + (long)n * jl) + 8 * h;
+        *reinterpret_cast<bf16x8*>(X + off) = hi;
+        *reinterpret_cast<bf16x8*>(X + rows * Kp + off) = lo;
+    }
+}
+
 // Mode-3 operand in the row order of U's K-blocked layout: Psi rows (i, j), j in [j0, j0 +
 // u) (r0 = n j0), as X rows m = (i / 16) 16 u + 16 (j - j0) + i % 16 for i < Kp (rows
 // i >= n, the K padding of U, are zero), split and K-blocked over k like
@@ -1848,16 +1907,27 @@ long mode3_rows(int n, size_t cap_bytes, long tiles_per_launch)
 // skipped products are all exactly zero, so U is the same bit for bit.
 size_t mask_bytes(long rows, int n) { return Carve::need((size_t)((rows + 255) / 256) * (padk(n) / 16)); }
 
+// swapped: psi holds the tensor with its first two indices swapped (r0, rows multiples
+// of n; no kmask)
 int mode3_block(hipStream_t s, const float* psi, int n, long r0, long rows, const Split3& B,
                 __bf16* X, const OutMap& o, bool split_out, int ncols = 0,
-                unsigned char* kmask = nullptr)
+                unsigned char* kmask = nullptr, bool swapped = false)
 {
     const long n2 = (long)n * n;
     const int Kp = padk(n);
-    if (kmask && (!split_out || Kp / 16 > v3::kMaxKBlocks)) kmask = nullptr;
-    const dim3 gt((unsigned)((Kp / 16) * ((rows + 255) / 256)));
-    hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows, X,
-                       kmask, Kp / 16);
+    if (kmask && (swapped || !split_out || Kp / 16 > v3::kMaxKBlocks)) kmask = nullptr;
+    if (swapped) {
+        if (r0 % n != 0 || rows % n != 0)
+            return set_error(FC3_ERR_ARGUMENT, "mcc3: swapped mode-3 rows must be whole slabs");
+        const long u = rows / n;
+        const dim3 gt((unsigned)((Kp / 16) * ((u + 31) / 32) * ((n + 15) / 16)));
+        hipLaunchKernelGGL(v3::k_split_transpose_swapped, gt, dim3(256), 0, s, psi, n2, n, Kp,
+                           r0 / n, u, X);
+    } else {
+        const dim3 gt((unsigned)((Kp / 16) * ((rows + 255) / 256)));
+        hipLaunchKernelGGL(v3::k_split_transpose, gt, dim3(256), 0, s, psi, n2, n, Kp, r0, rows,
+                           X, kmask, Kp / 16);
+    }
     FC3_HIP(hipGetLastError());
     return gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, o, split_out,
                  rows, ncols > 0 ? ncols : n, Kp, 0, 0, 0, 0, 0, kmask);
@@ -2235,7 +2305,7 @@ int mcc3_mode3(hipStream_t s, const float* X, const float* M, int n, float* Y, v
 }
 
 int mcc3_modes31_t(hipStream_t s, const float* X, const float* M, int n, float* Y, void* ws,
-                   size_t ws_bytes)
+                   size_t ws_bytes, bool x_swapped)
 {
     Carve cv(ws, ws_bytes);
     Split3 B;
@@ -2250,7 +2320,7 @@ int mcc3_modes31_t(hipStream_t s, const float* X, const float* M, int n, float* 
     for (long r0 = 0; r0 < n2; r0 += a.R) {
         const long rows = (n2 - r0) < a.R ? (n2 - r0) : a.R;
         const OutMap oU{U + 16 * (r0 / n), n, 16, 16L * n, 2 * P, P};
-        FC3_TRY(mode3_block(s, X, n, r0, rows, B, Xs, oU, true));
+        FC3_TRY(mode3_block(s, X, n, r0, rows, B, Xs, oU, true, 0, nullptr, x_swapped));
     }
     // mode 1: C(c n + j, b) = Y[j + n b + n^2 c]
     const int nb = a.nb;

@@ -182,11 +182,13 @@ def test_lj_chain_sum_w_k3_squared(ctx):
         assert abs(fd - g[name]) <= 2e-4 * abs(fd)
 
 
-@pytest.mark.parametrize("shape,seed", [((96, 96, 96), 1500), ((7, 5, 3), (1 << 40) + 9)])
+@pytest.mark.parametrize("shape,seed", [((96, 96, 96), 1500), ((7, 5, 3), (1 << 40) + 9),
+                                        ((250, 250, 250), 77), ((251, 251, 250), 78)])
 def test_weighted_square_loss_matches_oracle(ctx, shape, seed):
     """fc3_weighted_square_loss against the oracle's Philox restatement: the cotangent
     2 w K3 bit for bit, L to FP64 summation order (1e-13); a count that is not a
-    multiple of 4 takes the scalar tail."""
+    multiple of 4 takes the scalar tail; counts past 4 x 3 grid strides (6.3M) take the
+    four-step loop, and 251^2 x 250 leaves a scalar tail behind it."""
     k3 = np.asfortranarray(np.random.default_rng(5).standard_normal(shape).astype(np.float32))
     d = fc3.DeviceArray.from_host(k3, ctx=ctx)
     L, g = fc3.weighted_square_loss(d, seed)

@@ -1290,29 +1290,43 @@ __global__ void k_fill_u8(unsigned char* __restrict__ p, long cnt, unsigned char
 
 // ---- gathered mode 1 (n % 16 == 0) ---------------------------------------------------
 // Psi's (i, j) pattern, Psi viewed as an n^2 x n matrix (rows r = i + n j contiguous):
-// pat[r] = 1 when some Psi[r + n^2 k] != 0.  Thread per 4 consecutive rows, k looped: every
-// load of a wave is 1 KB contiguous, and Psi is read once.  Eight loads go out before the
-// first is tested, so that each wave keeps 8 KB in flight.  n^2 % 4 == 0.
+// pat[r] = 1 when some Psi[r + n^2 k] != 0.  Thread per 4 consecutive rows, k looped in
+// K blocks of 16 (n % 16 == 0): every load of a wave is 1 KB contiguous, Psi is read
+// once, and a block's 16 loads go out before the first is tested.  Also jk[jb nkb + kb]
+// = 1 when some Psi[i + n j + n^2 k] != 0 with j in column group jb = j / 16 and k in K
+// block kb (jk zeroed before; nkb = n / 16): one ballot per wave and block, at most two
+// groups per wave (a wave's 256 rows lie within 16 n).
 __global__ void __launch_bounds__(256) k_psi_pattern(const float* __restrict__ psi, long n2, int n,
-                                                     unsigned char* __restrict__ pat)
+                                                     unsigned char* __restrict__ pat,
+                                                     unsigned char* __restrict__ jk, int nkb)
 {
     const long r = 4 * ((long)blockIdx.x * 256 + threadIdx.x);
-    if (r >= n2) return;
+    const bool live = r < n2;   // no early return: the whole wave takes part in the ballots
+    const int lane = threadIdx.x & 63;
     // the bit patterns: a value is nonzero when its bits without the sign are (-0.0f == 0)
     using v2::u32x4;
-    const u32x4* p = reinterpret_cast<const u32x4*>(psi + r);
-    const long step = n2 / 4;
-    u32x4 any = {0u, 0u, 0u, 0u};
-    int k = 0;
-    for (; k + 8 <= n; k += 8) {
-        u32x4 v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(p + step * (k + u));
-#pragma unroll
-        for (int u = 0; u < 8; ++u) any |= v[u];
-    }
-    for (; k < n; ++k) any |= p[step * k];
     const unsigned mag = 0x7fffffffu;
+    const u32x4* p = reinterpret_cast<const u32x4*>(psi + (live ? r : 0));
+    const long step = n2 / 4;
+    const int jb = live ? (int)(r / n / 16) : -1;
+    const int jb_first = __shfl(jb, 0);   // lane 0 is live whenever any lane is
+    u32x4 any = {0u, 0u, 0u, 0u};
+    for (int kb = 0; kb < nkb; ++kb) {
+        u32x4 v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            v[u] = live ? __builtin_nontemporal_load(p + step * (16L * kb + u)) : u32x4{0u, 0u, 0u, 0u};
+        u32x4 blk = v[0];
+#pragma unroll
+        for (int u = 1; u < 16; ++u) blk |= v[u];
+        any |= blk;
+        const bool nz = ((blk.x | blk.y | blk.z | blk.w) & mag) != 0;
+        const unsigned long long m0 = __ballot(nz && jb == jb_first);
+        const unsigned long long m1 = __ballot(nz && jb != jb_first);
+        if (lane == 0 && m0) jk[(long)jb_first * nkb + kb] = 1;
+        if (m1 && lane == __ffsll((long long)m1) - 1) jk[(long)jb * nkb + kb] = 1;
+    }
+    if (!live) return;
     const bool a = (any.x & mag) != 0, b = (any.y & mag) != 0, c = (any.z & mag) != 0,
                d = (any.w & mag) != 0;
     uchar4 o;
@@ -1392,11 +1406,13 @@ __global__ void __launch_bounds__(256) k_gather_phi_list(const float* __restrict
 // (t = 16 tb + t % 16; zero when t >= cnt[jb]) and j = 16 jb + jj, split and K-blocked over
 // k like k_split_transpose_u (rows = 16 G n).  A 256-row tile is one (jb, tb): a tile past
 // the group's rows only records an empty mask (its GEMM tile then writes nothing, and mode 1
-// never reads those rows).  kmask[tile nkb + kb] = any nonzero in the tile's K block kb.
+// never reads those rows); so does a K block that jk marks zero for the whole group.
+// kmask[tile nkb + kb] = any nonzero in the tile's K block kb.
 // Block (K block, tile) on a 1-D grid, tiles fastest; thread (jj, t % 16).
 __global__ void __launch_bounds__(256) k_split_gather(const float* __restrict__ psi, long n2, int n,
                                                       int jb0, const int* __restrict__ list,
                                                       const int* __restrict__ cnt,
+                                                      const unsigned char* __restrict__ jk,
                                                       __bf16* __restrict__ X, long rows,
                                                       unsigned char* __restrict__ kmask, int nkb)
 {
@@ -1406,7 +1422,9 @@ __global__ void __launch_bounds__(256) k_split_gather(const float* __restrict__ 
     const int tpg = n / 16;                        // tiles per group
     const int jb = jb0 + (int)(tile / tpg), tb = (int)(tile % tpg);
     const int L = cnt[jb];
-    if (16 * tb >= L) {                            // uniform over the block
+    // uniform over the block: rows past the group's list, or a K block in which the whole
+    // group is zero (k_psi_pattern's jk): nothing to read
+    if (16 * tb >= L || !jk[(long)jb * nkb + kb]) {
         if (threadIdx.x == 0) kmask[tile * nkb + kb] = 0;
         return;
     }
@@ -2002,7 +2020,7 @@ size_t gathered_bytes(int n)
 {
     const long G = n / 16, Npad = round_up(n, v3::BN);
     return Carve::need((size_t)n * n) + Carve::need(sizeof(int) * (size_t)G * n) +
-           Carve::need(sizeof(int) * (size_t)G) + Carve::need((size_t)G * G) +
+           Carve::need(sizeof(int) * (size_t)G) + 2 * Carve::need((size_t)G * G) +
            Carve::need(2 * sizeof(__bf16) * (size_t)G * n * Npad);
 }
 
@@ -2108,6 +2126,7 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
     int* glist = gath ? cv.take<int>((size_t)NG * n) : nullptr;
     int* gcnt = gath ? cv.take<int>(NG) : nullptr;
     unsigned char* gmask = gath ? cv.take<unsigned char>((size_t)NG * NG) : nullptr;
+    unsigned char* jk = gath ? cv.take<unsigned char>((size_t)NG * NG) : nullptr;
     __bf16* Bg = gath ? cv.take<__bf16>((size_t)2 * NG * n * Npad) : nullptr;
     // unique entries only: every batch's mode-1 and mode-2 tile lists, uploaded once
     const size_t list_cap = k3u ? unique_list_bytes(n, nb) / sizeof(int) : 0;
@@ -2135,8 +2154,10 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
     FC3_TRY(zero_kpad(s, S, n, nb));
     if (gath) {
         // Psi's (i, j) pattern (one read of Psi), the groups' row lists and gathered phi
+        hipLaunchKernelGGL(v3::k_fill_u8, dim3((unsigned)((NG * NG + 255) / 256)), dim3(256), 0, s,
+                           jk, (long)NG * NG, (unsigned char)0);
         hipLaunchKernelGGL(v3::k_psi_pattern, dim3((unsigned)((n2 / 4 + 255) / 256)), dim3(256), 0,
-                           s, psi, n2, n, pat);
+                           s, psi, n2, n, pat, jk, NG);
         hipLaunchKernelGGL(v3::k_group_lists, dim3((unsigned)NG), dim3(256), 0, s, pat, n, glist,
                            gcnt, NG, gmask);
         hipLaunchKernelGGL(v3::k_gather_phi_list,
@@ -2151,8 +2172,8 @@ int mcc3_v3(hipStream_t s, const float* psi, const float* phi, int n, float* k3,
             const int g = std::min(G, NG - jb0);
             const long rows = 16L * g * n;
             hipLaunchKernelGGL(v3::k_split_gather, dim3((unsigned)((Kp / 16) * (rows / 256))),
-                               dim3(256), 0, s, psi, n2, n, jb0, glist, gcnt, X, rows, mask,
-                               Kp / 16);
+                               dim3(256), 0, s, psi, n2, n, jb0, glist, gcnt, jk, X, rows,
+                               mask, Kp / 16);
             FC3_HIP(hipGetLastError());
             const OutMap oU{U + 16L * n * jb0, rows, 0, 16, 2 * P, P};
             FC3_TRY(gemm3(s, FC3_KERNEL_GEMM_MODE3, X, rows * Kp, rows, 0, 16 * rows, B, oU, true,
